@@ -1,0 +1,245 @@
+"""Initial conditions for the hot-path workloads (setup only; never on the timed path).
+
+The generators follow the reference's setup rules so that particle counts and layouts match what
+OpenSPH would build for the same request:
+
+* hexagonal close packing inside a sphere: HexagonalPacking::generate
+  (core/sph/initial/Distribution.cpp:126-201), lattice loop of Box::iterateWithIndices
+  (core/objects/geometry/Box.h:182-195), centring on the domain centre;
+* smoothing length times eta and masses proportional to h^3 normalised to rho0 * V:
+  InitialConditions::setQuantities / getMasses (core/sph/initial/Initial.cpp:293-333);
+* two-body collision placement: CollisionGeometrySetupJob::evaluate
+  (core/run/jobs/ParticleJobs.cpp:532-577), bounding spheres from getBoundingSphere (:499-510);
+* Grady-Kipp flaws, accumulate mode: ScalarGradyKippModel::setFlaws (core/physics/Damage.cpp:17-126),
+  with this module's own seeded generator in place of the reference RNG;
+* minimal density of ContinuityEquation::create (core/sph/equations/EquationTerm.cpp:318-336).
+
+Random inputs (velocity perturbation, flaw draws) come from numpy's PCG64 with an explicit seed; the
+same arrays are fed to every implementation that is compared.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Tuple
+
+import numpy as np
+
+from .state import INFTY, ParticleState, RunConfig, basalt
+
+PI = 3.14159265358979323846264338327950288  # core/math/MathUtils.h:379
+
+
+def sphere_volume(radius: float) -> float:
+    """sphereVolume (core/math/MathUtils.h:393-395)."""
+    return 1.3333333333333333333333 * PI * (radius * radius * radius)
+
+
+def _axis_values(lo: float, hi: float, step: float) -> List[float]:
+    out = []
+    v = lo
+    while v <= hi:
+        out.append(v)
+        v += step
+    return out
+
+
+def hcp_sphere(n_requested: int, radius: float, center=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Hexagonal close packing inside a sphere; returns (N, 4) positions with lane 3 = h (before eta)."""
+    volume = sphere_volume(radius)
+    particle_density = float(n_requested) / volume
+    h = 1.0 / np.cbrt(particle_density)
+    dx = 1.1 * h
+    dy = math.sqrt(3.0) * 0.5 * dx
+    dz = math.sqrt(6.0) / 3.0 * dx
+    c = np.asarray(center, dtype=np.float64)
+    lower = c - radius
+    upper = c + radius
+    lo = lower + 0.5 * np.array([dx, dy, dz])
+    xs = np.array(_axis_values(lo[0], upper[0], dx))
+    ys = np.array(_axis_values(lo[1], upper[1], dy))
+    zs = np.array(_axis_values(lo[2], upper[2], dz))
+    delta_x = 0.5 * dx
+    delta_y = math.sqrt(3.0) / 6.0 * dx
+    K, J, I = np.meshgrid(np.arange(len(zs)), np.arange(len(ys)), np.arange(len(xs)), indexing="ij")
+    X = xs[I].copy()
+    Y = ys[J].copy()
+    Z = zs[K].copy()
+    even_k = (K % 2) == 0
+    shift_x = np.where(even_k, (J % 2) == 1, (J % 2) == 0)
+    X[shift_x] += delta_x
+    Y[~even_k] += delta_y
+    X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
+    ddx, ddy, ddz = X - c[0], Y - c[1], Z - c[2]
+    inside = (ddx * ddx + ddy * ddy + ddz * ddz) <= radius * radius
+    pos = np.stack([X[inside], Y[inside], Z[inside]], axis=1)
+    # centre of mass of the lattice onto the domain centre (sequential sum, as the reference loop)
+    com = np.array([np.cumsum(pos[:, k])[-1] for k in range(3)]) / pos.shape[0]
+    pos = pos + (c - com)
+    out = np.empty((pos.shape[0], 4))
+    out[:, :3] = pos
+    out[:, 3] = h
+    return out
+
+
+def _masses(h: np.ndarray, total_mass: float) -> np.ndarray:
+    m = h * h * h
+    prelim = np.cumsum(m)[-1]
+    return m * (total_mass / prelim)
+
+
+def sphere_body(n_requested: int, radius: float, material: dict, center=(0.0, 0.0, 0.0), eta: float = 1.3):
+    """Positions (N,4) with h *= eta and masses for one monolithic spherical body."""
+    r = hcp_sphere(n_requested, radius, center)
+    r[:, 3] *= eta
+    m = _masses(r[:, 3], sphere_volume(radius) * material["rho0"])
+    return r, m
+
+
+def young_modulus(material: dict) -> float:
+    """Young modulus set by ScalarGradyKippModel::setFlaws (core/physics/Damage.cpp:43-48)."""
+    mu, A = material["shear_modulus"], material["A"]
+    return mu * 9.0 * A / (3.0 * A + mu)
+
+
+def grady_kipp_flaws(r: np.ndarray, m: np.ndarray, rho: np.ndarray, material: dict, rng: np.random.Generator,
+                     k_weibull: float = 4.0e35, m_weibull: float = 9.0, rayleigh: float = 0.4,
+                     kernel_radius: float = 2.0):
+    """Flaw activation thresholds, Weibull accumulate mode (core/physics/Damage.cpp:17-126).
+
+    Returns (eps_min, m_zero, growth, n_flaws)."""
+    n = r.shape[0]
+    mu, A, rho0 = material["shear_modulus"], material["A"], material["rho0"]
+    cg = rayleigh * math.sqrt((A + 4.0 / 3.0 * mu) / rho0)
+    growth = cg / (kernel_radius * r[:, 3])
+    V = np.cumsum(m / rho)[-1]
+    denom = 1.0 / (math.pow(k_weibull, 1.0 / m_weibull) * math.pow(V, 1.0 / m_weibull))
+    # draw particle indices until every particle holds at least one flaw
+    chunks = []
+    covered = np.zeros(n, dtype=bool)
+    n_cov = 0
+    total = 0
+    while n_cov < n:
+        draw = (rng.random(max(4 * n, 1 << 16)) * n).astype(np.int64)
+        first_new = []
+        chunks.append(draw)
+        for start in range(0, draw.size, 1 << 20):
+            part = draw[start:start + (1 << 20)]
+            uniq, first = np.unique(part, return_index=True)
+            newmask = ~covered[uniq]
+            covered[uniq[newmask]] = True
+            n_cov += int(newmask.sum())
+            if n_cov >= n:
+                # the loop stops right after the draw that flaws the last particle
+                last_pos = start + int(first[newmask].max())
+                chunks[-1] = draw[: last_pos + 1]
+                break
+        total += chunks[-1].size
+    idx = np.concatenate(chunks)
+    K = idx.size
+    order = np.argsort(idx, kind="stable")
+    sidx = idx[order]
+    bounds = np.flatnonzero(np.diff(sidx)) + 1
+    starts = np.concatenate([[0], bounds])
+    ends = np.concatenate([bounds, [K]]) - 1
+    part = sidx[starts]
+    first_p = order[starts] + 1  # flaw number p is 1-based
+    last_p = order[ends] + 1
+    n_flaws = np.zeros(n, dtype=np.uint32)
+    n_flaws[part] = (ends - starts + 1).astype(np.uint32)
+    eps_min = np.zeros(n)
+    eps_max = np.zeros(n)
+    eps_min[part] = denom * np.power(first_p.astype(np.float64), 1.0 / m_weibull)
+    eps_max[part] = denom * np.power(last_p.astype(np.float64), 1.0 / m_weibull)
+    m_zero = np.ones(n)
+    many = n_flaws > 1
+    m_zero[many] = np.log(n_flaws[many].astype(np.float64)) / np.log(eps_max[many] / eps_min[many])
+    return eps_min, m_zero, growth, n_flaws
+
+
+def kernel_w0() -> float:
+    """Central value of the M4 LUT kernel, LutKernel::valueImpl(0) = CubicSpline<3>::valueImpl(0)."""
+    q = 0.0
+    return (1.0 / PI) * (0.25 * (2.0 - q) ** 3 - (1.0 - q) ** 3)
+
+
+def density_lower_bound(r: np.ndarray, m: np.ndarray, material: dict) -> float:
+    """ContinuityEquation::create: max(min_i m_i w0 / h_i^3, DENSITY_RANGE.lower)."""
+    w0 = kernel_w0()
+    h = r[:, 3]
+    limit = float(np.min(m * w0 / (h * h * h)))
+    return max(limit, material["rho_lo"])
+
+
+def _fill_body(st: ParticleState, sl: slice, r, m, flag: int, mat: int, material: dict):
+    st["r"][sl] = r
+    st["m"][sl] = m
+    st["flag"][sl] = flag
+    st["mat"][sl] = mat
+    st["rho"][sl] = material["rho0"]
+    st["u"][sl] = 0.0
+    st["reduce"][sl] = 1.0
+
+
+def fluid_sphere(n_requested: int = 10000, radius: float = 1000.0, sigma_v: float = 1.0, seed: int = 1,
+                 material: dict | None = None) -> Tuple[ParticleState, RunConfig]:
+    """Config: one basalt sphere, PressureForce + StandardAV + ContinuityEquation, ConstSmoothingLength,
+    Courant criterion only; zero internal energy and a seeded Gaussian velocity perturbation."""
+    mat = basalt(**(material or {}))
+    r, m = sphere_body(n_requested, radius, mat)
+    st = ParticleState(r.shape[0], [mat])
+    _fill_body(st, slice(None), r, m, 0, 0, mat)
+    rng = np.random.default_rng(seed)
+    st["v"][:, :3] = rng.normal(0.0, sigma_v, size=(st.n, 3))
+    lo = density_lower_bound(st["r"], st["m"], mat)
+    st.materials[0]["rho_lo"] = lo
+    cfg = RunConfig(solid_stress=0, adaptive_h=0, has_reduce=0, has_damage=0, criteria=1)
+    return st, cfg
+
+
+def impact(n_target: int = 950_000, n_projectile: int = 50_000, r_target: float = 50e3, r_projectile: float = 5e3,
+           speed: float = 5e3, angle_deg: float = 45.0, offset_h: float = 4.0, seed: int = 1234,
+           flaws: bool = True) -> Tuple[ParticleState, RunConfig]:
+    """Config: two-body solid impact, SolidStressForce + StandardAV + ContinuityEquation, Hooke + von Mises +
+    scalar Grady-Kipp damage, AdaptiveSmoothingLength, all time-step criteria."""
+    mat = basalt(yielding=1, damage=1)
+    mat["young_modulus"] = young_modulus(mat)
+    rt, mt = sphere_body(n_target, r_target, mat)
+    rp, mp = sphere_body(n_projectile, r_projectile, mat)
+
+    def bounding(r, m):
+        # getCenterOfMass then the largest distance (ParticleJobs.cpp:499-510)
+        c = np.array([np.cumsum(m * r[:, k])[-1] for k in range(3)]) / np.cumsum(m)[-1]
+        d = np.sqrt(np.sum((r[:, :3] - c) ** 2, axis=1))
+        return c, float(d.max())
+
+    ct, rad_t = bounding(rt, mt)
+    cp, rad_p = bounding(rp, mp)
+    rt[:, :3] -= ct
+    dist = rad_t + rad_p
+    h = rt[0, 3]
+    phi = angle_deg * (PI / 180.0)
+    x = dist * math.cos(phi) + offset_h * h
+    y = dist * math.sin(phi)
+    rp[:, :3] += -cp + np.array([x, y, 0.0])
+
+    nt, npj = rt.shape[0], rp.shape[0]
+    st = ParticleState(nt + npj, [mat, dict(mat)])
+    _fill_body(st, slice(0, nt), rt, mt, 0, 0, mat)
+    _fill_body(st, slice(nt, nt + npj), rp, mp, 1, 1, mat)
+    st["v"][nt:, 0] = -speed
+    lo = density_lower_bound(st["r"], st["m"], mat)  # over the whole storage, as solver.create runs on it
+    for mm in st.materials:
+        mm["rho_lo"] = lo
+    rng = np.random.default_rng(seed)
+    for sl in (slice(0, nt), slice(nt, nt + npj)):
+        if flaws:
+            e, mz, g, nf = grady_kipp_flaws(st["r"][sl], st["m"][sl], st["rho"][sl], mat, rng)
+        else:
+            e, mz, g, nf = (np.full(sl.stop - sl.start, 1.0), np.ones(sl.stop - sl.start),
+                            np.zeros(sl.stop - sl.start), np.ones(sl.stop - sl.start, np.uint32))
+        st["eps_min"][sl] = e
+        st["m_zero"][sl] = mz
+        st["growth"][sl] = g
+        st["n_flaws"][sl] = nf
+    cfg = RunConfig(solid_stress=1, adaptive_h=1, has_reduce=1, has_damage=1, criteria=15)
+    return st, cfg

@@ -1,0 +1,255 @@
+"""TEST INFRASTRUCTURE -- ctypes bridge to the CPU checkers built by oracle/Makefile.
+
+* `Oracle`  wraps liboracle.so, the plain-C restatement of the hot path (osph_oracle.c).
+* `RefRun`  wraps _ref/libosph_ref.so, the real OpenSPH core plus ref_driver.cpp (only where it was built).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libosph_ref.so")
+
+# plane indices, in the order of the enums in osph_oracle.h
+_DPLANES = ["X", "Y", "Z", "H", "VX", "VY", "VZ", "VH", "AX", "AY", "AZ", "RHO", "DRHO", "U", "DU",
+            "SXX", "SYY", "SXY", "SXZ", "SYZ", "DSXX", "DSYY", "DSXY", "DSXZ", "DSYZ", "D", "DD",
+            "M", "P", "CS", "RED", "DIVV", "GXX", "GYY", "GZZ", "GXY", "GXZ", "GYZ",
+            "EPSMIN", "MZERO", "GROWTH",
+            "PAX", "PAY", "PAZ", "PDRHO", "PDU", "PDSXX", "PDSYY", "PDSXY", "PDSXZ", "PDSYZ", "PDD"]
+D = {k: i for i, k in enumerate(_DPLANES)}
+NP = len(_DPLANES)
+_IPLANES = ["FLAG", "MAT", "NCNT", "NFLAWS"]
+IP = {k: i for i, k in enumerate(_IPLANES)}
+NI = len(_IPLANES)
+
+# ParticleState field -> list of planes
+_MAP = {
+    "r": ["X", "Y", "Z", "H"], "v": ["VX", "VY", "VZ", "VH"], "dv": ["AX", "AY", "AZ", None],
+    "rho": ["RHO"], "drho": ["DRHO"], "u": ["U"], "du": ["DU"],
+    "s": ["SXX", "SYY", "SXY", "SXZ", "SYZ"], "ds": ["DSXX", "DSYY", "DSXY", "DSXZ", "DSYZ"],
+    "damage": ["D"], "ddamage": ["DD"], "m": ["M"], "p": ["P"], "cs": ["CS"], "reduce": ["RED"],
+    "divv": ["DIVV"], "gradv": ["GXX", "GYY", "GZZ", "GXY", "GXZ", "GYZ"],
+    "eps_min": ["EPSMIN"], "m_zero": ["MZERO"], "growth": ["GROWTH"],
+}
+_IMAP = {"flag": "FLAG", "mat": "MAT", "ncnt": "NCNT", "n_flaws": "NFLAWS"}
+
+
+class Material(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("rho0", "A", "B", "a", "b", "alpha", "beta", "u0", "uiv", "ucv")] + [
+        ("yielding", C.c_int), ("damage", C.c_int)] + [
+        (k, C.c_double) for k in ("shear_modulus", "elasticity_limit", "melt_energy", "young_modulus",
+                                  "rho_lo", "rho_hi", "u_lo", "u_hi", "d_lo", "d_hi",
+                                  "rho_min", "u_min", "s_min", "d_min")]
+
+
+class Config(C.Structure):
+    _fields_ = [("solid_stress", C.c_int), ("adaptive_h", C.c_int), ("sum_only_undamaged", C.c_int),
+                ("has_reduce", C.c_int), ("has_damage", C.c_int),
+                ("av_alpha", C.c_double), ("av_beta", C.c_double), ("h_lo", C.c_double), ("h_hi", C.c_double),
+                ("criteria", C.c_int), ("courant", C.c_double), ("deriv_factor", C.c_double),
+                ("div_factor", C.c_double), ("max_dt", C.c_double),
+                ("n_mat", C.c_int), ("mat", C.POINTER(Material))]
+
+
+def make_config(cfg, materials):
+    mats = (Material * len(materials))()
+    for i, m in enumerate(materials):
+        for name, _ in Material._fields_:
+            setattr(mats[i], name, type(getattr(mats[i], name))(m[name]))
+    c = Config()
+    for name, _ in Config._fields_:
+        if name in ("n_mat", "mat"):
+            continue
+        setattr(c, name, getattr(cfg, name))
+    c.n_mat = len(materials)
+    c.mat = C.cast(mats, C.POINTER(Material))
+    c._keep = mats
+    return c
+
+
+def to_planes(st):
+    P = np.zeros((NP, st.n))
+    I = np.zeros((NI, st.n), dtype=np.uint32)
+    for f, planes in _MAP.items():
+        a = st[f]
+        for k, pl in enumerate(planes):
+            if pl is None:
+                continue
+            P[D[pl]] = a if a.ndim == 1 else a[:, k]
+    for f, pl in _IMAP.items():
+        I[IP[pl]] = st[f]
+    return P, I
+
+
+def from_planes(st, P, I):
+    for f, planes in _MAP.items():
+        a = st.a[f]
+        for k, pl in enumerate(planes):
+            if pl is None:
+                continue
+            if a.ndim == 1:
+                a[:] = P[D[pl]]
+            else:
+                a[:, k] = P[D[pl]]
+    for f, pl in _IMAP.items():
+        st.a[f][:] = I[IP[pl]]
+
+
+def _ptr(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run make -C oracle)")
+        L = self.lib = C.CDLL(path)
+        L.orc_lut.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_tillotson.argtypes = [C.POINTER(Material), C.c_double, C.c_double, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+        L.orc_eigenvalues.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_find_pairs.restype = C.c_size_t
+        L.orc_find_pairs.argtypes = [C.c_size_t] + [C.POINTER(C.c_double)] * 4 + [C.POINTER(C.c_uint32)] * 2 + [
+            C.c_size_t]
+        L.orc_integrate.argtypes = [C.POINTER(Config), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t]
+        L.orc_zero_derivatives.argtypes = [C.POINTER(C.c_double), C.c_size_t]
+        L.orc_step_pc.restype = C.c_double
+        L.orc_step_pc.argtypes = [C.POINTER(Config), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t,
+                                  C.c_double, C.POINTER(C.c_int)]
+        L.orc_timestep.restype = C.c_double
+        L.orc_timestep.argtypes = [C.POINTER(Config), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t,
+                                   C.POINTER(C.c_int)]
+
+    def lut(self):
+        v = np.zeros(40001)
+        g = np.zeros(40001)
+        self.lib.orc_lut(_ptr(v), _ptr(g))
+        return v, g
+
+    def tillotson(self, material, rho, u):
+        m = make_config(type("c", (), {k: 0 for k, _ in Config._fields_})(), [material]).mat
+        p, cs = C.c_double(), C.c_double()
+        self.lib.orc_tillotson(m, rho, u, C.byref(p), C.byref(cs))
+        return p.value, cs.value
+
+    def eigenvalues(self, t6):
+        t = np.ascontiguousarray(t6, dtype=np.float64)
+        out = np.zeros(3)
+        self.lib.orc_eigenvalues(_ptr(t), _ptr(out))
+        return out
+
+    def pairs(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        n = r.shape[0]
+        cols = [np.ascontiguousarray(r[:, k]) for k in range(4)]
+        cap = max(64 * n, 1024)
+        while True:
+            pi = np.zeros(cap, dtype=np.uint32)
+            pj = np.zeros(cap, dtype=np.uint32)
+            cnt = self.lib.orc_find_pairs(n, *[_ptr(c) for c in cols], _ptr(pi, C.c_uint32), _ptr(pj, C.c_uint32),
+                                          cap)
+            if cnt <= cap:
+                return pi[:cnt], pj[:cnt]
+            cap = cnt
+
+    def integrate(self, st, cfg):
+        P, I = to_planes(st)
+        c = make_config(cfg, st.materials)
+        self.lib.orc_zero_derivatives(_ptr(P), st.n)
+        self.lib.orc_integrate(C.byref(c), _ptr(P), _ptr(I, C.c_uint32), st.n)
+        out = st.copy()
+        from_planes(out, P, I)
+        return out
+
+    def run_pc(self, st, cfg, steps, dt0=None):
+        """Runs `steps` PredictorCorrector steps; returns (state, list of (dt_used, next_dt, criterion))."""
+        P, I = to_planes(st)
+        c = make_config(cfg, st.materials)
+        dt = cfg.initial_dt if dt0 is None else dt0
+        log = []
+        crit = C.c_int(0)
+        for _ in range(steps):
+            nxt = self.lib.orc_step_pc(C.byref(c), _ptr(P), _ptr(I, C.c_uint32), st.n, dt, C.byref(crit))
+            log.append((dt, nxt, crit.value))
+            dt = nxt
+        out = st.copy()
+        from_planes(out, P, I)
+        return out, log
+
+
+class RefRun:
+    """The real OpenSPH SymmetricSolver<3> + PredictorCorrector on a copy of the given state."""
+
+    _lib = None
+
+    @classmethod
+    def available(cls):
+        return os.path.exists(REF_SO)
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            L = C.CDLL(REF_SO)
+            L.ref_create.restype = C.c_void_p
+            L.ref_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t,
+                                     C.c_double, C.c_int]
+            L.ref_destroy.argtypes = [C.c_void_p]
+            L.ref_integrate.argtypes = [C.c_void_p]
+            L.ref_step.restype = C.c_double
+            L.ref_step.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+            L.ref_get_dt.restype = C.c_double
+            L.ref_get_dt.argtypes = [C.c_void_p]
+            L.ref_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t]
+            L.ref_pairs.restype = C.c_size_t
+            L.ref_pairs.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_size_t]
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, st, cfg, nthreads=0, dt0=None):
+        L = self.lib()
+        self.st = st.copy()
+        self.P, self.I = to_planes(st)
+        self.c = make_config(cfg, st.materials)
+        dt = cfg.initial_dt if dt0 is None else dt0
+        self.h = L.ref_create(C.byref(self.c), _ptr(self.P), _ptr(self.I, C.c_uint32), st.n, dt, nthreads)
+
+    def close(self):
+        if self.h:
+            self.lib().ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def integrate(self):
+        self.lib().ref_integrate(self.h)
+        return self.read()
+
+    def step(self):
+        crit = C.c_int(0)
+        dt = self.lib().ref_get_dt(self.h)
+        nxt = self.lib().ref_step(self.h, C.byref(crit))
+        return dt, nxt, crit.value
+
+    def read(self):
+        self.lib().ref_read(self.h, _ptr(self.P), _ptr(self.I, C.c_uint32), self.st.n)
+        out = self.st.copy()
+        from_planes(out, self.P, self.I)
+        return out
+
+    def pairs(self):
+        n = self.st.n
+        cap = max(64 * n, 1024)
+        while True:
+            pi = np.zeros(cap, dtype=np.uint32)
+            pj = np.zeros(cap, dtype=np.uint32)
+            cnt = self.lib().ref_pairs(self.h, _ptr(pi, C.c_uint32), _ptr(pj, C.c_uint32), cap)
+            if cnt <= cap:
+                return pi[:cnt], pj[:cnt]
+            cap = cnt

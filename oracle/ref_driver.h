@@ -1,0 +1,44 @@
+/*
+ * TEST INFRASTRUCTURE -- not part of the product.
+ *
+ * extern "C" driver over the real OpenSPH core (compiled from its own sources into
+ * oracle/_ref/libosph_ref.so by oracle/Makefile).  It builds a Storage from the oracle's plane layout
+ * (osph_oracle.h), runs the reference SymmetricSolver<3> / PredictorCorrector / MultiCriterion on it,
+ * and reads the state back.  Used to pin the oracle and as the CPU baseline ("reference" kind).
+ *
+ * Particles must be grouped by body: each maximal run of equal FLAG values becomes one body with its
+ * own material instance (as InitialConditions::addMonolithicBody creates them).
+ */
+#ifndef OSPH_REF_DRIVER_H
+#define OSPH_REF_DRIVER_H
+
+#include "osph_oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* nthreads == 0: SequentialScheduler (bit-stable, used for parity); > 0: ThreadPool of that size. */
+void* ref_create(const orc_config* cfg, const double* planes, const uint32_t* iplanes, size_t n, double dt0,
+                 int nthreads);
+void ref_destroy(void* h);
+
+/* Storage::zeroHighestDerivatives + SymmetricSolver<3>::integrate on the current state. */
+void ref_integrate(void* h);
+
+/* One ITimeStepping::step of PredictorCorrector; returns the next time step, *crit = CriterionId. */
+double ref_step(void* h, int* crit);
+
+double ref_get_dt(void* h);
+
+/* Copies the reference Storage back into the plane layout. */
+void ref_read(void* h, double* planes, uint32_t* iplanes, size_t n);
+
+/* KdTree<KdNode> (leaf 25) findLowerRank(i, 2 h_i) filtered by the SymmetricSolver cutoff; pairs are
+ * returned as (min, max) sorted.  Returns the count (writes at most cap). */
+size_t ref_pairs(void* h, uint32_t* pi, uint32_t* pj, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
