@@ -1,0 +1,148 @@
+/*
+ * osph_gpu.h -- C ABI of the MI355X (gfx950) implementation of OpenSPH's per-timestep hot path.
+ *
+ * The library keeps the particle state resident in HBM between calls and exposes it at the
+ * granularity of the reference's own interfaces:
+ *
+ *   osph_set_quantity / osph_get_quantity  <- Storage value/dt/d2t buffers
+ *                                             (core/quantities/Storage.h:242-575, Quantity.h:58-160)
+ *   osph_build_neighbors / osph_get_pairs  <- ISymmetricFinder::build + findLowerRank as used by
+ *                                             SymmetricSolver<3>::loop
+ *                                             (core/objects/finders/NeighborFinder.h:87-133,
+ *                                              core/sph/solvers/SymmetricSolver.cpp:109-141)
+ *   osph_integrate                         <- ISolver::integrate, SymmetricSolver<3>::integrate
+ *                                             (core/timestepping/ISolver.h:38,
+ *                                              core/sph/solvers/SymmetricSolver.cpp:58-95)
+ *   osph_step_pc                           <- ITimeStepping::step of PredictorCorrector
+ *                                             (core/timestepping/TimeStepping.cpp:34-75,286-346)
+ *   osph_timestep                          <- MultiCriterion::compute
+ *                                             (core/timestepping/TimeStepCriterion.cpp:389-419)
+ *
+ * Conventions: every function returns 0 on success and a nonzero code on failure; the message is
+ * available from osph_last_error().  No exceptions cross the ABI.  A context owns one HIP stream and
+ * all device memory; host buffers are only borrowed for the duration of a call.  A context is not
+ * re-entrant and must be used from one thread at a time (as ISolver, core/timestepping/ISolver.h:28-31).
+ */
+#ifndef OSPH_GPU_H
+#define OSPH_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct osph_ctx osph_ctx;
+
+enum {
+    OSPH_OK = 0,
+    OSPH_ERR_ARG = 1,
+    OSPH_ERR_HIP = 2,
+    OSPH_ERR_STATE = 3,
+    OSPH_ERR_UNSUPPORTED = 4
+};
+
+/* Per-material parameters (BodySettings of one body, core/system/Settings.cpp:793-943). */
+typedef struct {
+    /* Tillotson EoS (TillotsonEos, core/physics/Eos.cpp:184-238) */
+    double rho0, A, B, a, b, alpha, beta, u0, uiv, ucv;
+    int yielding;          /* 0 = NONE (EosMaterial), 1 = VON_MISES (SolidMaterial) */
+    int damage;            /* 0 = NONE, 1 = SCALAR_GRADY_KIPP */
+    double shear_modulus, elasticity_limit, melt_energy, young_modulus;
+    /* IMaterial::range (clamping) and IMaterial::minimal (derivative criterion) */
+    double rho_lo, rho_hi, u_lo, u_hi, d_lo, d_hi;
+    double rho_min, u_min, s_min, d_min;
+} osph_material;
+
+/* Run settings read by the hot path (RunSettings, core/system/Settings.cpp:486-715). */
+typedef struct {
+    int solid_stress;        /* ForceEnum::SOLID_STRESS: SolidStressForce in the equation set */
+    int adaptive_h;          /* SmoothingLengthEnum::CONTINUITY_EQUATION: AdaptiveSmoothingLength */
+    int sum_only_undamaged;  /* SPH_SUM_ONLY_UNDAMAGED */
+    int has_reduce;          /* the storage holds STRESS_REDUCING (some material has a rheology) */
+    int has_damage;          /* the storage holds DAMAGE */
+    double av_alpha, av_beta;
+    double h_lo, h_hi;       /* SPH_SMOOTHING_LENGTH_RANGE */
+    int criteria;            /* bit0 COURANT, bit1 DERIVATIVES, bit2 ACCELERATION, bit3 DIVERGENCE */
+    double courant, deriv_factor, div_factor, max_dt;
+    int precision;           /* OSPH_PRECISION_FP64 or OSPH_PRECISION_MIXED */
+} osph_config;
+
+enum { OSPH_PRECISION_FP64 = 0, OSPH_PRECISION_MIXED = 1 };
+
+/* Quantity ids (a subset of QuantityId, core/quantities/QuantityIds.h) */
+enum {
+    OSPH_Q_POSITION = 0,          /* Vector: x, y, z, h; orders 0, 1, 2 */
+    OSPH_Q_MASS,                  /* double */
+    OSPH_Q_DENSITY,               /* double; orders 0, 1 */
+    OSPH_Q_ENERGY,                /* double; orders 0, 1 */
+    OSPH_Q_DEVIATORIC_STRESS,     /* TracelessTensor; orders 0, 1 */
+    OSPH_Q_DAMAGE,                /* double; orders 0, 1 */
+    OSPH_Q_PRESSURE,              /* double */
+    OSPH_Q_SOUND_SPEED,           /* double */
+    OSPH_Q_STRESS_REDUCING,       /* double */
+    OSPH_Q_VELOCITY_DIVERGENCE,   /* double */
+    OSPH_Q_VELOCITY_GRADIENT,     /* SymmetricTensor */
+    OSPH_Q_NEIGHBOR_CNT,          /* uint32 */
+    OSPH_Q_FLAG,                  /* uint32 (body index) */
+    OSPH_Q_MATERIAL_ID,           /* uint32 (index into the material table) */
+    OSPH_Q_EPS_MIN,               /* double */
+    OSPH_Q_M_ZERO,                /* double */
+    OSPH_Q_EXPLICIT_GROWTH,       /* double */
+    OSPH_Q_N_FLAWS,               /* uint32 */
+    OSPH_Q_COUNT
+};
+
+/* Host element layouts are those of the reference types:
+ *   Vector          4 doubles x, y, z, h                  natural stride 32 B (Vector.h:21-26)
+ *   TracelessTensor xx, yy, xy, xz at 0..24, yz at 32     natural stride 64 B (TracelessTensor.h:36-45)
+ *   SymmetricTensor xx, yy, zz at 0..16, xy, xz, yz at 32..48, natural stride 64 B (SymmetricTensor.h:13-14)
+ *   double 8 B, uint32 4 B.
+ * A stride of 0 selects the natural stride. */
+
+/* ---- lifetime ------------------------------------------------------------------------------------- */
+int osph_create(osph_ctx** ctx, const osph_config* cfg, const osph_material* materials, int n_materials);
+void osph_destroy(osph_ctx* ctx);
+const char* osph_last_error(const osph_ctx* ctx);
+int osph_device_info(char* name, size_t cap, int* compute_units);
+
+/* ---- state (Storage) -------------------------------------------------------------------------------- */
+int osph_resize(osph_ctx* ctx, size_t n);
+int osph_set_quantity(osph_ctx* ctx, int quantity, int order, const void* host, size_t stride_bytes);
+int osph_get_quantity(osph_ctx* ctx, int quantity, int order, void* host, size_t stride_bytes);
+/* Storage::zeroHighestDerivatives (core/quantities/Storage.cpp:594-602) */
+int osph_zero_derivatives(osph_ctx* ctx);
+
+/* ---- neighbour search (ISymmetricFinder) ----------------------------------------------------------- */
+/* Builds the cell grid and the neighbour lists for the current positions. */
+int osph_build_neighbors(osph_ctx* ctx);
+/* Unordered pairs {i<j} with |r_i - r_j|^2 < (2 hbar)^2 in storage indices, sorted by (i, j).
+ * Writes min(count, cap) pairs and the total into *count. */
+int osph_get_pairs(osph_ctx* ctx, uint32_t* pi, uint32_t* pj, size_t cap, size_t* count);
+
+/* ---- solver / integrator ---------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t neighbors_min, neighbors_max;
+    double neighbors_mean;   /* StatisticsId::NEIGHBOR_COUNT (SymmetricSolver.cpp:172-179) */
+} osph_stats;
+
+int osph_integrate(osph_ctx* ctx, osph_stats* stats);
+int osph_step_pc(osph_ctx* ctx, double dt, double* next_dt, int* criterion);
+int osph_timestep(osph_ctx* ctx, double* dt, int* criterion);
+int osph_sync(osph_ctx* ctx);
+
+/* ---- kernel table and profiling --------------------------------------------------------------------- */
+/* The 40001-entry M4 look-up tables the device uses (LutKernel<3>, core/sph/kernel/Kernel.h:83-101). */
+int osph_kernel_lut(double* values, double* grads);
+/* When enabled, every kernel launch is bracketed by HIP events on the context's stream and its
+ * duration accumulated per kernel name. */
+int osph_profile_enable(osph_ctx* ctx, int enable);
+/* Copies up to cap (name, total ms, launches) records; returns the count in *count. */
+int osph_profile_read(osph_ctx* ctx, char* names, size_t name_cap, double* ms, uint64_t* launches, int cap,
+                      int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

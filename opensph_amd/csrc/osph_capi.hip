@@ -1,0 +1,742 @@
+// C ABI of the MI355X OpenSPH hot path (declared in include/osph_gpu.h): context, HBM-resident state,
+// neighbour build orchestration, SymmetricSolver<3>::integrate and PredictorCorrector::step.
+#include "osph_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace osph;
+
+namespace {
+
+constexpr double PI_H = 3.14159265358979323846264338327950288;
+
+struct ProfEvent {
+    const char* name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct osph_ctx {
+    osph_config cfg{};
+    Params prm{};
+    std::vector<osph_material> mats;
+    osph_material* mats_dev = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    size_t n = 0, cap = 0;
+    DevState s{};
+    double* dblock = nullptr;
+    uint32_t* ublock = nullptr;
+
+    // neighbour structures (all sized for cap particles)
+    uint32_t *keys = nullptr, *keys_sorted = nullptr, *iota = nullptr, *order = nullptr;
+    void* sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    uint32_t *cstart = nullptr, *cend = nullptr;
+    size_t cell_cap = 0;
+    PairRec* rec = nullptr;
+    uint32_t* group = nullptr;
+    uint32_t* list = nullptr;
+    uint32_t* count = nullptr;
+    uint32_t kmax = 96;
+    size_t list_cap = 0;
+    unsigned long long* small = nullptr;     // [0..7] bounds, [8..11] criteria, [12] overflow, [16..47] classes
+    unsigned long long* small_host = nullptr;  // pinned mirror
+    double* lut = nullptr;
+    Grid grid{};
+    bool neighbors_valid = false;
+
+    unsigned char* staging = nullptr;
+    size_t staging_cap = 0;
+
+    bool prof = false;
+    std::vector<ProfEvent> events;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, uint64_t>> prof_acc;
+
+    std::string err;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+};
+
+namespace {
+
+#define HIPCHK(ctx, call)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            return (ctx)->fail(OSPH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+        }                                                                                              \
+    } while (0)
+
+hipEvent_t take_event(osph_ctx* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Runs a launcher; with profiling on, brackets it with events on the context's stream.
+template <typename F>
+void timed(osph_ctx* c, const char* name, F&& f) {
+    if (!c->prof) {
+        f();
+        return;
+    }
+    ProfEvent ev{ name, take_event(c), take_event(c) };
+    if (ev.start) hipEventRecord(ev.start, c->stream);
+    f();
+    if (ev.stop) hipEventRecord(ev.stop, c->stream);
+    c->events.push_back(ev);
+}
+
+void resolve_events(osph_ctx* c) {
+    for (ProfEvent& e : c->events) {
+        float ms = 0.f;
+        if (e.start && e.stop && hipEventSynchronize(e.stop) == hipSuccess &&
+            hipEventElapsedTime(&ms, e.start, e.stop) == hipSuccess) {
+            auto& acc = c->prof_acc[e.name];
+            acc.first += ms;
+            acc.second += 1;
+        }
+        if (e.start) c->event_pool.push_back(e.start);
+        if (e.stop) c->event_pool.push_back(e.stop);
+    }
+    c->events.clear();
+}
+
+Launch L(osph_ctx* c) { return Launch{ c->stream }; }
+
+void free_state(osph_ctx* c) {
+    hipFree(c->dblock);
+    hipFree(c->ublock);
+    hipFree(c->keys);
+    hipFree(c->keys_sorted);
+    hipFree(c->iota);
+    hipFree(c->order);
+    hipFree(c->sort_temp);
+    hipFree(c->rec);
+    hipFree(c->group);
+    hipFree(c->count);
+    c->dblock = nullptr;
+    c->ublock = nullptr;
+    c->keys = c->keys_sorted = c->iota = c->order = nullptr;
+    c->sort_temp = nullptr;
+    c->rec = nullptr;
+    c->group = c->count = nullptr;
+    c->cap = 0;
+}
+
+int ensure_staging(osph_ctx* c, size_t bytes) {
+    if (bytes <= c->staging_cap) return OSPH_OK;
+    hipFree(c->staging);
+    c->staging = nullptr;
+    c->staging_cap = 0;
+    HIPCHK(c, hipMalloc(&c->staging, bytes));
+    c->staging_cap = bytes;
+    return OSPH_OK;
+}
+
+int ensure_list(osph_ctx* c, size_t entries) {
+    if (entries <= c->list_cap) return OSPH_OK;
+    hipFree(c->list);
+    c->list = nullptr;
+    c->list_cap = 0;
+    HIPCHK(c, hipMalloc(&c->list, entries * sizeof(uint32_t)));
+    c->list_cap = entries;
+    return OSPH_OK;
+}
+
+int ensure_cells(osph_ctx* c, size_t cells) {
+    if (cells <= c->cell_cap) return OSPH_OK;
+    hipFree(c->cstart);
+    hipFree(c->cend);
+    c->cstart = c->cend = nullptr;
+    c->cell_cap = 0;
+    HIPCHK(c, hipMalloc(&c->cstart, cells * sizeof(uint32_t)));
+    HIPCHK(c, hipMalloc(&c->cend, cells * sizeof(uint32_t)));
+    c->cell_cap = cells;
+    return OSPH_OK;
+}
+
+Params make_params(const osph_config& cfg) {
+    Params p{};
+    p.solid_stress = cfg.solid_stress;
+    p.adaptive_h = cfg.adaptive_h;
+    p.sum_only_undamaged = cfg.sum_only_undamaged;
+    p.has_reduce = cfg.has_reduce;
+    p.has_damage = cfg.has_damage;
+    p.av_alpha = cfg.av_alpha;
+    p.av_beta = cfg.av_beta;
+    p.h_lo = cfg.h_lo;
+    p.h_hi = cfg.h_hi;
+    p.criteria = cfg.criteria;
+    p.courant = cfg.courant;
+    p.deriv_factor = cfg.deriv_factor;
+    p.div_factor = cfg.div_factor;
+    p.max_dt = cfg.max_dt;
+    return p;
+}
+
+// CubicSpline<3>::valueImpl / gradImpl (core/sph/kernel/Kernel.h:148-188) tabulated as in the
+// LutKernel<3> constructor (:83-101).  Built on the host in strict IEEE double so that the device
+// table equals the reference's table bit for bit.
+void build_lut(double* values, double* grads) {
+#pragma clang fp contract(off)
+    const double norm = 1.0 / PI_H;
+    const double rad = 2.0;
+    const double qSqrToIdx = double(LUT_ENTRIES) * (1.0 / (rad * rad));
+    for (int i = 0; i < LUT_ENTRIES + 1; ++i) {
+        const double qSqr = double(i) / qSqrToIdx;
+        const double q = std::sqrt(qSqr);
+        double v = 0.0, g = 0.0;
+        if (q < 1.0) {
+            v = norm * (0.25 * ((2.0 - q) * (2.0 - q) * (2.0 - q)) - (1.0 - q) * (1.0 - q) * (1.0 - q));
+        } else if (q < 2.0) {
+            v = norm * (0.25 * ((2.0 - q) * (2.0 - q) * (2.0 - q)));
+        }
+        if (q == 0.0) {
+            g = -3.0 * norm;
+        } else if (q < 1.0) {
+            g = (1.0 / q) * norm * (-0.75 * ((2.0 - q) * (2.0 - q)) + 3.0 * ((1.0 - q) * (1.0 - q)));
+        } else if (q < 2.0) {
+            g = (1.0 / q) * norm * (-0.75 * ((2.0 - q) * (2.0 - q)));
+        }
+        values[i] = v;
+        grads[i] = g;
+    }
+}
+
+double decode_ord(unsigned long long e) {
+    unsigned long long b = (e >> 63) ? (e & ~0x8000000000000000ull) : ~e;
+    double v;
+    std::memcpy(&v, &b, sizeof(v));
+    return v;
+}
+
+struct QDesc {
+    int ncomp;
+    int elem;  // 8 double, 4 uint32
+    int natural;
+    int planes[8];
+    int off[8];
+    bool u32;
+};
+
+bool describe(int q, int order, QDesc& d) {
+    d = QDesc{};
+    auto set = [&](int ncomp, int elem, int natural, std::initializer_list<int> pl, std::initializer_list<int> off,
+                   bool u32) {
+        d.ncomp = ncomp;
+        d.elem = elem;
+        d.natural = natural;
+        d.u32 = u32;
+        int k = 0;
+        for (int p : pl) d.planes[k++] = p;
+        k = 0;
+        for (int o : off) d.off[k++] = o;
+    };
+    switch (q) {
+    case OSPH_Q_POSITION:
+        if (order == 0) set(4, 8, 32, { X, Y, Z, H }, { 0, 8, 16, 24 }, false);
+        else if (order == 1) set(4, 8, 32, { VX, VY, VZ, VH }, { 0, 8, 16, 24 }, false);
+        else if (order == 2) set(3, 8, 32, { AX, AY, AZ }, { 0, 8, 16 }, false);
+        else return false;
+        return true;
+    case OSPH_Q_DEVIATORIC_STRESS:
+        if (order == 0) set(5, 8, 64, { SXX, SYY, SXY, SXZ, SYZ }, { 0, 8, 16, 24, 32 }, false);
+        else if (order == 1) set(5, 8, 64, { DSXX, DSYY, DSXY, DSXZ, DSYZ }, { 0, 8, 16, 24, 32 }, false);
+        else return false;
+        return true;
+    case OSPH_Q_VELOCITY_GRADIENT:
+        if (order != 0) return false;
+        set(6, 8, 64, { GXX, GYY, GZZ, GXY, GXZ, GYZ }, { 0, 8, 16, 32, 40, 48 }, false);
+        return true;
+    default:
+        break;
+    }
+    int plane = -1;
+    bool u32 = false;
+    switch (q) {
+    case OSPH_Q_MASS: plane = order == 0 ? M : -1; break;
+    case OSPH_Q_DENSITY: plane = order == 0 ? RHO : (order == 1 ? DRHO : -1); break;
+    case OSPH_Q_ENERGY: plane = order == 0 ? U : (order == 1 ? DU : -1); break;
+    case OSPH_Q_DAMAGE: plane = order == 0 ? D : (order == 1 ? DD : -1); break;
+    case OSPH_Q_PRESSURE: plane = order == 0 ? P : -1; break;
+    case OSPH_Q_SOUND_SPEED: plane = order == 0 ? CS : -1; break;
+    case OSPH_Q_STRESS_REDUCING: plane = order == 0 ? RED : -1; break;
+    case OSPH_Q_VELOCITY_DIVERGENCE: plane = order == 0 ? DIVV : -1; break;
+    case OSPH_Q_EPS_MIN: plane = order == 0 ? EPSMIN : -1; break;
+    case OSPH_Q_M_ZERO: plane = order == 0 ? MZERO : -1; break;
+    case OSPH_Q_EXPLICIT_GROWTH: plane = order == 0 ? GROWTH : -1; break;
+    case OSPH_Q_NEIGHBOR_CNT: plane = order == 0 ? NCNT : -1; u32 = true; break;
+    case OSPH_Q_FLAG: plane = order == 0 ? FLAG : -1; u32 = true; break;
+    case OSPH_Q_MATERIAL_ID: plane = order == 0 ? MAT : -1; u32 = true; break;
+    case OSPH_Q_N_FLAWS: plane = order == 0 ? NFLAWS : -1; u32 = true; break;
+    default: return false;
+    }
+    if (plane < 0) return false;
+    set(1, u32 ? 4 : 8, u32 ? 4 : 8, { plane }, { 0 }, u32);
+    return true;
+}
+
+void plane_ptrs(osph_ctx* c, const QDesc& d, void** out) {
+    for (int k = 0; k < d.ncomp; ++k) {
+        out[k] = d.u32 ? static_cast<void*>(c->s.u[d.planes[k]]) : static_cast<void*>(c->s.d[d.planes[k]]);
+    }
+}
+
+int sync_check(osph_ctx* c) {
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OSPH_OK;
+}
+
+// Grid + sort + pack + neighbour lists for the current positions (ISymmetricFinder::build).
+int build_neighbors(osph_ctx* c) {
+    const size_t n = c->n;
+    c->neighbors_valid = false;
+    if (n == 0) {
+        c->neighbors_valid = true;
+        return OSPH_OK;
+    }
+    Launch l = L(c);
+    uint32_t* acc32 = reinterpret_cast<uint32_t*>(c->small);
+    timed(c, "init_small", [&] {
+        launch_clear_u32(l, acc32, 6, 0xffffffffu);        // min x, y, z
+        launch_clear_u32(l, acc32 + 6, 8, 0u);             // max x, y, z, max h
+        launch_clear_u32(l, acc32 + 14, 2, 0xffffffffu);   // min h
+        launch_clear_u32(l, acc32 + 24, 2, 0u);            // overflow flag
+    });
+    timed(c, "bounds", [&] { launch_bounds_reduce(l, c->s, c->small, n); });
+    HIPCHK(c, hipMemcpyAsync(c->small_host, c->small, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    if (int e = sync_check(c)) return e;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = decode_ord(c->small_host[k]);
+        hi[k] = decode_ord(c->small_host[3 + k]);
+        if (!std::isfinite(lo[k]) || !std::isfinite(hi[k])) return c->fail(OSPH_ERR_STATE, "non-finite positions");
+    }
+    const double hmax = decode_ord(c->small_host[6]);
+    const double hmin = decode_ord(c->small_host[7]);
+    if (!(hmin > 0.0) || !std::isfinite(hmax)) return c->fail(OSPH_ERR_STATE, "smoothing lengths must be positive");
+
+    Grid g{};
+    g.hmin = hmin;
+    const double ratio = hmax / hmin;
+    g.nclass = ratio < 2.0 ? 1 : std::min(MAX_CLASSES, 1 + int(std::floor(std::log2(ratio))));
+    double clo[MAX_CLASSES][3], chi[MAX_CLASSES][3], chmax[MAX_CLASSES];
+    unsigned long long ccount[MAX_CLASSES] = { 0, 0, 0, 0 };
+    if (g.nclass == 1) {
+        for (int k = 0; k < 3; ++k) {
+            clo[0][k] = lo[k];
+            chi[0][k] = hi[k];
+        }
+        chmax[0] = hmax;
+        ccount[0] = n;
+    } else {
+        unsigned long long* cacc = c->small + 16;
+        uint32_t* cacc32 = reinterpret_cast<uint32_t*>(cacc);
+        for (int cl = 0; cl < MAX_CLASSES; ++cl) {
+            launch_clear_u32(l, cacc32 + 16 * cl, 6, 0xffffffffu);
+            launch_clear_u32(l, cacc32 + 16 * cl + 6, 10, 0u);
+        }
+        timed(c, "class_bounds", [&] { launch_class_bounds(l, c->s, g.nclass, hmin, cacc, n); });
+        HIPCHK(c, hipMemcpyAsync(c->small_host + 16, cacc, 8 * MAX_CLASSES * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, c->stream));
+        if (int e = sync_check(c)) return e;
+        for (int cl = 0; cl < g.nclass; ++cl) {
+            const unsigned long long* a = c->small_host + 16 + 8 * cl;
+            for (int k = 0; k < 3; ++k) {
+                clo[cl][k] = decode_ord(a[k]);
+                chi[cl][k] = decode_ord(a[3 + k]);
+            }
+            chmax[cl] = decode_ord(a[6]);
+            ccount[cl] = a[7];
+        }
+    }
+    uint32_t base = 0;
+    for (int cl = 0; cl < g.nclass; ++cl) {
+        GridClass& G = g.c[cl];
+        G.base = base;
+        if (ccount[cl] == 0) {
+            G.empty = 1;
+            continue;
+        }
+        double ext = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            G.lo[k] = clo[cl][k];
+            G.hi[k] = chi[cl][k];
+            ext = std::max(ext, chi[cl][k] - clo[cl][k]);
+        }
+        G.hmax = chmax[cl];
+        // cells no smaller than the largest pair distance within the class, at most 250 per axis
+        G.cell = std::max(2.0 * G.hmax * (1.0 + 1e-9), ext / 250.0);
+        G.inv_cell = 1.0 / G.cell;
+        int maxdim = 1;
+        for (int k = 0; k < 3; ++k) {
+            G.dim[k] = int(std::floor((G.hi[k] - G.lo[k]) / G.cell)) + 1;
+            maxdim = std::max(maxdim, G.dim[k]);
+        }
+        int bits = 1;
+        while ((1 << bits) < maxdim) ++bits;
+        G.bits = bits;
+        base += uint32_t(1) << (3 * bits);
+    }
+    c->grid = g;
+    const size_t cells = base;
+    if (int e = ensure_cells(c, cells)) return e;
+
+    timed(c, "cell_keys", [&] { launch_cell_keys(l, c->s, g, c->keys, c->iota, n); });
+    timed(c, "sort", [&] {
+        sort_pairs(l, c->sort_temp, c->sort_temp_bytes, c->keys, c->keys_sorted, c->iota, c->order, n);
+    });
+    timed(c, "cell_ranges", [&] {
+        launch_clear_u32(l, c->cstart, cells, 0u);
+        launch_clear_u32(l, c->cend, cells, 0u);
+        launch_cell_ranges(l, c->keys_sorted, g, c->cstart, c->cend, n);
+    });
+    timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->group, n); });
+    uint32_t* overflow = acc32 + 24;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (int e = ensure_list(c, n * size_t(c->kmax))) return e;
+        timed(c, "neighbor_lists", [&] {
+            launch_neighbor_lists(l, c->rec, g, c->cstart, c->cend, c->list, c->count, c->kmax, overflow, n);
+        });
+        uint32_t flag = 0;
+        HIPCHK(c, hipMemcpyAsync(&c->small_host[12], overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        if (int e = sync_check(c)) return e;
+        std::memcpy(&flag, &c->small_host[12], sizeof(uint32_t));
+        if (!flag) {
+            c->neighbors_valid = true;
+            return OSPH_OK;
+        }
+        // a list overflowed: grow to the largest count and rebuild
+        std::vector<uint32_t> cnt(n);
+        HIPCHK(c, hipMemcpy(cnt.data(), c->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        const uint32_t mx = *std::max_element(cnt.begin(), cnt.end());
+        c->kmax = std::max<uint32_t>(mx + 8, c->kmax * 2);
+        launch_clear_u32(l, overflow, 1, 0u);
+    }
+    return c->fail(OSPH_ERR_STATE, "neighbour list did not fit after growing");
+}
+
+int integrate(osph_ctx* c, osph_stats* stats) {
+    const size_t n = c->n;
+    Launch l = L(c);
+    timed(c, "material_init", [&] { launch_material_init(l, c->s, c->mats_dev, c->prm, n); });
+    if (int e = build_neighbors(c)) return e;
+    timed(c, "pairs_fp64", [&] {
+        launch_pairs_fp64(l, c->s, c->prm, c->rec, c->group, c->order, c->list, c->count, c->lut, c->kmax, n);
+    });
+    timed(c, "finalize", [&] { launch_finalize(l, c->s, c->mats_dev, c->prm, n); });
+    if (stats) {
+        std::vector<uint32_t> cnt(n);
+        if (n) HIPCHK(c, hipMemcpyAsync(cnt.data(), c->s.u[NCNT], n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                         c->stream));
+        if (int e = sync_check(c)) return e;
+        stats->neighbors_min = n ? *std::min_element(cnt.begin(), cnt.end()) : 0;
+        stats->neighbors_max = n ? *std::max_element(cnt.begin(), cnt.end()) : 0;
+        double sum = 0.0;
+        for (uint32_t v : cnt) sum += v;
+        stats->neighbors_mean = n ? sum / double(n) : 0.0;
+    }
+    return OSPH_OK;
+}
+
+// MultiCriterion selection over the four per-criterion minima (TimeStepCriterion.cpp:389-419); each
+// criterion is limited by the maximal step on its own.  Ids follow CriterionId.
+void select_timestep(const osph_ctx* c, const unsigned long long* mins, double* dt, int* crit) {
+    const int own[4] = { 3, 2, 4, 5 };  // CFL_CONDITION, DERIVATIVE, ACCELERATION, DIVERGENCE
+    double best = INFINITY;
+    int id = 0;
+    for (int k = 0; k < 4; ++k) {
+        if (!(c->cfg.criteria & (1 << k))) continue;
+        double v = INFINITY;
+        if (mins[k] != ~0ull) std::memcpy(&v, &mins[k], sizeof(double));
+        int cid = own[k];
+        if (v > c->cfg.max_dt) {
+            v = c->cfg.max_dt;
+            cid = 1;  // MAXIMAL_VALUE
+        }
+        if (v < best) {
+            best = v;
+            id = cid;
+        }
+    }
+    if (dt) *dt = best;
+    if (crit) *crit = id;
+}
+
+}  // namespace
+
+extern "C" {
+
+int osph_device_info(char* name, size_t cap, int* compute_units) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return OSPH_ERR_HIP;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return OSPH_ERR_HIP;
+    if (name && cap) {
+        std::snprintf(name, cap, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (compute_units) *compute_units = p.multiProcessorCount;
+    return OSPH_OK;
+}
+
+int osph_kernel_lut(double* values, double* grads) {
+    if (!values || !grads) return OSPH_ERR_ARG;
+    build_lut(values, grads);
+    return OSPH_OK;
+}
+
+int osph_create(osph_ctx** out, const osph_config* cfg, const osph_material* materials, int n_materials) {
+    if (!out || !cfg || !materials || n_materials <= 0) return OSPH_ERR_ARG;
+    *out = nullptr;
+    if (cfg->precision != OSPH_PRECISION_FP64 && cfg->precision != OSPH_PRECISION_MIXED) return OSPH_ERR_ARG;
+    osph_ctx* c = new osph_ctx();
+    c->cfg = *cfg;
+    c->prm = make_params(*cfg);
+    c->mats.assign(materials, materials + n_materials);
+    auto bail = [&](int code) {
+        osph_destroy(c);
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (hipMalloc(&c->mats_dev, sizeof(osph_material) * n_materials) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (hipMemcpy(c->mats_dev, materials, sizeof(osph_material) * n_materials, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(OSPH_ERR_HIP);
+    if (hipMalloc(&c->small, 64 * sizeof(unsigned long long)) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (hipHostMalloc(&c->small_host, 64 * sizeof(unsigned long long)) != hipSuccess) return bail(OSPH_ERR_HIP);
+    std::vector<double> values(LUT_ENTRIES + 1), grads(LUT_ENTRIES + 1);
+    build_lut(values.data(), grads.data());
+    if (hipMalloc(&c->lut, grads.size() * sizeof(double)) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (hipMemcpy(c->lut, grads.data(), grads.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(OSPH_ERR_HIP);
+    *out = c;
+    return OSPH_OK;
+}
+
+void osph_destroy(osph_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    resolve_events(c);
+    for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
+    free_state(c);
+    hipFree(c->cstart);
+    hipFree(c->cend);
+    hipFree(c->list);
+    hipFree(c->staging);
+    hipFree(c->mats_dev);
+    hipFree(c->small);
+    hipFree(c->lut);
+    if (c->small_host) hipHostFree(c->small_host);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* osph_last_error(const osph_ctx* c) {
+    return c ? c->err.c_str() : "null context";
+}
+
+int osph_resize(osph_ctx* c, size_t n) {
+    if (!c) return OSPH_ERR_ARG;
+    if (n > 0x7fffffffull) return c->fail(OSPH_ERR_ARG, "particle count exceeds 2^31-1");
+    c->neighbors_valid = false;
+    if (n > c->cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        free_state(c);
+        const size_t cap = n;
+        HIPCHK(c, hipMalloc(&c->dblock, sizeof(double) * NDPLANES * cap));
+        HIPCHK(c, hipMalloc(&c->ublock, sizeof(uint32_t) * NIPLANES * cap));
+        HIPCHK(c, hipMalloc(&c->keys, sizeof(uint32_t) * cap));
+        HIPCHK(c, hipMalloc(&c->keys_sorted, sizeof(uint32_t) * cap));
+        HIPCHK(c, hipMalloc(&c->iota, sizeof(uint32_t) * cap));
+        HIPCHK(c, hipMalloc(&c->order, sizeof(uint32_t) * cap));
+        HIPCHK(c, hipMalloc(&c->rec, sizeof(PairRec) * cap));
+        HIPCHK(c, hipMalloc(&c->group, sizeof(uint32_t) * cap));
+        HIPCHK(c, hipMalloc(&c->count, sizeof(uint32_t) * cap));
+        c->sort_temp_bytes = sort_pairs_temp_bytes(cap);
+        HIPCHK(c, hipMalloc(&c->sort_temp, std::max<size_t>(c->sort_temp_bytes, 256)));
+        c->cap = cap;
+        for (int k = 0; k < NDPLANES; ++k) c->s.d[k] = c->dblock + size_t(k) * cap;
+        for (int k = 0; k < NIPLANES; ++k) c->s.u[k] = c->ublock + size_t(k) * cap;
+    }
+    c->n = n;
+    if (c->cap) {
+        // every plane starts at zero (doubles: all-zero bits)
+        launch_clear_u32(L(c), reinterpret_cast<uint32_t*>(c->dblock), size_t(2) * NDPLANES * c->cap, 0u);
+        launch_clear_u32(L(c), c->ublock, size_t(NIPLANES) * c->cap, 0u);
+    }
+    return sync_check(c);
+}
+
+int osph_set_quantity(osph_ctx* c, int q, int order, const void* host, size_t stride) {
+    if (!c) return OSPH_ERR_ARG;
+    QDesc d;
+    if (!describe(q, order, d)) return c->fail(OSPH_ERR_ARG, "unknown quantity/order");
+    if (stride == 0) stride = size_t(d.natural);
+    if (stride < size_t(d.off[d.ncomp - 1] + d.elem)) return c->fail(OSPH_ERR_ARG, "stride smaller than element");
+    const size_t n = c->n;
+    if (n == 0) return OSPH_OK;
+    if (!host) return c->fail(OSPH_ERR_ARG, "null host buffer");
+    if (q == OSPH_Q_MATERIAL_ID) {
+        // the kernels index the material table with these values: validate them here
+        const unsigned char* b = static_cast<const unsigned char*>(host);
+        for (size_t i = 0; i < n; ++i) {
+            uint32_t v;
+            std::memcpy(&v, b + i * stride, sizeof(v));
+            if (v >= c->mats.size()) return c->fail(OSPH_ERR_ARG, "material id out of range");
+        }
+    }
+    if (int e = ensure_staging(c, n * stride)) return e;
+    HIPCHK(c, hipMemcpyAsync(c->staging, host, n * stride, hipMemcpyHostToDevice, c->stream));
+    void* dst[8];
+    plane_ptrs(c, d, dst);
+    launch_stage_to_planes(L(c), c->staging, stride, d.ncomp, d.off, d.elem, dst, n);
+    c->neighbors_valid = false;
+    return sync_check(c);
+}
+
+int osph_get_quantity(osph_ctx* c, int q, int order, void* host, size_t stride) {
+    if (!c) return OSPH_ERR_ARG;
+    QDesc d;
+    if (!describe(q, order, d)) return c->fail(OSPH_ERR_ARG, "unknown quantity/order");
+    if (stride == 0) stride = size_t(d.natural);
+    if (stride < size_t(d.off[d.ncomp - 1] + d.elem)) return c->fail(OSPH_ERR_ARG, "stride smaller than element");
+    const size_t n = c->n;
+    if (n == 0) return OSPH_OK;
+    if (!host) return c->fail(OSPH_ERR_ARG, "null host buffer");
+    if (int e = ensure_staging(c, n * stride)) return e;
+    if ((n * stride) % 4 == 0) launch_clear_u32(L(c), reinterpret_cast<uint32_t*>(c->staging), n * stride / 4, 0u);
+    void* src[8];
+    plane_ptrs(c, d, src);
+    launch_planes_to_stage(L(c), c->staging, stride, d.ncomp, d.off, d.elem, src, n);
+    HIPCHK(c, hipMemcpyAsync(host, c->staging, n * stride, hipMemcpyDeviceToHost, c->stream));
+    return sync_check(c);
+}
+
+int osph_zero_derivatives(osph_ctx* c) {
+    if (!c) return OSPH_ERR_ARG;
+    timed(c, "zero_derivatives", [&] { launch_zero_derivatives(L(c), c->s, c->n); });
+    return sync_check(c);
+}
+
+int osph_build_neighbors(osph_ctx* c) {
+    if (!c) return OSPH_ERR_ARG;
+    if (int e = build_neighbors(c)) return e;
+    return sync_check(c);
+}
+
+int osph_get_pairs(osph_ctx* c, uint32_t* pi, uint32_t* pj, size_t cap, size_t* count) {
+    if (!c || !count) return OSPH_ERR_ARG;
+    if (!c->neighbors_valid) return c->fail(OSPH_ERR_STATE, "neighbours not built");
+    const size_t n = c->n;
+    std::vector<uint32_t> cnt(n), order(n), list(n * size_t(c->kmax));
+    if (n) {
+        HIPCHK(c, hipMemcpy(cnt.data(), c->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(order.data(), c->order, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(list.data(), c->list, list.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    std::vector<uint64_t> keys;
+    for (size_t k = 0; k < n; ++k) {
+        const uint64_t i = order[k];
+        for (uint32_t m = 0; m < cnt[k] && m < c->kmax; ++m) {
+            const uint64_t j = order[list[size_t(m) * n + k]];
+            if (i < j) keys.push_back((i << 32) | j);
+        }
+    }
+    std::sort(keys.begin(), keys.end());
+    *count = keys.size();
+    for (size_t k = 0; k < keys.size() && k < cap; ++k) {
+        if (pi) pi[k] = uint32_t(keys[k] >> 32);
+        if (pj) pj[k] = uint32_t(keys[k] & 0xffffffffu);
+    }
+    return OSPH_OK;
+}
+
+int osph_integrate(osph_ctx* c, osph_stats* stats) {
+    if (!c) return OSPH_ERR_ARG;
+    if (int e = integrate(c, stats)) return e;
+    return sync_check(c);
+}
+
+int osph_timestep(osph_ctx* c, double* dt, int* crit) {
+    if (!c) return OSPH_ERR_ARG;
+    uint32_t* m32 = reinterpret_cast<uint32_t*>(c->small + 8);
+    launch_clear_u32(L(c), m32, 8, 0xffffffffu);
+    timed(c, "criteria", [&] { launch_criteria(L(c), c->s, c->mats_dev, c->prm, c->small + 8, c->n); });
+    HIPCHK(c, hipMemcpyAsync(c->small_host + 8, c->small + 8, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    if (int e = sync_check(c)) return e;
+    select_timestep(c, c->small_host + 8, dt, crit);
+    return OSPH_OK;
+}
+
+int osph_step_pc(osph_ctx* c, double dt, double* next_dt, int* crit) {
+    if (!c) return OSPH_ERR_ARG;
+    if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(OSPH_ERR_ARG, "time step must be positive");
+    const size_t n = c->n;
+    Launch l = L(c);
+    timed(c, "predict", [&] { launch_predict(l, c->s, c->mats_dev, c->prm, dt, n); });
+    if (int e = integrate(c, nullptr)) return e;
+    uint32_t* m32 = reinterpret_cast<uint32_t*>(c->small + 8);
+    launch_clear_u32(l, m32, 8, 0xffffffffu);
+    timed(c, "correct", [&] { launch_correct(l, c->s, c->mats_dev, c->prm, dt, c->small + 8, n); });
+    HIPCHK(c, hipMemcpyAsync(c->small_host + 8, c->small + 8, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    if (int e = sync_check(c)) return e;
+    select_timestep(c, c->small_host + 8, next_dt, crit);
+    return OSPH_OK;
+}
+
+int osph_sync(osph_ctx* c) {
+    if (!c) return OSPH_ERR_ARG;
+    return sync_check(c);
+}
+
+int osph_profile_enable(osph_ctx* c, int enable) {
+    if (!c) return OSPH_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    resolve_events(c);
+    c->prof = enable != 0;
+    c->prof_acc.clear();
+    return OSPH_OK;
+}
+
+int osph_profile_read(osph_ctx* c, char* names, size_t name_cap, double* ms, uint64_t* launches, int cap,
+                      int* count) {
+    if (!c || !count) return OSPH_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    resolve_events(c);
+    int k = 0;
+    for (const auto& kv : c->prof_acc) {
+        if (k < cap) {
+            if (names && name_cap) std::snprintf(names + size_t(k) * name_cap, name_cap, "%s", kv.first.c_str());
+            if (ms) ms[k] = kv.second.first;
+            if (launches) launches[k] = kv.second.second;
+        }
+        ++k;
+    }
+    *count = k;
+    return OSPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,942 @@
+// HIP kernels of the OpenSPH per-timestep hot path for gfx950 (MI355X).
+//
+// Layout: particle state is SoA in HBM (one plane per component, storage order).  Every evaluation
+// sorts the particles by the Morton code of a uniform cell grid (cell edge >= 2 h_max), packs the
+// per-neighbour data into 128-byte records in that order, builds neighbour lists with the exact FP64
+// pair test of SymmetricSolver<3>::loop, and evaluates all pair sums in gather form: each particle
+// sums over its own neighbours, so every sum is written by one thread, in a fixed order, without atomics.
+#include "osph_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+namespace osph {
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+inline unsigned grid_for(size_t n, int block = BLOCK) {
+    size_t g = (n + block - 1) / block;
+    return unsigned(g > 0 ? g : 1);
+}
+
+__device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }  // MathBasic.h:15-17
+__device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }  // MathBasic.h:20-22
+
+// EPS and LARGE are float literals widened to double (core/math/MathUtils.h:26-34).
+__device__ constexpr double EPS_D = double(1.e-12f);
+__device__ constexpr double LARGE_D = double(1.e20f);
+__device__ constexpr double PI_D = 3.14159265358979323846264338327950288;
+
+// ---- material physics (per particle; compiled without FMA contraction so that branch decisions
+// ---- follow the reference's rounding) -------------------------------------------------------------
+
+// TillotsonEos::evaluate (core/physics/Eos.cpp:198-238)
+__device__ void tillotson(const osph_material& m, double rho, double u, double& p_out, double& cs_out) {
+#pragma clang fp contract(off)
+    const double eta = rho / m.rho0;
+    const double mu = eta - 1.0;
+    const double denom = u / (m.u0 * eta * eta) + 1.0;
+    const double pc = (m.a + m.b / denom) * rho * u + m.A * mu + m.B * mu * mu;
+    double dpdu = m.a * rho + m.b * rho / (denom * denom);
+    double dpdrho = m.a * u + m.b * u * (3.0 * denom - 2.0) / (denom * denom) + m.A / m.rho0 + 2.0 * m.B * mu / m.rho0;
+    const double csc = dpdrho + dpdu * pc / (rho * rho);
+    const double rhoExp = m.rho0 / rho - 1.0;
+    const double betaExp = exp(-dmin(m.beta * rhoExp, 70.0));
+    const double alphaExp = exp(-dmin(m.alpha * (rhoExp * rhoExp), 70.0));
+    const double pe = m.a * rho * u + (m.b * rho * u / denom + m.A * mu * betaExp) * alphaExp;
+    dpdu = m.a * rho + alphaExp * m.b * rho / (denom * denom);
+    dpdrho = m.a * u + alphaExp * (m.b * u * (3.0 * denom - 2.0) / (denom * denom)) +
+             alphaExp * (m.b * u * rho / denom) * m.rho0 * (2.0 * m.alpha * rhoExp) / (rho * rho) +
+             alphaExp * m.A * betaExp * (1.0 / m.rho0 + m.rho0 * mu / (rho * rho) * (2.0 * m.alpha * rhoExp + m.beta));
+    double cse = dpdrho + dpdu * pe / (rho * rho);
+    cse = dmax(cse, 0.0);
+    double p = pc, cs = csc;
+    if (rho <= m.rho0 && u > m.ucv) {
+        p = pe;
+        cs = cse;
+    } else if (rho <= m.rho0 && u > m.uiv && u <= m.ucv) {
+        p = ((u - m.uiv) * pe + (m.ucv - u) * pc) / (m.ucv - m.uiv);
+        cs = ((u - m.uiv) * cse + (m.ucv - u) * csc) / (m.ucv - m.uiv);
+    }
+    cs = dmax(cs, 0.25 * m.A / m.rho0);
+    p_out = p;
+    cs_out = sqrt(cs);
+}
+
+// sqrtInv / sqrtApprox (core/math/MathUtils.h:44-62): the single-precision bit-trick estimate with one
+// Newton step that norm(Vector) uses (core/objects/geometry/Vector.h:681-685).
+__device__ double sqrt_approx(double f) {
+#pragma clang fp contract(off)
+    if (f == 0.0) return 0.0;
+    const float x2 = float(f) * 0.5f;
+    float y = float(f);
+    int i = __float_as_int(y);
+    i = 0x5f3759df - (i >> 1);
+    y = __int_as_float(i);
+    const float r = y * (1.5f - (x2 * y * y));
+    return 1.0 / double(r);
+}
+
+// findEigenvalues (core/objects/geometry/SymmetricTensor.h:377-398); t = {xx, yy, zz, xy, xz, yz}
+__device__ void eigenvalues(const double t[6], double out[3]) {
+#pragma clang fp contract(off)
+    const double d0 = t[0], d1 = t[1], d2 = t[2], o0 = t[3], o1 = t[4], o2 = t[5];
+    const double mx0 = dmax(d0, o0), mx1 = dmax(d1, o1), mx2 = dmax(d2, o2);
+    const double n = sqrt_approx(mx0 * mx0 + mx1 * mx1 + mx2 * mx2);
+    if (n < 1.e-12) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    const double I1 = d0 * 1.0 + d1 * 1.0 + d2 * 1.0;
+    const double I2 = (o0 * o0 + o1 * o1 + o2 * o2) - (d1 * d2 + d2 * d0 + d0 * d1);
+    const double I3 = d0 * d1 * d2 + 2.0 * o0 * o1 * o2 - ((o0 * o0) * d2 + (o1 * o1) * d1 + (o2 * o2) * d0);
+    const double p = -I1 / n;
+    const double q = -I2 / (n * n);
+    const double r = -I3 / (n * n * n);
+    const double a = q - p * p / 3.0;
+    const double b = (2.0 * (p * p * p) - 9.0 * p * q + 27.0 * r) / 27.0;
+    const double aCub = (a * a * a) / 27.0;
+    if (0.25 * b * b + aCub >= 0.0) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    const double t1 = 2.0 * sqrt(-a / 3.0);
+    const double phi = acos(-0.5 * b / sqrt(-aCub));
+    const double p3 = p / 3.0;
+    out[0] = (t1 * cos(phi / 3.0) - p3) * n;
+    out[1] = (t1 * cos((phi + 2.0 * PI_D) / 3.0) - p3) * n;
+    out[2] = (t1 * cos((phi + 4.0 * PI_D) / 3.0) - p3) * n;
+}
+
+// EosMaterial::initialize (core/sph/Materials.cpp:47-58), VonMisesRheology::initialize
+// (core/physics/Rheology.cpp:36-83) and the h clamp of AdaptiveSmoothingLength::initialize
+// (core/sph/equations/EquationTerm.cpp:356-364).
+__global__ void __launch_bounds__(BLOCK) k_material_init(DevState s, const osph_material* __restrict__ mats,
+                                                         Params prm, size_t n) {
+#pragma clang fp contract(off)
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const osph_material m = mats[s.u[MAT][i]];
+    double p, cs;
+    tillotson(m, s.d[RHO][i], s.d[U][i], p, cs);
+    s.d[CS][i] = cs;
+    if (m.yielding == 1) {
+        const double dmg = prm.has_damage ? s.d[D][i] : 0.0;
+        const double d = prm.has_damage ? dmg * dmg * dmg : 0.0;
+        if (p < 0.0) p = (1.0 - d) * p;
+        const double u = s.d[U][i];
+        const double unorm = u / m.melt_energy;
+        double Y = unorm < 1.e-5 ? m.elasticity_limit : m.elasticity_limit * dmax(1.0 - unorm, 0.0);
+        Y = (1.0 - d) * Y;
+        if (Y < EPS_D) {
+            s.d[RED][i] = 0.0;
+            s.d[SXX][i] = 0.0;
+            s.d[SYY][i] = 0.0;
+            s.d[SXY][i] = 0.0;
+            s.d[SXZ][i] = 0.0;
+            s.d[SYZ][i] = 0.0;
+        } else {
+            const double xx = s.d[SXX][i], yy = s.d[SYY][i], zz = -xx - yy;
+            const double xy = s.d[SXY][i], xz = s.d[SXZ][i], yz = s.d[SYZ][i];
+            const double dd = (xx * xx + yy * yy + zz * zz) + 2.0 * (xy * xy + xz * xz + yz * yz);
+            const double J2 = 0.5 * dd + 1.e-15;
+            const double red = dmin(Y / sqrt(3.0 * J2), 1.0);
+            s.d[RED][i] = red;
+            s.d[SXX][i] = xx * red;
+            s.d[SYY][i] = yy * red;
+            s.d[SXY][i] = xy * red;
+            s.d[SXZ][i] = xz * red;
+            s.d[SYZ][i] = yz * red;
+        }
+    }
+    s.d[P][i] = p;
+    if (prm.adaptive_h) {
+        s.d[H][i] = dmax(prm.h_lo, dmin(s.d[H][i], prm.h_hi));
+    }
+}
+
+// ---- grid ------------------------------------------------------------------------------------------
+
+// Order-preserving map of doubles onto unsigned 64-bit integers (for atomic min/max).
+__device__ __forceinline__ unsigned long long ord_bits(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ unsigned long long wave_min(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Block reduction of NV order-encoded values; is_min[k] selects min or max for value k.
+template <int NV>
+__device__ void block_reduce_ord(unsigned long long (&v)[NV], const bool (&is_min)[NV], unsigned long long* acc) {
+    __shared__ unsigned long long red[BLOCK / 64][NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const unsigned long long r = is_min[k] ? wave_min(v[k]) : wave_max(v[k]);
+        if (lane == 0) red[wave][k] = r;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < NV; k += blockDim.x) {
+        unsigned long long r = red[0][k];
+        for (int w = 1; w < BLOCK / 64; ++w) {
+            const unsigned long long x = red[w][k];
+            r = is_min[k] ? (x < r ? x : r) : (x > r ? x : r);
+        }
+        if (is_min[k]) {
+            atomicMin(&acc[k], r);
+        } else {
+            atomicMax(&acc[k], r);
+        }
+    }
+}
+
+// acc[0..2] = min x, y, z; acc[3..5] = max x, y, z; acc[6] = max h; acc[7] = min h (order-encoded).
+__global__ void __launch_bounds__(BLOCK) k_bounds(DevState s, unsigned long long* acc, size_t n) {
+    unsigned long long v[8] = { ~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull, 0ull, ~0ull };
+    const bool is_min[8] = { true, true, true, false, false, false, false, true };
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            const unsigned long long b = ord_bits(s.d[X + k][i]);
+            v[k] = b < v[k] ? b : v[k];
+            v[3 + k] = b > v[3 + k] ? b : v[3 + k];
+        }
+        const unsigned long long hb = ord_bits(s.d[H][i]);
+        v[6] = hb > v[6] ? hb : v[6];
+        v[7] = hb < v[7] ? hb : v[7];
+    }
+    block_reduce_ord<8>(v, is_min, acc);
+}
+
+__device__ __forceinline__ int h_class(double h, double hmin, int nclass) {
+    if (nclass <= 1) return 0;
+    const int c = int(floor(log2(h / hmin)));
+    return c < 0 ? 0 : (c >= nclass ? nclass - 1 : c);
+}
+
+// Per class c: acc[8c + 0..2] min x, y, z; [3..5] max x, y, z; [6] max h (order-encoded); [7] count.
+__global__ void __launch_bounds__(BLOCK) k_class_bounds(DevState s, int nclass, double hmin, unsigned long long* acc,
+                                                        size_t n) {
+    unsigned long long v[MAX_CLASSES * 7];
+    bool is_min[MAX_CLASSES * 7];
+    unsigned long long cnt[MAX_CLASSES] = { 0, 0, 0, 0 };
+    for (int c = 0; c < MAX_CLASSES; ++c) {
+        for (int k = 0; k < 7; ++k) {
+            is_min[7 * c + k] = k < 3;
+            v[7 * c + k] = k < 3 ? ~0ull : 0ull;
+        }
+    }
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const double h = s.d[H][i];
+        const int c = h_class(h, hmin, nclass);
+        for (int cc = 0; cc < MAX_CLASSES; ++cc) {
+            if (cc != c) continue;  // keep the register arrays statically indexed
+            for (int k = 0; k < 3; ++k) {
+                const unsigned long long b = ord_bits(s.d[X + k][i]);
+                v[7 * cc + k] = b < v[7 * cc + k] ? b : v[7 * cc + k];
+                v[7 * cc + 3 + k] = b > v[7 * cc + 3 + k] ? b : v[7 * cc + 3 + k];
+            }
+            const unsigned long long hb = ord_bits(h);
+            v[7 * cc + 6] = hb > v[7 * cc + 6] ? hb : v[7 * cc + 6];
+            cnt[cc] += 1;
+        }
+    }
+    __shared__ unsigned long long red[BLOCK / 64][MAX_CLASSES * 7 + MAX_CLASSES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < MAX_CLASSES * 7; ++k) {
+        const unsigned long long r = is_min[k] ? wave_min(v[k]) : wave_max(v[k]);
+        if (lane == 0) red[wave][k] = r;
+    }
+#pragma unroll
+    for (int c = 0; c < MAX_CLASSES; ++c) {
+        unsigned long long t = cnt[c];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane == 0) red[wave][MAX_CLASSES * 7 + c] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < MAX_CLASSES * 7) {
+        const int k = threadIdx.x;
+        unsigned long long r = red[0][k];
+        for (int w = 1; w < BLOCK / 64; ++w) {
+            const unsigned long long x = red[w][k];
+            r = (k % 7) < 3 ? (x < r ? x : r) : (x > r ? x : r);
+        }
+        unsigned long long* dst = &acc[8 * (k / 7) + (k % 7)];
+        if ((k % 7) < 3) {
+            atomicMin(dst, r);
+        } else {
+            atomicMax(dst, r);
+        }
+    } else if (threadIdx.x < MAX_CLASSES * 8) {
+        const int c = threadIdx.x - MAX_CLASSES * 7;
+        unsigned long long t = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) t += red[w][MAX_CLASSES * 7 + c];
+        if (t) atomicAdd(&acc[8 * c + 7], t);
+    }
+}
+
+__device__ __forceinline__ int cell_coord(const GridClass& g, double x, int k) {
+    return int(floor((x - g.lo[k]) * g.inv_cell));
+}
+
+__device__ __forceinline__ void cell_of(const GridClass& g, double x, double y, double z, int c[3]) {
+    const double v[3] = { x, y, z };
+    for (int k = 0; k < 3; ++k) {
+        int ci = cell_coord(g, v[k], k);
+        ci = ci < 0 ? 0 : (ci >= g.dim[k] ? g.dim[k] - 1 : ci);
+        c[k] = ci;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cell_keys(DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cls = h_class(s.d[H][i], g.hmin, g.nclass);
+    int c[3];
+    cell_of(g.c[cls], s.d[X][i], s.d[Y][i], s.d[Z][i], c);
+    keys[i] = (uint32_t(cls) << CLASS_SHIFT) | morton3(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
+    idx[i] = uint32_t(i);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill_u32(uint32_t* p, size_t count, uint32_t value) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x) {
+        p[i] = value;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cell_ranges(const uint32_t* __restrict__ keys, Grid g, uint32_t* cstart,
+                                                       uint32_t* cend, size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t key = keys[k];
+    const uint32_t cell = g.c[key >> CLASS_SHIFT].base + (key & ((1u << CLASS_SHIFT) - 1u));
+    if (k == 0 || keys[k - 1] != key) cstart[cell] = uint32_t(k);
+    if (k == n - 1 || keys[k + 1] != key) cend[cell] = uint32_t(k + 1);
+}
+
+// Pack the neighbour-side data of particle order[k] into rec[k].
+__global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const uint32_t* __restrict__ order,
+                                                PairRec* __restrict__ rec, uint32_t* __restrict__ group, size_t n) {
+#pragma clang fp contract(off)
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = order[k];
+    PairRec r;
+    r.x = s.d[X][i];
+    r.y = s.d[Y][i];
+    r.z = s.d[Z][i];
+    r.h = s.d[H][i];
+    r.vx = s.d[VX][i];
+    r.vy = s.d[VY][i];
+    r.vz = s.d[VZ][i];
+    r.m = s.d[M][i];
+    const double rho = s.d[RHO][i];
+    const double rho2 = rho * rho;
+    r.rho = rho;
+    r.cs = s.d[CS][i];
+    r.pr = s.d[P][i] / rho2;  // StandardForceDiscr: p / sqr(rho) (EquationTerm.cpp:20-23)
+    if (prm.solid_stress) {
+        r.sxx = s.d[SXX][i] / rho2;
+        r.syy = s.d[SYY][i] / rho2;
+        r.sxy = s.d[SXY][i] / rho2;
+        r.sxz = s.d[SXZ][i] / rho2;
+        r.syz = s.d[SYZ][i] / rho2;
+    } else {
+        r.sxx = r.syy = r.sxy = r.sxz = r.syz = 0.0;
+    }
+    rec[k] = r;
+    // DerivativeTemplate / AccelerationTemplate sum filter (DerivativeHelpers.h:132-139,262-269):
+    // a pair enters the SUM_ONLY_UNDAMAGED sums iff both flags are equal and both reductions nonzero.
+    uint32_t gid = 0;
+    if (prm.sum_only_undamaged && prm.has_reduce) {
+        gid = (s.d[RED][i] == 0.0) ? NO_GROUP : s.u[FLAG][i];
+    }
+    group[k] = gid;
+}
+
+// The pair test of SymmetricSolver<3>::loop (core/sph/solvers/SymmetricSolver.cpp:126-131) evaluated
+// bit for bit: getSqrLength sums the squares left to right without contraction and
+// sqr(2 * 0.5 * (h_i + h_j)) == (h_i + h_j)^2 exactly.
+__device__ __forceinline__ bool exact_pair(double xi, double yi, double zi, double hi, double xj, double yj, double zj,
+                                           double hj) {
+    const double dx = __dsub_rn(xi, xj), dy = __dsub_rn(yi, yj), dz = __dsub_rn(zi, zj);
+    const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+    const double hs = __dadd_rn(hi, hj);
+    return d2 < __dmul_rn(hs, hs);
+}
+
+// One thread per particle (sorted position k).  For every smoothing-length class whose bounding box
+// lies within h_i + h_max(class), the cells of that class within that radius are scanned with the exact
+// pair test; neighbour positions go to list[c * n + k] (column-major, coalesced across the wavefront).
+__global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const PairRec* __restrict__ rec, Grid g,
+                                                          const uint32_t* __restrict__ cstart,
+                                                          const uint32_t* __restrict__ cend, uint32_t* __restrict__ list,
+                                                          uint32_t* __restrict__ count, uint32_t kmax,
+                                                          uint32_t* overflow, size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double xi = rec[k].x, yi = rec[k].y, zi = rec[k].z, hi = rec[k].h;
+    const double p[3] = { xi, yi, zi };
+    uint32_t cnt = 0;
+    for (int cls = 0; cls < g.nclass; ++cls) {
+        const GridClass& G = g.c[cls];
+        if (G.empty) continue;
+        const double R = hi + G.hmax;  // h_i + h_j <= R for every member j
+        double d2b = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const double t = dmax(G.lo[a] - p[a], p[a] - G.hi[a]);
+            if (t > 0.0) d2b += t * t;
+        }
+        if (d2b > R * R * (1.0 + 1e-6)) continue;
+        const int r = int(ceil(R * G.inv_cell * (1.0 + 1e-9)));
+        int lo[3], up[3];
+        for (int a = 0; a < 3; ++a) {
+            const int c = cell_coord(G, p[a], a);
+            lo[a] = c - r < 0 ? 0 : c - r;
+            up[a] = c + r >= G.dim[a] ? G.dim[a] - 1 : c + r;
+        }
+        for (int cz = lo[2]; cz <= up[2]; ++cz) {
+            for (int cy = lo[1]; cy <= up[1]; ++cy) {
+                for (int cx = lo[0]; cx <= up[0]; ++cx) {
+                    const uint32_t cell = G.base + morton3(uint32_t(cx), uint32_t(cy), uint32_t(cz));
+                    const uint32_t b = cstart[cell], e = cend[cell];
+                    for (uint32_t k2 = b; k2 < e; ++k2) {
+                        if (k2 == k) continue;
+                        const PairRec& o = rec[k2];
+                        if (exact_pair(xi, yi, zi, hi, o.x, o.y, o.z, o.h)) {
+                            if (cnt < kmax) list[size_t(cnt) * n + k] = k2;
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    count[k] = cnt;
+    if (cnt > kmax) atomicOr(overflow, 1u);
+}
+
+// LutKernel<3>::gradImpl (core/sph/kernel/Kernel.h:129-144): linear interpolation in q^2.
+__device__ __forceinline__ double lut_grad(const double* __restrict__ grads, double q2) {
+    if (!(q2 < 4.0)) return 0.0;  // outside the support (also catches NaN, which must not become an index)
+    const double fi = double(LUT_ENTRIES / 4) * q2;
+    const uint32_t i1 = uint32_t(fi);
+    const double ratio = fi - double(i1);
+    return grads[i1] * (1.0 - ratio) + grads[i1 + 1] * ratio;
+}
+
+// FP64 pair sums in gather form.  For particle i (sorted position k) and each neighbour j, the kernel
+// gradient of SymmetrizeSmoothingLengths<LutKernel<3>> (Kernel.h:32-36,640-643) at hbar, then the
+// contributions every derivative of the term set makes to particle i:
+//   VelocityDivergence<CenterDensityDiscr>   DerivativeHelpers.h:355-371
+//   PressureGradient<StandardForceDiscr>     EquationTerm.cpp:61-66 via AccelerationTemplate :229-242
+//   VelocityGradient (SUM_ONLY_UNDAMAGED)    DerivativeHelpers.h:382-388, SymmetricTensor.h:362-368
+//   StressDivergence (SUM_ONLY_UNDAMAGED)    EquationTerm.cpp:134-139, TracelessTensor.h:151-155
+//   StandardAV                               av/Standard.h:62-83
+//   NeighborCountTerm                        HelperTerms.h:38-46
+// The accumulated buffers replace the storage buffers (Accumulated::store, Accumulated.cpp:87-100).
+__global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, const PairRec* __restrict__ rec,
+                                                      const uint32_t* __restrict__ group,
+                                                      const uint32_t* __restrict__ order,
+                                                      const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count,
+                                                      const double* __restrict__ lut, uint32_t kmax, size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const PairRec me = rec[k];
+    const uint32_t gi = group[k];
+    const double rhoInv = 1.0 / me.rho;
+    double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, divv = 0.0;
+    double gxx = 0.0, gyy = 0.0, gzz = 0.0, gxy = 0.0, gxz = 0.0, gyz = 0.0;
+    const uint32_t cnt = count[k];
+    const uint32_t listed = cnt < kmax ? cnt : kmax;
+    const bool solid = prm.solid_stress != 0;
+    for (uint32_t c = 0; c < listed; ++c) {
+        const uint32_t k2 = list[size_t(c) * n + k];
+        const PairRec o = rec[k2];
+        const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
+        const double hbar = 0.5 * (me.h + o.h);
+        const double hInv = 1.0 / hbar;
+        const double hInv2 = hInv * hInv;
+        const double d2 = rx * rx + ry * ry + rz * rz;
+        const double gk = lut_grad(lut, d2 * hInv2);
+        const double h5g = (hInv2 * hInv2) * hInv * gk;
+        const double gx = rx * h5g, gy = ry * h5g, gz = rz * h5g;
+        const double wx = o.vx - me.vx, wy = o.vy - me.vy, wz = o.vz - me.vz;  // v_j - v_i
+        const double mj = o.m;
+        const double mjr = mj * rhoInv;
+        // velocity divergence
+        divv += mjr * (wx * gx + wy * gy + wz * gz);
+        // pressure gradient
+        {
+            const double coef = me.pr + o.pr;
+            ax -= mj * (coef * gx);
+            ay -= mj * (coef * gy);
+            az -= mj * (coef * gz);
+        }
+        if (solid && gi == group[k2] && gi != NO_GROUP) {
+            gxx += mjr * (wx * gx);
+            gyy += mjr * (wy * gy);
+            gzz += mjr * (wz * gz);
+            gxy += mjr * (0.5 * (wx * gy + wy * gx));
+            gxz += mjr * (0.5 * (wx * gz + wz * gx));
+            gyz += mjr * (0.5 * (wy * gz + wz * gy));
+            const double xx = me.sxx + o.sxx, yy = me.syy + o.syy, xy = me.sxy + o.sxy;
+            const double xz = me.sxz + o.sxz, yz = me.syz + o.syz;
+            ax += mj * (xx * gx + xy * gy + xz * gz);
+            ay += mj * (xy * gx + yy * gy + yz * gz);
+            az += mj * (xz * gx + yz * gy + (-xx - yy) * gz);
+        }
+        // artificial viscosity, only in convergent flow
+        const double dvdr = -(wx * rx + wy * ry + wz * rz);  // (v_i - v_j) . (r_i - r_j)
+        if (dvdr < 0.0) {
+            const double rhobar = 0.5 * (me.rho + o.rho);
+            const double csbar = 0.5 * (me.cs + o.cs);
+            const double mu = hbar * dvdr / (d2 + 1.e-2 * (hbar * hbar));
+            const double av = (-prm.av_alpha * csbar * mu + prm.av_beta * (mu * mu)) / rhobar;
+            ax -= mj * (av * gx);
+            ay -= mj * (av * gy);
+            az -= mj * (av * gz);
+            du += mj * (0.5 * av * -(wx * gx + wy * gy + wz * gz));
+        }
+    }
+    const uint32_t i = order[k];
+    s.d[AX][i] = ax;
+    s.d[AY][i] = ay;
+    s.d[AZ][i] = az;
+    s.d[DU][i] = du;
+    s.d[DIVV][i] = divv;
+    if (solid) {
+        s.d[GXX][i] = gxx;
+        s.d[GYY][i] = gyy;
+        s.d[GZZ][i] = gzz;
+        s.d[GXY][i] = gxy;
+        s.d[GXZ][i] = gxz;
+        s.d[GYZ][i] = gyz;
+    }
+    s.u[NCNT][i] = cnt;
+}
+
+// ScalarGradyKippModel::integrate (core/physics/Damage.cpp:128-170) for particle i.
+__device__ void damage_rate(DevState& s, const osph_material& m, size_t i) {
+#pragma clang fp contract(off)
+    const double D0 = s.d[D][i];
+    if (D0 >= m.d_hi) {
+        s.d[DD][i] = LARGE_D;
+        return;
+    }
+    const double pr = s.d[P][i];
+    const double sxx = s.d[SXX][i], syy = s.d[SYY][i];
+    const double t[6] = { sxx - pr * 1.0, syy - pr * 1.0, (-sxx - syy) - pr * 1.0,
+                          s.d[SXY][i] - pr * 0.0, s.d[SXZ][i] - pr * 0.0, s.d[SYZ][i] - pr * 0.0 };
+    double sig[3];
+    eigenvalues(t, sig);
+    const double sigMax = dmax(dmax(sig[0], sig[1]), sig[2]);
+    const double young_red = dmax((1.0 - D0 * D0 * D0) * m.young_modulus, 1.e-20);
+    const double strain = sigMax / young_red;
+    const double ratio = strain / s.d[EPSMIN][i];
+    if (ratio <= 1.0) return;
+    s.d[DD][i] = s.d[GROWTH][i] * cbrt(dmin(pow(ratio, s.d[MZERO][i]), double(s.u[NFLAWS][i])));
+}
+
+// EquationHolder::finalize in reverse term order (EquationTerm.h:293-297): AdaptiveSmoothingLength
+// (EquationTerm.cpp:366-387) or ConstSmoothingLength (:427-434), ContinuityEquation (:289-299),
+// SolidStressForce (:177-203), PressureForce (:90-99); then the material finalize (damage).
+__global__ void __launch_bounds__(BLOCK) k_finalize(DevState s, const osph_material* __restrict__ mats, Params prm,
+                                                    size_t n) {
+#pragma clang fp contract(off)
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const osph_material m = mats[s.u[MAT][i]];
+    const double rho = s.d[RHO][i], divv = s.d[DIVV][i];
+    const double h = s.d[H][i];
+    s.d[VH][i] = (prm.adaptive_h && h > 2.0 * prm.h_lo) ? h / 3.0 * divv : 0.0;
+    s.d[DRHO][i] += -rho * divv;
+    double du = s.d[DU][i];
+    if (prm.solid_stress && m.yielding == 1) {
+        const double sxx = s.d[SXX][i], syy = s.d[SYY][i], szz = -sxx - syy;
+        const double gxx = s.d[GXX][i], gyy = s.d[GYY][i], gzz = s.d[GZZ][i];
+        const double gxy = s.d[GXY][i], gxz = s.d[GXZ][i], gyz = s.d[GYZ][i];
+        const double dd = (sxx * gxx + syy * gyy + szz * gzz) +
+                          2.0 * (s.d[SXY][i] * gxy + s.d[SXZ][i] * gxz + s.d[SYZ][i] * gyz);
+        du += 1.0 / rho * dd;
+        const double tr3 = (gxx * 1.0 + gyy * 1.0 + gzz * 1.0) / 3.0;
+        const double twomu = 2.0 * m.shear_modulus;
+        s.d[DSXX][i] += twomu * (gxx - tr3);
+        s.d[DSYY][i] += twomu * (gyy - tr3);
+        s.d[DSXY][i] += twomu * gxy;
+        s.d[DSXZ][i] += twomu * gxz;
+        s.d[DSYZ][i] += twomu * gyz;
+    }
+    du -= s.d[P][i] / rho * divv;
+    s.d[DU][i] = du;
+    if (prm.has_damage && m.damage == 1) damage_rate(s, m, i);
+}
+
+// Storage::zeroHighestDerivatives (core/quantities/Storage.cpp:594-602)
+__global__ void __launch_bounds__(BLOCK) k_zero_derivatives(DevState s, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int planes[] = { AX, AY, AZ, DRHO, DU, DSXX, DSYY, DSXY, DSXZ, DSYZ, DD };
+#pragma unroll
+    for (int k = 0; k < 11; ++k) s.d[planes[k]][i] = 0.0;
+}
+
+// clampWithDerivative<Float> (core/objects/wrappers/Interval.h:158-162)
+__device__ __forceinline__ void clamp_deriv(double& v, double& dv, double lo, double hi) {
+    const bool zero = (v >= hi && dv > 0.0) || (v <= lo && dv < 0.0);
+    v = dmax(lo, dmin(v, hi));
+    if (zero) dv = 0.0;
+}
+
+// PredictorCorrector::makePredictions (core/timestepping/TimeStepping.cpp:286-300), then the swap of
+// the highest derivatives into the predictions storage and Storage::zeroHighestDerivatives (:331-334).
+__global__ void __launch_bounds__(BLOCK) k_predict(DevState s, const osph_material* __restrict__ mats, Params prm,
+                                                   double dt, size_t n) {
+#pragma clang fp contract(off)
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double dt2 = 0.5 * (dt * dt);
+    for (int k = 0; k < 3; ++k) {
+        const double a = s.d[AX + k][i];
+        const double v = s.d[VX + k][i];
+        s.d[X + k][i] += v * dt + a * dt2;
+        s.d[VX + k][i] = v + a * dt;
+        s.d[PAX + k][i] = a;
+        s.d[AX + k][i] = 0.0;
+    }
+    s.d[H][i] += s.d[VH][i] * dt + 0.0 * dt2;
+    const osph_material m = mats[s.u[MAT][i]];
+    {
+        double x = s.d[RHO][i], dx = s.d[DRHO][i];
+        x += dx * dt;
+        clamp_deriv(x, dx, m.rho_lo, m.rho_hi);
+        s.d[RHO][i] = x;
+        s.d[PDRHO][i] = dx;
+        s.d[DRHO][i] = 0.0;
+    }
+    {
+        double x = s.d[U][i], dx = s.d[DU][i];
+        x += dx * dt;
+        clamp_deriv(x, dx, m.u_lo, m.u_hi);
+        s.d[U][i] = x;
+        s.d[PDU][i] = dx;
+        s.d[DU][i] = 0.0;
+    }
+    if (prm.solid_stress) {
+        for (int k = 0; k < 5; ++k) {
+            const double dx = s.d[DSXX + k][i];
+            s.d[SXX + k][i] += dx * dt;
+            s.d[PDSXX + k][i] = dx;
+            s.d[DSXX + k][i] = 0.0;
+        }
+    }
+    if (prm.has_damage) {
+        double x = s.d[D][i], dx = s.d[DD][i];
+        x += dx * dt;
+        clamp_deriv(x, dx, m.d_lo, m.d_hi);
+        s.d[D][i] = x;
+        s.d[PDD][i] = dx;
+        s.d[DD][i] = 0.0;
+    }
+}
+
+__device__ __forceinline__ void atomic_min_pos(unsigned long long* addr, double v) {
+    // v >= 0 (or +inf): the IEEE bit patterns of non-negative doubles are ordered like the values
+    atomicMin(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// Per-particle values of the four criteria of MultiCriterion (TimeStepCriterion.cpp:315-356 Courant,
+// :123-211 DerivativeCriterion with the minimal-step mean, :221-257 Acceleration, :267-305 Divergence),
+// reduced to the minimum of each criterion in mins[0..3].
+__device__ void criteria_values(const DevState& s, const osph_material& m, const Params& prm, size_t i, double out[4]) {
+#pragma clang fp contract(off)
+    out[0] = out[1] = out[2] = out[3] = INFINITY;
+    const double h = s.d[H][i];
+    if (prm.criteria & 1) {
+        const double cs = s.d[CS][i];
+        if (cs > 0.0) out[0] = prm.courant * h / cs;
+    }
+    if (prm.criteria & 2) {
+        double mn = INFINITY;
+        auto add = [&](double v, double dv, double minValue) {
+            const double av = fabs(v), adv = fabs(dv);
+            if (av < 2.0 * minValue) return;
+            mn = dmin(mn, prm.deriv_factor * (av + minValue) / (adv + EPS_D));
+        };
+        add(s.d[RHO][i], s.d[DRHO][i], m.rho_min);
+        add(s.d[U][i], s.d[DU][i], m.u_min);
+        if (prm.solid_stress) {
+            const double xx = s.d[SXX][i], yy = s.d[SYY][i];
+            const double dxx = s.d[DSXX][i], dyy = s.d[DSYY][i];
+            add(xx, dxx, m.s_min);
+            add(yy, dyy, m.s_min);
+            add(-xx - yy, -dxx - dyy, m.s_min);
+            add(s.d[SXY][i], s.d[DSXY][i], m.s_min);
+            add(s.d[SXZ][i], s.d[DSXZ][i], m.s_min);
+            add(s.d[SYZ][i], s.d[DSYZ][i], m.s_min);
+        }
+        if (prm.has_damage) add(s.d[D][i], s.d[DD][i], m.d_min);
+        out[1] = mn;
+    }
+    if (prm.criteria & 4) {
+        const double ax = s.d[AX][i], ay = s.d[AY][i], az = s.d[AZ][i];
+        const double dvNorm = ax * ax + ay * ay + az * az;
+        if (dvNorm > EPS_D) out[2] = prm.deriv_factor * sqrt(sqrt((h * h) / dvNorm));
+    }
+    if (prm.criteria & 8) {
+        const double dv = fabs(s.d[DIVV][i]);
+        if (dv > EPS_D) out[3] = prm.div_factor / dv;
+    }
+}
+
+__device__ void reduce_criteria(double v[4], unsigned long long* mins) {
+    __shared__ double red[BLOCK / 64][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = 0; c < 4; ++c) {
+        double x = v[c];
+        for (int o = 32; o > 0; o >>= 1) x = dmin(x, __shfl_xor(x, o, 64));
+        if (lane == 0) red[wave][c] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double x = red[0][threadIdx.x];
+        for (int w = 1; w < BLOCK / 64; ++w) x = dmin(x, red[w][threadIdx.x]);
+        if (x < INFINITY) atomic_min_pos(&mins[threadIdx.x], x);
+    }
+}
+
+// PredictorCorrector::makeCorrections (TimeStepping.cpp:302-322): pdv is the new derivative, cdv the
+// saved prediction; the clamp zeroes the new derivative at the bounds.  Fused with the criteria.
+__global__ void __launch_bounds__(BLOCK) k_correct(DevState s, const osph_material* __restrict__ mats, Params prm,
+                                                   double dt, unsigned long long* mins, size_t n) {
+#pragma clang fp contract(off)
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double v[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
+    if (i < n) {
+        const double dt2 = 0.5 * (dt * dt);
+        const double ca = 1.0 / 3.0, cb = 0.5;
+        for (int k = 0; k < 3; ++k) {
+            const double pdv = s.d[AX + k][i], cdv = s.d[PAX + k][i];
+            s.d[X + k][i] -= ca * (cdv - pdv) * dt2;
+            s.d[VX + k][i] -= cb * (cdv - pdv) * dt;
+        }
+        const osph_material m = mats[s.u[MAT][i]];
+        {
+            double x = s.d[RHO][i], px = s.d[DRHO][i];
+            x -= 0.5 * (s.d[PDRHO][i] - px) * dt;
+            clamp_deriv(x, px, m.rho_lo, m.rho_hi);
+            s.d[RHO][i] = x;
+            s.d[DRHO][i] = px;
+        }
+        {
+            double x = s.d[U][i], px = s.d[DU][i];
+            x -= 0.5 * (s.d[PDU][i] - px) * dt;
+            clamp_deriv(x, px, m.u_lo, m.u_hi);
+            s.d[U][i] = x;
+            s.d[DU][i] = px;
+        }
+        if (prm.solid_stress) {
+            for (int k = 0; k < 5; ++k) {
+                s.d[SXX + k][i] -= 0.5 * (s.d[PDSXX + k][i] - s.d[DSXX + k][i]) * dt;
+            }
+        }
+        if (prm.has_damage) {
+            double x = s.d[D][i], px = s.d[DD][i];
+            x -= 0.5 * (s.d[PDD][i] - px) * dt;
+            clamp_deriv(x, px, m.d_lo, m.d_hi);
+            s.d[D][i] = x;
+            s.d[DD][i] = px;
+        }
+        criteria_values(s, m, prm, i, v);
+    }
+    reduce_criteria(v, mins);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_criteria(DevState s, const osph_material* __restrict__ mats, Params prm,
+                                                    unsigned long long* mins, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double v[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
+    if (i < n) {
+        const osph_material m = mats[s.u[MAT][i]];
+        criteria_values(s, m, prm, i, v);
+    }
+    reduce_criteria(v, mins);
+}
+
+struct StageDesc {
+    int ncomp;
+    int elem;
+    int off[8];
+    void* p[8];
+};
+
+__global__ void __launch_bounds__(BLOCK) k_stage_to_planes(const unsigned char* __restrict__ st, size_t stride,
+                                                           StageDesc d, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned char* e = st + i * stride;
+    for (int c = 0; c < d.ncomp; ++c) {
+        if (d.elem == 8) {
+            static_cast<double*>(d.p[c])[i] = *reinterpret_cast<const double*>(e + d.off[c]);
+        } else {
+            static_cast<uint32_t*>(d.p[c])[i] = *reinterpret_cast<const uint32_t*>(e + d.off[c]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_planes_to_stage(unsigned char* __restrict__ st, size_t stride, StageDesc d,
+                                                           size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned char* e = st + i * stride;
+    for (int c = 0; c < d.ncomp; ++c) {
+        if (d.elem == 8) {
+            *reinterpret_cast<double*>(e + d.off[c]) = static_cast<const double*>(d.p[c])[i];
+        } else {
+            *reinterpret_cast<uint32_t*>(e + d.off[c]) = static_cast<const uint32_t*>(d.p[c])[i];
+        }
+    }
+}
+
+}  // namespace
+
+// ---- launchers ----------------------------------------------------------------------------------------
+
+void launch_material_init(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n) {
+    if (!n) return;
+    k_material_init<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, n);
+}
+
+void launch_bounds_reduce(const Launch& l, DevState s, unsigned long long* acc, size_t n) {
+    if (!n) return;
+    unsigned g = grid_for(n);
+    g = g > 1024 ? 1024 : g;
+    k_bounds<<<g, BLOCK, 0, l.stream>>>(s, acc, n);
+}
+
+void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n) {
+    if (!n) return;
+    k_cell_keys<<<grid_for(n), BLOCK, 0, l.stream>>>(s, g, keys, idx, n);
+}
+
+void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value) {
+    if (!count) return;
+    unsigned g = grid_for(count);
+    g = g > 4096 ? 4096 : g;
+    k_fill_u32<<<g, BLOCK, 0, l.stream>>>(p, count, value);
+}
+
+void launch_cell_ranges(const Launch& l, const uint32_t* keys, Grid g, uint32_t* cstart, uint32_t* cend, size_t n) {
+    if (!n) return;
+    k_cell_ranges<<<grid_for(n), BLOCK, 0, l.stream>>>(keys, g, cstart, cend, n);
+}
+
+void launch_class_bounds(const Launch& l, DevState s, int nclass, double hmin, unsigned long long* acc, size_t n) {
+    if (!n) return;
+    unsigned g = grid_for(n);
+    g = g > 1024 ? 1024 : g;
+    k_class_bounds<<<g, BLOCK, 0, l.stream>>>(s, nclass, hmin, acc, n);
+}
+
+void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, uint32_t* group,
+                 size_t n) {
+    if (!n) return;
+    k_pack<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, order, rec, group, n);
+}
+
+void launch_neighbor_lists(const Launch& l, const PairRec* rec, Grid g, const uint32_t* cstart, const uint32_t* cend,
+                           uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow, size_t n) {
+    if (!n) return;
+    k_neighbor_lists<<<grid_for(n), BLOCK, 0, l.stream>>>(rec, g, cstart, cend, list, count, kmax, overflow, n);
+}
+
+void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
+                       const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
+                       uint32_t kmax, size_t n) {
+    if (!n) return;
+    k_pairs_fp64<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n);
+}
+
+void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n) {
+    if (!n) return;
+    k_finalize<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, n);
+}
+
+void launch_zero_derivatives(const Launch& l, DevState s, size_t n) {
+    if (!n) return;
+    k_zero_derivatives<<<grid_for(n), BLOCK, 0, l.stream>>>(s, n);
+}
+
+void launch_predict(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt, size_t n) {
+    if (!n) return;
+    k_predict<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, n);
+}
+
+void launch_correct(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt,
+                    unsigned long long* mins, size_t n) {
+    if (!n) return;
+    k_correct<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, mins, n);
+}
+
+void launch_criteria(const Launch& l, DevState s, const osph_material* mats, Params prm, unsigned long long* mins,
+                     size_t n) {
+    if (!n) return;
+    k_criteria<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, mins, n);
+}
+
+static StageDesc make_desc(int ncomp, const int* off, int elem, void* const* p) {
+    StageDesc d{};
+    d.ncomp = ncomp;
+    d.elem = elem;
+    for (int c = 0; c < ncomp && c < 8; ++c) {
+        d.off[c] = off[c];
+        d.p[c] = p[c];
+    }
+    return d;
+}
+
+void launch_stage_to_planes(const Launch& l, const unsigned char* staging, size_t stride, int ncomp,
+                            const int* offsets_bytes, int elem_bytes, void* const* dst, size_t n) {
+    if (!n) return;
+    k_stage_to_planes<<<grid_for(n), BLOCK, 0, l.stream>>>(staging, stride,
+                                                            make_desc(ncomp, offsets_bytes, elem_bytes, dst), n);
+}
+
+void launch_planes_to_stage(const Launch& l, unsigned char* staging, size_t stride, int ncomp,
+                            const int* offsets_bytes, int elem_bytes, void* const* src, size_t n) {
+    if (!n) return;
+    k_planes_to_stage<<<grid_for(n), BLOCK, 0, l.stream>>>(staging, stride,
+                                                            make_desc(ncomp, offsets_bytes, elem_bytes, src), n);
+}
+
+size_t sort_pairs_temp_bytes(size_t n) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, int(n), 0, 32);
+    return bytes;
+}
+
+void sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                const uint32_t* vals_in, uint32_t* vals_out, size_t n) {
+    if (!n) return;
+    (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, int(n), 0, 32,
+                                             l.stream);
+}
+
+}  // namespace osph

@@ -1,0 +1,289 @@
+"""ctypes binding of libosph_gpu.so (the C ABI in include/osph_gpu.h).
+
+`GpuSolver` is the host-side counterpart of the reference's SymmetricSolver<3> + PredictorCorrector
+pair for this path: it uploads a `ParticleState` into HBM once, then `integrate()` mirrors
+ISolver::integrate (core/timestepping/ISolver.h:38) and `step()` mirrors ITimeStepping::step of
+PredictorCorrector (core/timestepping/TimeStepping.cpp:34-75,286-346).  There is no CPU fallback:
+if the library is missing or no GPU is visible, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Tuple
+
+import numpy as np
+
+from .state import ParticleState, RunConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libosph_gpu.so")
+
+# quantity ids (include/osph_gpu.h)
+Q = dict(POSITION=0, MASS=1, DENSITY=2, ENERGY=3, DEVIATORIC_STRESS=4, DAMAGE=5, PRESSURE=6, SOUND_SPEED=7,
+         STRESS_REDUCING=8, VELOCITY_DIVERGENCE=9, VELOCITY_GRADIENT=10, NEIGHBOR_CNT=11, FLAG=12,
+         MATERIAL_ID=13, EPS_MIN=14, M_ZERO=15, EXPLICIT_GROWTH=16, N_FLAWS=17)
+PRECISION = {"fp64": 0, "mixed": 1}
+
+EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info", "osph_resize",
+           "osph_set_quantity", "osph_get_quantity", "osph_zero_derivatives", "osph_build_neighbors",
+           "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
+           "osph_profile_enable", "osph_profile_read"]
+
+
+class Material(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("rho0", "A", "B", "a", "b", "alpha", "beta", "u0", "uiv", "ucv")] + [
+        ("yielding", C.c_int), ("damage", C.c_int)] + [
+        (k, C.c_double) for k in ("shear_modulus", "elasticity_limit", "melt_energy", "young_modulus",
+                                  "rho_lo", "rho_hi", "u_lo", "u_hi", "d_lo", "d_hi",
+                                  "rho_min", "u_min", "s_min", "d_min")]
+
+
+class Config(C.Structure):
+    _fields_ = [("solid_stress", C.c_int), ("adaptive_h", C.c_int), ("sum_only_undamaged", C.c_int),
+                ("has_reduce", C.c_int), ("has_damage", C.c_int),
+                ("av_alpha", C.c_double), ("av_beta", C.c_double), ("h_lo", C.c_double), ("h_hi", C.c_double),
+                ("criteria", C.c_int), ("courant", C.c_double), ("deriv_factor", C.c_double),
+                ("div_factor", C.c_double), ("max_dt", C.c_double), ("precision", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("neighbors_min", C.c_uint32), ("neighbors_max", C.c_uint32), ("neighbors_mean", C.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libosph_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m opensph_amd.build` (no CPU fallback)")
+    L = C.CDLL(path)
+    vp, i, sz, dp = C.c_void_p, C.c_int, C.c_size_t, C.POINTER(C.c_double)
+    L.osph_create.argtypes = [C.POINTER(vp), C.POINTER(Config), C.POINTER(Material), i]
+    L.osph_destroy.argtypes = [vp]
+    L.osph_destroy.restype = None
+    L.osph_last_error.argtypes = [vp]
+    L.osph_last_error.restype = C.c_char_p
+    L.osph_device_info.argtypes = [C.c_char_p, sz, C.POINTER(i)]
+    L.osph_resize.argtypes = [vp, sz]
+    L.osph_set_quantity.argtypes = [vp, i, i, vp, sz]
+    L.osph_get_quantity.argtypes = [vp, i, i, vp, sz]
+    L.osph_zero_derivatives.argtypes = [vp]
+    L.osph_build_neighbors.argtypes = [vp]
+    L.osph_get_pairs.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), sz, C.POINTER(sz)]
+    L.osph_integrate.argtypes = [vp, C.POINTER(Stats)]
+    L.osph_step_pc.argtypes = [vp, C.c_double, dp, C.POINTER(i)]
+    L.osph_timestep.argtypes = [vp, dp, C.POINTER(i)]
+    L.osph_sync.argtypes = [vp]
+    L.osph_kernel_lut.argtypes = [dp, dp]
+    L.osph_profile_enable.argtypes = [vp, i]
+    L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
+    _lib = L
+    return L
+
+
+def kernel_lut() -> Tuple[np.ndarray, np.ndarray]:
+    """The M4 look-up tables the device uses (host-built, no GPU call)."""
+    L = load_library()
+    v = np.zeros(40001)
+    g = np.zeros(40001)
+    L.osph_kernel_lut(v.ctypes.data_as(C.POINTER(C.c_double)), g.ctypes.data_as(C.POINTER(C.c_double)))
+    return v, g
+
+
+def _material(m: dict) -> Material:
+    out = Material()
+    for name, t in Material._fields_:
+        setattr(out, name, t(m[name]).value)
+    return out
+
+
+def _config(cfg: RunConfig, precision: str) -> Config:
+    c = Config()
+    for name, t in Config._fields_:
+        if name == "precision":
+            continue
+        setattr(c, name, t(getattr(cfg, name)).value)
+    c.precision = PRECISION[precision]
+    return c
+
+
+# ParticleState field -> (quantity, order, host layout builder)
+def _vec4(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _traceless(a):
+    out = np.zeros((a.shape[0], 8))
+    out[:, :5] = a
+    return out
+
+
+def _symmetric(a):
+    out = np.zeros((a.shape[0], 8))
+    out[:, 0:3] = a[:, 0:3]
+    out[:, 4:7] = a[:, 3:6]
+    return out
+
+
+_UPLOAD = [
+    ("r", "POSITION", 0), ("v", "POSITION", 1), ("dv", "POSITION", 2),
+    ("m", "MASS", 0), ("rho", "DENSITY", 0), ("drho", "DENSITY", 1), ("u", "ENERGY", 0), ("du", "ENERGY", 1),
+    ("s", "DEVIATORIC_STRESS", 0), ("ds", "DEVIATORIC_STRESS", 1), ("damage", "DAMAGE", 0),
+    ("ddamage", "DAMAGE", 1), ("p", "PRESSURE", 0), ("cs", "SOUND_SPEED", 0), ("reduce", "STRESS_REDUCING", 0),
+    ("divv", "VELOCITY_DIVERGENCE", 0), ("gradv", "VELOCITY_GRADIENT", 0), ("ncnt", "NEIGHBOR_CNT", 0),
+    ("flag", "FLAG", 0), ("mat", "MATERIAL_ID", 0), ("eps_min", "EPS_MIN", 0), ("m_zero", "M_ZERO", 0),
+    ("growth", "EXPLICIT_GROWTH", 0), ("n_flaws", "N_FLAWS", 0),
+]
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+class GpuSolver:
+    """HBM-resident SymmetricSolver<3> + PredictorCorrector for one ParticleState."""
+
+    def __init__(self, st: ParticleState, cfg: RunConfig, precision: str = "fp64"):
+        self.L = load_library()
+        self.cfg = cfg
+        self.precision = precision
+        self.materials = st.materials
+        mats = (Material * len(st.materials))(*[_material(m) for m in st.materials])
+        self._mats = mats
+        self._cfg = _config(cfg, precision)
+        self.ctx = C.c_void_p()
+        rc = self.L.osph_create(C.byref(self.ctx), C.byref(self._cfg), mats, len(st.materials))
+        if rc != 0:
+            raise GpuError(f"osph_create failed ({rc}): no usable GPU or HIP runtime error")
+        self.n = 0
+        self.dt = cfg.initial_dt
+        self.upload(st)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.L.osph_last_error(self.ctx).decode()
+            raise GpuError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.L.osph_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, st: ParticleState):
+        self._check(self.L.osph_resize(self.ctx, st.n), "osph_resize")
+        self.n = st.n
+        for field, q, order in _UPLOAD:
+            a = st[field]
+            if field == "s" or field == "ds":
+                host = _traceless(a)
+            elif field == "gradv":
+                host = _symmetric(a)
+            elif a.ndim == 2:
+                host = _vec4(a)
+            else:
+                host = np.ascontiguousarray(a)
+            self._check(self.L.osph_set_quantity(self.ctx, Q[q], order, host.ctypes.data, 0),
+                        f"osph_set_quantity({field})")
+
+    def download(self, template: ParticleState | None = None) -> ParticleState:
+        st = ParticleState(self.n, self.materials) if template is None else template.copy()
+        for field, q, order in _UPLOAD:
+            a = st.a[field]
+            if field in ("s", "ds", "gradv"):
+                host = np.zeros((self.n, 8))
+            elif a.ndim == 2:
+                host = np.zeros((self.n, 4))
+            else:
+                host = np.zeros(self.n, dtype=a.dtype)
+            self._check(self.L.osph_get_quantity(self.ctx, Q[q], order, host.ctypes.data, 0),
+                        f"osph_get_quantity({field})")
+            if field in ("s", "ds"):
+                a[:] = host[:, :5]
+            elif field == "gradv":
+                a[:, 0:3] = host[:, 0:3]
+                a[:, 3:6] = host[:, 4:7]
+            elif field == "dv":
+                a[:, :3] = host[:, :3]
+                a[:, 3] = 0.0
+            else:
+                a[...] = host
+        return st
+
+    def zero_derivatives(self):
+        self._check(self.L.osph_zero_derivatives(self.ctx), "osph_zero_derivatives")
+
+    def integrate(self) -> Dict[str, float]:
+        """zeroHighestDerivatives + SymmetricSolver<3>::integrate."""
+        self.zero_derivatives()
+        s = Stats()
+        self._check(self.L.osph_integrate(self.ctx, C.byref(s)), "osph_integrate")
+        return dict(neighbors_min=s.neighbors_min, neighbors_max=s.neighbors_max, neighbors_mean=s.neighbors_mean)
+
+    def step(self, dt: float | None = None) -> Tuple[float, float, int]:
+        """One PredictorCorrector step; returns (dt used, next dt, CriterionId)."""
+        dt = self.dt if dt is None else dt
+        nxt = C.c_double()
+        crit = C.c_int()
+        self._check(self.L.osph_step_pc(self.ctx, dt, C.byref(nxt), C.byref(crit)), "osph_step_pc")
+        self.dt = nxt.value
+        return dt, nxt.value, crit.value
+
+    def timestep(self) -> Tuple[float, int]:
+        dt = C.c_double()
+        crit = C.c_int()
+        self._check(self.L.osph_timestep(self.ctx, C.byref(dt), C.byref(crit)), "osph_timestep")
+        return dt.value, crit.value
+
+    def pairs(self) -> Tuple[np.ndarray, np.ndarray]:
+        self._check(self.L.osph_build_neighbors(self.ctx), "osph_build_neighbors")
+        cnt = C.c_size_t()
+        self._check(self.L.osph_get_pairs(self.ctx, None, None, 0, C.byref(cnt)), "osph_get_pairs")
+        pi = np.zeros(cnt.value, dtype=np.uint32)
+        pj = np.zeros(cnt.value, dtype=np.uint32)
+        self._check(self.L.osph_get_pairs(self.ctx, pi.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                          pj.ctypes.data_as(C.POINTER(C.c_uint32)), cnt.value, C.byref(cnt)),
+                    "osph_get_pairs")
+        return pi, pj
+
+    def sync(self):
+        self._check(self.L.osph_sync(self.ctx), "osph_sync")
+
+    def profile(self, enable: bool = True):
+        self._check(self.L.osph_profile_enable(self.ctx, int(enable)), "osph_profile_enable")
+
+    def profile_read(self) -> Dict[str, Tuple[float, int]]:
+        cap, name_cap = 64, 64
+        names = C.create_string_buffer(cap * name_cap)
+        ms = np.zeros(cap)
+        launches = np.zeros(cap, dtype=np.uint64)
+        cnt = C.c_int()
+        self._check(self.L.osph_profile_read(self.ctx, names, name_cap, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                             launches.ctypes.data_as(C.POINTER(C.c_uint64)), cap, C.byref(cnt)),
+                    "osph_profile_read")
+        out = {}
+        raw = names.raw
+        for k in range(min(cnt.value, cap)):
+            name = raw[k * name_cap:(k + 1) * name_cap].split(b"\0", 1)[0].decode()
+            out[name] = (float(ms[k]), int(launches[k]))
+        return out
+
+
+def device_info() -> Tuple[str, int]:
+    L = load_library()
+    buf = C.create_string_buffer(256)
+    cu = C.c_int()
+    rc = L.osph_device_info(buf, 256, C.byref(cu))
+    if rc != 0:
+        raise GpuError("no HIP device")
+    return buf.value.decode(), cu.value

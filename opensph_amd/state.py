@@ -92,7 +92,7 @@ class ParticleState:
         return out
 
     def subset(self, idx) -> "ParticleState":
-        idx = np.asarray(idx)
+        idx = np.asarray(idx, dtype=np.int64)
         out = ParticleState(len(idx), self.materials)
         for k, v in self.a.items():
             out.a[k][...] = v[idx]

@@ -1,0 +1,200 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the reference's golden vectors.
+
+Bars: neighbour pair sets bit-exact; FP64 derivatives, states and time steps within 1e-10 relative
+(L2 over the particles, per quantity) of the FP64 CPU result, per step."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from opensph_amd import gpu, initial
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+DERIV = ("dv", "du", "divv", "drho", "p", "cs", "ds", "gradv", "ddamage", "reduce", "s")
+STATE = ("r", "v", "dv", "rho", "drho", "u", "du", "s", "ds", "damage", "ddamage", "divv")
+
+
+def _cases():
+    from cases import exercised_fluid, exercised_impact
+
+    return {
+        "fluid10k": lambda: initial.fluid_sphere(),
+        "fluid_exercised": lambda: exercised_fluid(),
+        "impact_exercised": lambda: exercised_impact(),
+    }
+
+
+def _print_case(name, st):
+    print(f"[case] {name} n={st.n}", flush=True)
+
+
+def _assert_close(a, b, fields, tol=TOL):
+    for k in fields:
+        x, y = a[k], b[k]
+        if k == "dv":
+            x, y = x[:, :3], y[:, :3]
+        assert rel_l2(x, y) < tol, (k, rel_l2(x, y))
+
+
+@pytest.mark.parametrize("name", ["fluid10k", "fluid_exercised", "impact_exercised"])
+def test_pairs_bit_exact(oracle, name):
+    st, cfg = _cases()[name]()
+    _print_case(name, st)
+    g = gpu.GpuSolver(st, cfg)
+    pi, pj = g.pairs()
+    oi, oj = oracle.pairs(st["r"])
+    assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
+
+
+def test_pairs_mixed_smoothing_lengths(oracle):
+    """Four smoothing-length classes (h spans a factor 16): every class grid and cross-class search."""
+    rng = np.random.default_rng(5)
+    st, cfg = initial.fluid_sphere(n_requested=3000, radius=1000.0)
+    st["r"][:, :3] += rng.normal(0, 10.0, size=(st.n, 3))
+    st["r"][:, 3] *= np.exp2(rng.uniform(-2.0, 2.0, size=st.n))
+    _print_case("mixed_h", st)
+    g = gpu.GpuSolver(st, cfg)
+    pi, pj = g.pairs()
+    oi, oj = oracle.pairs(st["r"])
+    assert len(pi) > 0
+    assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
+
+
+def test_pairs_dense_cluster_grows_lists(oracle):
+    """More than the default list capacity of neighbours: the build must grow its lists and stay exact."""
+    rng = np.random.default_rng(9)
+    n = 700
+    st, cfg = initial.fluid_sphere(n_requested=n, radius=10.0)
+    st["r"][:, :3] = rng.uniform(-1.0, 1.0, size=(st.n, 3))
+    st["r"][:, 3] = 0.8
+    _print_case("dense", st)
+    g = gpu.GpuSolver(st, cfg)
+    pi, pj = g.pairs()
+    oi, oj = oracle.pairs(st["r"])
+    assert len(pi) / st.n > 100
+    assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
+
+
+@pytest.mark.parametrize("name", ["fluid10k", "fluid_exercised", "impact_exercised"])
+def test_integrate_matches_oracle(oracle, name):
+    st, cfg = _cases()[name]()
+    _print_case(name, st)
+    g = gpu.GpuSolver(st, cfg)
+    g.integrate()
+    a = g.download(st)
+    b = oracle.integrate(st, cfg)
+    _assert_close(a, b, DERIV)
+    assert np.array_equal(a["ncnt"], b["ncnt"])
+
+
+@pytest.mark.parametrize("name,steps", [("fluid10k", 20), ("impact_exercised", 6)])
+def test_predictor_corrector_matches_oracle(oracle, name, steps):
+    st, cfg = _cases()[name]()
+    _print_case(name, st)
+    g = gpu.GpuSolver(st, cfg)
+    glog = [g.step() for _ in range(steps)]
+    so, olog = oracle.run_pc(st, cfg, steps)
+    for (dt, nxt, crit), (odt, onxt, ocrit) in zip(glog, olog):
+        assert nxt == pytest.approx(onxt, rel=TOL)
+        assert crit == ocrit
+    _assert_close(g.download(st), so, STATE)
+
+
+@pytest.mark.parametrize("name", ["fluid.npz", "impact.npz"])
+def test_matches_reference_golden(name):
+    from test_oracle import _state_from, golden_case
+
+    g, meta, cfg, st = golden_case(name)
+    _print_case(name, st)
+    solver = gpu.GpuSolver(st, cfg)
+    solver.integrate()
+    a = solver.download(st)
+    ref = _state_from(g, "integ", meta["materials"])
+    _assert_close(a, ref, DERIV)
+    assert np.array_equal(a["ncnt"], ref["ncnt"])
+    solver.upload(st)
+    solver.dt = cfg.initial_dt
+    log = [solver.step() for _ in range(int(meta["steps"]))]
+    for (dt, nxt, crit), (rdt, rnxt, rcrit) in zip(log, g["dts"]):
+        assert nxt == pytest.approx(rnxt, rel=TOL)
+        assert crit == int(rcrit)
+    _assert_close(solver.download(st), _state_from(g, "steps", meta["materials"]),
+                  ("r", "v", "rho", "u", "s", "damage", "dv", "drho", "du", "ds"))
+
+
+def test_empty_and_single_particle():
+    st, cfg = initial.fluid_sphere(n_requested=200, radius=100.0)
+    one = st.subset([0])
+    _print_case("single", one)
+    g = gpu.GpuSolver(one, cfg)
+    g.integrate()
+    a = g.download(one)
+    assert a["ncnt"][0] == 0 and np.all(a["dv"] == 0.0)
+    dt, nxt, crit = g.step()
+    assert np.isfinite(nxt)
+    empty = st.subset([])
+    _print_case("empty", empty)
+    g = gpu.GpuSolver(empty, cfg)
+    g.integrate()
+    assert g.pairs()[0].size == 0
+
+
+def test_pair_cutoff_boundary(oracle):
+    """Two particles exactly at |r| = h_i + h_j are not neighbours; one ulp closer they are."""
+    st, cfg = initial.fluid_sphere(n_requested=200, radius=100.0)
+    two = st.subset([0, 1])
+    two["r"][:] = [[0.0, 0.0, 0.0, 1.0], [2.0, 0.0, 0.0, 1.0]]
+    _print_case("boundary", two)
+    g = gpu.GpuSolver(two, cfg)
+    assert g.pairs()[0].size == 0
+    two["r"][1, 0] = np.nextafter(2.0, 0.0)
+    g.upload(two)
+    pi, pj = g.pairs()
+    assert list(zip(pi, pj)) == [(0, 1)]
+    assert oracle.pairs(two["r"])[0].tolist() == [0]
+
+
+@pytest.mark.slow
+def test_full_size_impact_exact_pairs_and_derivatives(oracle):
+    """The 1,062,568-particle impact, perturbed so that every term is active: bit-exact pair set and
+    FP64 derivatives within 1e-10 of the oracle; momentum conserved to rounding."""
+    from cases import exercised_impact
+
+    st, cfg = exercised_impact(n_target=950_000, n_projectile=50_000, seed=17)
+    _print_case("impact_1M", st)
+    g = gpu.GpuSolver(st, cfg)
+    pi, pj = g.pairs()
+    oi, oj = oracle.pairs(st["r"])
+    assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
+    g.upload(st)
+    g.integrate()
+    a = g.download(st)
+    b = oracle.integrate(st, cfg)
+    _assert_close(a, b, DERIV)
+    assert np.array_equal(a["ncnt"], b["ncnt"])
+    p = (a["m"][:, None] * a["dv"][:, :3]).sum(axis=0)
+    scale = (a["m"][:, None] * np.abs(a["dv"][:, :3])).sum(axis=0)
+    assert np.all(np.abs(p) <= 1e-11 * scale)
+
+
+@pytest.mark.slow
+def test_full_size_impact_steps():
+    """The unperturbed 1M impact: neighbour statistics of the reference probe (29 / 66.3 / 68), first
+    step leaves the bodies undeformed (reference Impact test, core/sph/solvers/test/Impact.cpp:41-58),
+    and a few PC steps stay finite."""
+    st, cfg = initial.impact()
+    _print_case("impact_1M_plain", st)
+    g = gpu.GpuSolver(st, cfg)
+    stats = g.integrate()
+    assert stats["neighbors_min"] == 29 and stats["neighbors_max"] == 68
+    assert stats["neighbors_mean"] == pytest.approx(66.289, abs=1e-3)
+    g.upload(st)
+    g.step()
+    a = g.download(st)
+    assert np.all(a["gradv"] == 0.0) and np.all(a["s"] == 0.0) and np.all(a["rho"] == 2700.0)
+    for _ in range(3):
+        g.step()
+    a = g.download(st)
+    for k in ("r", "v", "rho", "u", "s"):
+        assert np.all(np.isfinite(a[k])), k
