@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run the 1 GiB device copy (HBM roof, FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X; "
@@ -83,6 +85,12 @@ def main():
         sys.exit(2)
 
     from opensph_amd import gpu, initial
+
+    if args.calibrate:
+        nbytes, iters = 1 << 30, 10
+        gbs = gpu.stream_copy(nbytes, iters)
+        print(json.dumps({"copy_bytes": nbytes, "iters": iters + 1, "read_plus_write_GBs": gbs}), flush=True)
+        return
 
     t0 = time.perf_counter()
     st, cfg = initial.impact()

@@ -135,6 +135,10 @@ int osph_sync(osph_ctx* ctx);
 /* ---- kernel table and profiling --------------------------------------------------------------------- */
 /* The 40001-entry M4 look-up tables the device uses (LutKernel<3>, core/sph/kernel/Kernel.h:83-101). */
 int osph_kernel_lut(double* values, double* grads);
+/* STREAM-style device copy of `bytes` bytes repeated `iters` times; *gbs = read+write GB/s.  Used to
+ * re-measure the HBM roof and to calibrate the FETCH_SIZE / WRITE_SIZE counters (no reference
+ * counterpart: measurement support). */
+int osph_stream_copy(size_t bytes, int iters, double* gbs);
 /* When enabled, every kernel launch is bracketed by HIP events on the context's stream and its
  * duration accumulated per kernel name. */
 int osph_profile_enable(osph_ctx* ctx, int enable);

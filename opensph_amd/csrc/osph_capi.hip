@@ -500,6 +500,35 @@ int osph_device_info(char* name, size_t cap, int* compute_units) {
     return OSPH_OK;
 }
 
+int osph_stream_copy(size_t bytes, int iters, double* gbs) {
+    if (bytes < 16 || iters <= 0 || !gbs) return OSPH_ERR_ARG;
+    bytes &= ~size_t(15);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = OSPH_ERR_HIP;
+    if (hipMalloc(&a, bytes) == hipSuccess && hipMalloc(&b, bytes) == hipSuccess &&
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+        hipEventCreate(&e1) == hipSuccess) {
+        launch_clear_u32(Launch{ st }, static_cast<uint32_t*>(a), bytes / 4, 0x3ff00000u);
+        launch_stream_copy(st, a, b, bytes);  // warm-up
+        hipEventRecord(e0, st);
+        for (int k = 0; k < iters; ++k) launch_stream_copy(st, a, b, bytes);
+        hipEventRecord(e1, st);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.f) {
+            *gbs = 2.0 * double(bytes) * iters / (double(ms) * 1e-3) / 1e9;
+            rc = OSPH_OK;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
+
 int osph_kernel_lut(double* values, double* grads) {
     if (!values || !grads) return OSPH_ERR_ARG;
     build_lut(values, grads);

@@ -814,7 +814,23 @@ __global__ void __launch_bounds__(BLOCK) k_planes_to_stage(unsigned char* __rest
     }
 }
 
+// STREAM-style copy: b[i] = a[i], 16 bytes per lane per iteration.
+__global__ void __launch_bounds__(BLOCK) k_stream_copy(const double2* __restrict__ a, double2* __restrict__ b,
+                                                       size_t n2) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += size_t(gridDim.x) * blockDim.x) {
+        b[i] = a[i];
+    }
+}
+
 }  // namespace
+
+void launch_stream_copy(hipStream_t stream, const void* a, void* b, size_t bytes) {
+    const size_t n2 = bytes / sizeof(double2);
+    if (!n2) return;
+    unsigned g = grid_for(n2);
+    g = g > 8192 ? 8192 : g;
+    k_stream_copy<<<g, BLOCK, 0, stream>>>(static_cast<const double2*>(a), static_cast<double2*>(b), n2);
+}
 
 // ---- launchers ----------------------------------------------------------------------------------------
 

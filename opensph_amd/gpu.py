@@ -28,6 +28,7 @@ PRECISION = {"fp64": 0, "mixed": 1}
 EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info", "osph_resize",
            "osph_set_quantity", "osph_get_quantity", "osph_zero_derivatives", "osph_build_neighbors",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
+           "osph_stream_copy",
            "osph_profile_enable", "osph_profile_read"]
 
 
@@ -80,6 +81,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_timestep.argtypes = [vp, dp, C.POINTER(i)]
     L.osph_sync.argtypes = [vp]
     L.osph_kernel_lut.argtypes = [dp, dp]
+    L.osph_stream_copy.argtypes = [sz, i, dp]
     L.osph_profile_enable.argtypes = [vp, i]
     L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
     _lib = L
@@ -277,6 +279,16 @@ class GpuSolver:
             name = raw[k * name_cap:(k + 1) * name_cap].split(b"\0", 1)[0].decode()
             out[name] = (float(ms[k]), int(launches[k]))
         return out
+
+
+def stream_copy(nbytes: int = 1 << 30, iters: int = 20) -> float:
+    """Read+write GB/s of a plain device copy of nbytes (HBM roof and counter calibration)."""
+    L = load_library()
+    gbs = C.c_double()
+    rc = L.osph_stream_copy(nbytes, iters, C.byref(gbs))
+    if rc != 0:
+        raise GpuError("osph_stream_copy failed")
+    return gbs.value
 
 
 def device_info() -> Tuple[str, int]:
