@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 # Algorithmic FP64 flops per unordered neighbour pair in the pair-sum kernel: the per-term counts of the
 # symmetric form (LUT gradient 18, pressure 24, velocity divergence 16, artificial viscosity 60,
 # velocity gradient 38, stress divergence 42), without the pair test (done by the list kernel).
@@ -69,13 +70,64 @@ def cpu_baseline(st, cfg, max_seconds=25.0):
         return None
 
 
+def total_energy(st):
+    v2 = np.sum(st["v"][:, :3] ** 2, axis=1)
+    return float(np.sum(st["m"] * (0.5 * v2 + st["u"])))
+
+
+def run_precision(gpu, st, cfg, prec, warmup, steps, tune=None):
+    solver = gpu.GpuSolver(st, cfg, precision=prec)
+    if tune:
+        solver.tune(**tune)
+    for _ in range(warmup):
+        solver.step()
+    solver.sync()
+    solver.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.step()
+    solver.sync()
+    elapsed = time.perf_counter() - t0
+    prof = solver.profile_read()
+    solver.profile(False)
+    final = solver.download(st)
+    stats = solver.integrate()  # neighbour statistics of the final state (outside the timed region)
+    solver.close()
+    return {"value": st.n * steps / elapsed, "elapsed": elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "prof": prof, "stats": stats, "final": final}
+
+
+def compare_precisions(gpu, st, cfg, runs):
+    """Acceleration error of the mixed pair sums against FP64 on the same (final FP64) state, and the
+    energy of both runs after warmup + steps (the 100-step workload)."""
+    ref_state = runs["fp64"]["final"]
+    out = {}
+    for prec in ("fp64", "mixed"):
+        g = gpu.GpuSolver(ref_state, cfg, precision=prec)
+        g.integrate()
+        out[prec] = g.download(ref_state)
+        g.close()
+    a64, amx = out["fp64"]["dv"][:, :3], out["mixed"]["dv"][:, :3]
+    e0 = total_energy(st)
+    e64, emx = total_energy(runs["fp64"]["final"]), total_energy(runs["mixed"]["final"])
+    return {"accel_rel_l2": float(np.linalg.norm(amx - a64) / np.linalg.norm(a64)),
+            "drho_rel_l2": float(np.linalg.norm(out["mixed"]["drho"] - out["fp64"]["drho"]) /
+                                 np.linalg.norm(out["fp64"]["drho"])),
+            "neighbor_counts_equal": bool(np.array_equal(out["mixed"]["ncnt"], out["fp64"]["ncnt"])),
+            "energy_drift_fp64": (e64 - e0) / e0, "energy_drift_mixed": (emx - e0) / e0,
+            "energy_mixed_vs_fp64": (emx - e64) / e0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=80)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
+                    help="both: time the FP64 drop-in and the mixed-precision form; value = mixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[],
+                    help="launch-shape setting key=value passed to osph_tune (e.g. pairs_lds=40960)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the 1 GiB device copy (HBM roof, FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -95,32 +147,34 @@ def main():
     t0 = time.perf_counter()
     st, cfg = initial.impact()
     log(f"setup {st.n} particles in {time.perf_counter() - t0:.1f}s")
-    solver = gpu.GpuSolver(st, cfg, precision=args.precision)
     name, cus = gpu.device_info()
-    for _ in range(args.warmup):
-        solver.step()
-    solver.sync()
-    solver.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        solver.step()
-    solver.sync()
-    elapsed = time.perf_counter() - t0
-    prof = solver.profile_read()
-    solver.profile(False)
-    stats = solver.integrate()  # neighbour statistics of the final state (outside the timed region)
-    value = st.n * args.steps / elapsed
+    precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {}
+    for prec in precisions:
+        tune = dict((kv.split("=")[0], int(kv.split("=")[1])) for kv in args.tune)
+        runs[prec] = run_precision(gpu, st, cfg, prec, args.warmup, args.steps, tune)
+        log(f"{prec}: {runs[prec]['value']:.4g} particle-steps/s, {runs[prec]['ms_per_step']:.3f} ms/step")
+    main_prec = precisions[-1]
+    r = runs[main_prec]
+    value = r["value"]
+    elapsed = r["elapsed"]
+    stats = r["stats"]
+    prof = r["prof"]
 
     pairs = stats["neighbors_mean"] * st.n / 2.0
-    kernel = "pairs_fp64"
+    kernel = "pairs_mixed" if main_prec == "mixed" else "pairs_fp64"
     ms, launches = prof.get(kernel, (float("nan"), 1))
     avg_ms = ms / max(launches, 1)
     achieved = FLOP_PER_PAIR * pairs / (avg_ms * 1e-3) / 1e12
+    peak = FP32_PEAK_TFLOPS if main_prec == "mixed" else FP64_PEAK_TFLOPS
     dominant = max(prof.items(), key=lambda kv: kv[1][0])[0] if prof else kernel
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kernel,
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None, "kernel": kernel,
                 "kernel_ms": avg_ms, "dominant_kernel": dominant,
                 "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in sorted(prof.items())}}
+    accuracy = None
+    if "fp64" in runs and "mixed" in runs:
+        accuracy = compare_precisions(gpu, st, cfg, runs)
 
     baseline = None if args.no_cpu else cpu_baseline(st, cfg)
     out = {
@@ -134,12 +188,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if main_prec == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "basalt two-body impact (R 50 km + 5 km, 5 km/s, 45 deg), solid stress + AV + "
                                "continuity + adaptive h, von Mises + Grady-Kipp damage, predictor-corrector",
-                   "particles": st.n, "precision": args.precision, "device": name, "compute_units": cus,
+                   "particles": st.n, "precision": main_prec, "device": name, "compute_units": cus,
                    "neighbors_mean": stats["neighbors_mean"]},
+        "fp64_value": runs["fp64"]["value"] if "fp64" in runs else None,
+        "fp64_kernel_ms_per_step": ({k: v[0] / max(v[1], 1) for k, v in sorted(runs["fp64"]["prof"].items())}
+                                    if "fp64" in runs else None),
+        "mixed_over_fp64": (runs["mixed"]["value"] / runs["fp64"]["value"]) if len(runs) == 2 else None,
+        "accuracy": accuracy,
         "roofline": roofline,
         "cpu_baseline": baseline,
     }

@@ -131,6 +131,12 @@ int osph_integrate(osph_ctx* ctx, osph_stats* stats);
 int osph_step_pc(osph_ctx* ctx, double dt, double* next_dt, int* criterion);
 int osph_timestep(osph_ctx* ctx, double* dt, int* criterion);
 int osph_sync(osph_ctx* ctx);
+/* Launch-shape tuning of the neighbour-list and pair kernels (no reference counterpart: the reference
+ * has no device kernels).  Keys: "pairs_block", "lists_block" (64, 128 or 256 threads) and
+ * "pairs_lds", "lists_lds" (0..65536 bytes of dynamic LDS per block, which caps resident blocks per CU),
+ * "cell_divisor" (1, 2 or 3: neighbour-grid cells of 2 h_max / divisor), "pair_tiles" (0 or 1: mixed
+ * precision stages each block's neighbour records in LDS). */
+int osph_tune(osph_ctx* ctx, const char* key, long long value);
 
 /* ---- kernel table and profiling --------------------------------------------------------------------- */
 /* The 40001-entry M4 look-up tables the device uses (LutKernel<3>, core/sph/kernel/Kernel.h:83-101). */

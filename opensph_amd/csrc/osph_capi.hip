@@ -43,6 +43,11 @@ struct osph_ctx {
     uint32_t *cstart = nullptr, *cend = nullptr;
     size_t cell_cap = 0;
     PairRec* rec = nullptr;
+    PairRecF* recf = nullptr;
+    uint32_t* tile_idx = nullptr;  // mixed precision: per-block neighbour tiles
+    uint32_t* tile_cnt = nullptr;
+    Lattice lattice{};
+    double4* pos = nullptr;
     uint32_t* group = nullptr;
     uint32_t* list = nullptr;
     uint32_t* count = nullptr;
@@ -51,6 +56,7 @@ struct osph_ctx {
     unsigned long long* small = nullptr;     // [0..7] bounds, [8..11] criteria, [12] overflow, [16..47] classes
     unsigned long long* small_host = nullptr;  // pinned mirror
     double* lut = nullptr;
+    float* lutf = nullptr;
     Grid grid{};
     bool neighbors_valid = false;
 
@@ -62,6 +68,7 @@ struct osph_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, uint64_t>> prof_acc;
 
+    Tuning tune{};
     std::string err;
 
     int fail(int code, const std::string& msg) {
@@ -120,7 +127,7 @@ void resolve_events(osph_ctx* c) {
     c->events.clear();
 }
 
-Launch L(osph_ctx* c) { return Launch{ c->stream }; }
+Launch L(osph_ctx* c) { return Launch{ c->stream, c->tune }; }
 
 void free_state(osph_ctx* c) {
     hipFree(c->dblock);
@@ -131,6 +138,10 @@ void free_state(osph_ctx* c) {
     hipFree(c->order);
     hipFree(c->sort_temp);
     hipFree(c->rec);
+    hipFree(c->recf);
+    hipFree(c->tile_idx);
+    hipFree(c->tile_cnt);
+    hipFree(c->pos);
     hipFree(c->group);
     hipFree(c->count);
     c->dblock = nullptr;
@@ -138,6 +149,9 @@ void free_state(osph_ctx* c) {
     c->keys = c->keys_sorted = c->iota = c->order = nullptr;
     c->sort_temp = nullptr;
     c->rec = nullptr;
+    c->recf = nullptr;
+    c->tile_idx = c->tile_cnt = nullptr;
+    c->pos = nullptr;
     c->group = c->count = nullptr;
     c->cap = 0;
 }
@@ -386,8 +400,8 @@ int build_neighbors(osph_ctx* c) {
             ext = std::max(ext, chi[cl][k] - clo[cl][k]);
         }
         G.hmax = chmax[cl];
-        // cells no smaller than the largest pair distance within the class, at most 250 per axis
-        G.cell = std::max(2.0 * G.hmax * (1.0 + 1e-9), ext / 250.0);
+        // cells of 2 h_max(class) / divisor, at most 250 per axis
+        G.cell = std::max(2.0 * G.hmax * (1.0 + 1e-9) / double(c->tune.cell_divisor), ext / 250.0);
         G.inv_cell = 1.0 / G.cell;
         int maxdim = 1;
         for (int k = 0; k < 3; ++k) {
@@ -399,25 +413,46 @@ int build_neighbors(osph_ctx* c) {
         G.bits = bits;
         base += uint32_t(1) << (3 * bits);
     }
+    int maxbits = 1;
+    for (int cl = 0; cl < g.nclass; ++cl) maxbits = std::max(maxbits, g.c[cl].empty ? 1 : g.c[cl].bits);
+    g.shift = 3 * maxbits;
+    g.key_bits = g.shift + (g.nclass > 2 ? 2 : (g.nclass > 1 ? 1 : 0));
+    if (g.key_bits > 32) return c->fail(OSPH_ERR_STATE, "sort key exceeds 32 bits");
     c->grid = g;
     const size_t cells = base;
     if (int e = ensure_cells(c, cells)) return e;
 
     timed(c, "cell_keys", [&] { launch_cell_keys(l, c->s, g, c->keys, c->iota, n); });
     timed(c, "sort", [&] {
-        sort_pairs(l, c->sort_temp, c->sort_temp_bytes, c->keys, c->keys_sorted, c->iota, c->order, n);
+        sort_pairs(l, c->sort_temp, c->sort_temp_bytes, c->keys, c->keys_sorted, c->iota, c->order, n, g.key_bits);
     });
     timed(c, "cell_ranges", [&] {
         launch_clear_u32(l, c->cstart, cells, 0u);
         launch_clear_u32(l, c->cend, cells, 0u);
         launch_cell_ranges(l, c->keys_sorted, g, c->cstart, c->cend, n);
     });
-    timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->group, n); });
+    if (c->cfg.precision == OSPH_PRECISION_MIXED) {
+        // lattice of power-of-two edge >= 2 h_min (10 bits per axis) for the FP32 relative positions
+        Lattice lat{};
+        double S = std::exp2(std::ceil(std::log2(2.0 * hmin)));
+        double ext = 0.0;
+        for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
+        while (ext / S > 1000.0) S *= 2.0;
+        for (int k = 0; k < 3; ++k) lat.lo[k] = lo[k];
+        lat.S = S;
+        lat.Sf = float(S);
+        c->lattice = lat;
+        timed(c, "pack", [&] { launch_pack_mixed(l, c->s, c->prm, lat, c->order, c->recf, c->pos, c->group, n); });
+    } else {
+        timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->pos, c->group, n); });
+    }
     uint32_t* overflow = acc32 + 24;
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (int e = ensure_list(c, n * size_t(c->kmax))) return e;
         timed(c, "neighbor_lists", [&] {
-            launch_neighbor_lists(l, c->rec, g, c->cstart, c->cend, c->list, c->count, c->kmax, overflow, n);
+            const bool tiles = c->cfg.precision == OSPH_PRECISION_MIXED && c->tune.pair_tiles;
+            launch_neighbor_lists(l, c->pos, g, c->cstart, c->cend, c->list, c->count, c->kmax, overflow,
+                                  tiles ? c->tile_idx : nullptr, tiles ? c->tile_cnt : nullptr, n);
         });
         uint32_t flag = 0;
         HIPCHK(c, hipMemcpyAsync(&c->small_host[12], overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -431,7 +466,7 @@ int build_neighbors(osph_ctx* c) {
         std::vector<uint32_t> cnt(n);
         HIPCHK(c, hipMemcpy(cnt.data(), c->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         const uint32_t mx = *std::max_element(cnt.begin(), cnt.end());
-        c->kmax = std::max<uint32_t>(mx + 8, c->kmax * 2);
+        c->kmax = (std::max<uint32_t>(mx + 8, c->kmax * 2) + 3u) & ~3u;
         launch_clear_u32(l, overflow, 1, 0u);
     }
     return c->fail(OSPH_ERR_STATE, "neighbour list did not fit after growing");
@@ -442,9 +477,17 @@ int integrate(osph_ctx* c, osph_stats* stats) {
     Launch l = L(c);
     timed(c, "material_init", [&] { launch_material_init(l, c->s, c->mats_dev, c->prm, n); });
     if (int e = build_neighbors(c)) return e;
-    timed(c, "pairs_fp64", [&] {
-        launch_pairs_fp64(l, c->s, c->prm, c->rec, c->group, c->order, c->list, c->count, c->lut, c->kmax, n);
-    });
+    if (c->cfg.precision == OSPH_PRECISION_MIXED) {
+        timed(c, "pairs_mixed", [&] {
+            const bool tiles = c->tune.pair_tiles != 0;
+            launch_pairs_mixed(l, c->s, c->prm, c->lattice, c->recf, c->order, c->list, c->count,
+                               tiles ? c->tile_idx : nullptr, tiles ? c->tile_cnt : nullptr, c->kmax, n);
+        });
+    } else {
+        timed(c, "pairs_fp64", [&] {
+            launch_pairs_fp64(l, c->s, c->prm, c->rec, c->group, c->order, c->list, c->count, c->lut, c->kmax, n);
+        });
+    }
     timed(c, "finalize", [&] { launch_finalize(l, c->s, c->mats_dev, c->prm, n); });
     if (stats) {
         std::vector<uint32_t> cnt(n);
@@ -559,6 +602,10 @@ int osph_create(osph_ctx** out, const osph_config* cfg, const osph_material* mat
     if (hipMalloc(&c->lut, grads.size() * sizeof(double)) != hipSuccess) return bail(OSPH_ERR_HIP);
     if (hipMemcpy(c->lut, grads.data(), grads.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return bail(OSPH_ERR_HIP);
+    std::vector<float> gradsf(grads.begin(), grads.end());
+    if (hipMalloc(&c->lutf, gradsf.size() * sizeof(float)) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (hipMemcpy(c->lutf, gradsf.data(), gradsf.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(OSPH_ERR_HIP);
     *out = c;
     return OSPH_OK;
 }
@@ -576,6 +623,7 @@ void osph_destroy(osph_ctx* c) {
     hipFree(c->mats_dev);
     hipFree(c->small);
     hipFree(c->lut);
+    hipFree(c->lutf);
     if (c->small_host) hipHostFree(c->small_host);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -599,7 +647,15 @@ int osph_resize(osph_ctx* c, size_t n) {
         HIPCHK(c, hipMalloc(&c->keys_sorted, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->iota, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->order, sizeof(uint32_t) * cap));
-        HIPCHK(c, hipMalloc(&c->rec, sizeof(PairRec) * cap));
+        if (c->cfg.precision == OSPH_PRECISION_MIXED) {
+            HIPCHK(c, hipMalloc(&c->recf, sizeof(PairRecF) * cap));
+            const size_t blocks = (cap + TILE_BLOCK - 1) / TILE_BLOCK;
+            HIPCHK(c, hipMalloc(&c->tile_idx, sizeof(uint32_t) * blocks * TILE_CAP));
+            HIPCHK(c, hipMalloc(&c->tile_cnt, sizeof(uint32_t) * blocks));
+        } else {
+            HIPCHK(c, hipMalloc(&c->rec, sizeof(PairRec) * cap));
+        }
+        HIPCHK(c, hipMalloc(&c->pos, sizeof(double4) * cap));
         HIPCHK(c, hipMalloc(&c->group, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->count, sizeof(uint32_t) * cap));
         c->sort_temp_bytes = sort_pairs_temp_bytes(cap);
@@ -678,17 +734,24 @@ int osph_get_pairs(osph_ctx* c, uint32_t* pi, uint32_t* pj, size_t cap, size_t* 
     if (!c || !count) return OSPH_ERR_ARG;
     if (!c->neighbors_valid) return c->fail(OSPH_ERR_STATE, "neighbours not built");
     const size_t n = c->n;
-    std::vector<uint32_t> cnt(n), order(n), list(n * size_t(c->kmax));
+    std::vector<uint32_t> cnt(n), order(n), list(n * size_t(c->kmax)), tiles;
+    const size_t blocks = (n + TILE_BLOCK - 1) / TILE_BLOCK;
     if (n) {
         HIPCHK(c, hipMemcpy(cnt.data(), c->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(order.data(), c->order, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(list.data(), c->list, list.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (c->tile_idx && c->tune.pair_tiles) {
+            tiles.resize(blocks * TILE_CAP);
+            HIPCHK(c, hipMemcpy(tiles.data(), c->tile_idx, tiles.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
     }
     std::vector<uint64_t> keys;
     for (size_t k = 0; k < n; ++k) {
         const uint64_t i = order[k];
         for (uint32_t m = 0; m < cnt[k] && m < c->kmax; ++m) {
-            const uint64_t j = order[list[size_t(m) * n + k]];
+            uint32_t e = list[size_t(k) * c->kmax + m];
+            if (!tiles.empty() && (e & LOCAL_BIT)) e = tiles[(k / TILE_BLOCK) * TILE_CAP + (e & ~LOCAL_BIT)];
+            const uint64_t j = order[e];
             if (i < j) keys.push_back((i << 32) | j);
         }
     }
@@ -733,6 +796,29 @@ int osph_step_pc(osph_ctx* c, double dt, double* next_dt, int* crit) {
                              c->stream));
     if (int e = sync_check(c)) return e;
     select_timestep(c, c->small_host + 8, next_dt, crit);
+    return OSPH_OK;
+}
+
+int osph_tune(osph_ctx* c, const char* key, long long value) {
+    if (!c || !key) return OSPH_ERR_ARG;
+    const std::string k(key);
+    const bool block_ok = value == 64 || value == 128 || value == 256;
+    const bool lds_ok = value >= 0 && value <= 65536;
+    if (k == "pairs_block" && block_ok) {
+        c->tune.pairs_block = int(value);
+    } else if (k == "lists_block" && block_ok) {
+        c->tune.lists_block = int(value);
+    } else if (k == "pairs_lds" && lds_ok) {
+        c->tune.pairs_lds = int(value);
+    } else if (k == "lists_lds" && lds_ok) {
+        c->tune.lists_lds = int(value);
+    } else if (k == "cell_divisor" && (value == 1 || value == 2 || value == 3)) {
+        c->tune.cell_divisor = int(value);
+    } else if (k == "pair_tiles" && (value == 0 || value == 1)) {
+        c->tune.pair_tiles = int(value);
+    } else {
+        return c->fail(OSPH_ERR_ARG, "unknown tuning key or value out of range: " + k);
+    }
     return OSPH_OK;
 }
 

@@ -6,7 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 
-#include "../../include/osph_gpu.h"
+#include "osph_gpu.h"
 
 namespace osph {
 
@@ -42,19 +42,39 @@ struct alignas(16) PairRec {
 };
 static_assert(sizeof(PairRec) == 128, "PairRec must be 128 bytes");
 
-// Mixed-precision neighbour record: cell-relative FP32 position, FP32 fields, 64 B.
+// Mixed-precision neighbour record, 96 B.  Positions are split into an integer lattice cell (10 bits
+// per axis, packed in `lat`) of power-of-two edge S and an FP32 offset inside that cell, so that
+// r_i - r_j = (L_i - L_j) S + (d_i - d_j) keeps FP32 relative precision at any distance from the origin.
+// Velocities stay FP64: bodies move at km/s with m/s internal differences, which FP32 would round away.
 struct alignas(16) PairRecF {
-    float x, y, z, h;
-    float vx, vy, vz, m;
-    float rho, cs, pr, pad;
-    float sxx, syy, sxy, sxz;
+    double vx, vy, vz;
+    float dx, dy, dz, h;
+    float m, rho, cs, pr;
+    float sxx, syy, sxy, sxz, syz;
+    uint32_t lat, group, pad0, pad1, pad2;
+};
+static_assert(sizeof(PairRecF) == 96, "PairRecF must be 96 bytes");
+
+// Neighbour tiles of the mixed-precision path: every block of TILE_BLOCK consecutive sorted particles
+// stages in LDS the records of its neighbours that lie within a window of TILE_WINDOW sorted positions
+// around the block (up to TILE_CAP of them, in sorted order).  A list entry with LOCAL_BIT set is a slot
+// in that tile; otherwise it is a sorted position read from global memory.
+constexpr int TILE_BLOCK = 256;
+constexpr int TILE_CAP = 1280;
+constexpr int TILE_WINDOW = 8192;                          // 256 bitmap words
+constexpr int TILE_BEFORE = (TILE_WINDOW - TILE_BLOCK) / 2;  // window start = block start - TILE_BEFORE
+constexpr uint32_t LOCAL_BIT = 0x80000000u;
+
+struct Lattice {
+    double lo[3];
+    double S;     // power of two
+    float Sf;
 };
 
 // Particles are binned into smoothing-length classes, class = min(nclass - 1, floor(log2(h / h_min))).
-// Each class has its own uniform cell grid (cell edge >= 2 h_max of the class) keyed by the Morton code
-// of the cell coordinates; the sort key is (class << 30) | morton.
+// Each class has its own uniform cell grid (cell edge 2 h_max(class) / divisor) keyed by the Morton code
+// of the cell coordinates; the sort key is (class << shift) | morton, shift = 3 * (largest bits per axis).
 constexpr int MAX_CLASSES = 4;
-constexpr int CLASS_SHIFT = 30;
 
 struct GridClass {
     double lo[3], hi[3];  // bounding box of the class members
@@ -68,6 +88,8 @@ struct GridClass {
 
 struct Grid {
     int nclass;
+    int shift;     // bit position of the class field in the sort key
+    int key_bits;  // bits of the sort key that can be nonzero
     double hmin;
     GridClass c[MAX_CLASSES];
 };
@@ -97,8 +119,21 @@ __host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) 
 }
 
 // ---- launchers implemented in osph_kernels.hip -------------------------------------------------------
+// Launch shapes of the pair-stage kernels, chosen by the tuner (osph_tune): threads per block and
+// dynamic LDS per block (which caps the number of resident blocks per CU, and with it the size of the
+// neighbour-record working set competing for each XCD's L2).
+struct Tuning {
+    int pairs_block = 256;
+    int pairs_lds = 0;
+    int lists_block = 256;
+    int lists_lds = 0;
+    int cell_divisor = 1;  // cells of 2 h_max(class) / divisor
+    int pair_tiles = 0;    // mixed precision: stage neighbour tiles in LDS
+};
+
 struct Launch {
     hipStream_t stream;
+    Tuning tune{};
 };
 
 void launch_material_init(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n);
@@ -107,13 +142,22 @@ void launch_class_bounds(const Launch& l, DevState s, int nclass, double hmin, u
 void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n);
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value);
 void launch_cell_ranges(const Launch& l, const uint32_t* keys, Grid g, uint32_t* cstart, uint32_t* cend, size_t n);
-void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, uint32_t* group,
-                 size_t n);
-void launch_neighbor_lists(const Launch& l, const PairRec* rec, Grid g, const uint32_t* cstart, const uint32_t* cend,
-                           uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow, size_t n);
+void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, double4* pos,
+                 uint32_t* group, size_t n);
+// Neighbour lists are row-major: list[k * kmax + c], kmax a multiple of 4.
+// With tiles != nullptr (mixed precision), also builds the per-block neighbour tiles: tile_idx[b * TILE_CAP + t]
+// holds the sorted position of slot t of block b, tile_cnt[b] the number of slots used.
+void launch_neighbor_lists(const Launch& l, const double4* pos, Grid g, const uint32_t* cstart, const uint32_t* cend,
+                           uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow, uint32_t* tile_idx,
+                           uint32_t* tile_cnt, size_t n);
 void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
                        uint32_t kmax, size_t n);
+void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
+                       double4* pos, uint32_t* group, size_t n);
+void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
+                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const uint32_t* tile_idx,
+                        const uint32_t* tile_cnt, uint32_t kmax, size_t n);
 void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n);
 void launch_zero_derivatives(const Launch& l, DevState s, size_t n);
 void launch_predict(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt, size_t n);
@@ -128,6 +172,6 @@ void launch_planes_to_stage(const Launch& l, unsigned char* staging, size_t stri
 void launch_stream_copy(hipStream_t stream, const void* a, void* b, size_t bytes);
 size_t sort_pairs_temp_bytes(size_t n);
 void sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                const uint32_t* vals_in, uint32_t* vals_out, size_t n);
+                const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits);
 
 }  // namespace osph

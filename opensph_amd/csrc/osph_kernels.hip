@@ -20,6 +20,16 @@ inline unsigned grid_for(size_t n, int block = BLOCK) {
     return unsigned(g > 0 ? g : 1);
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Renumber them so that XCD x
+// processes one contiguous range of logical blocks: consecutive sorted particles, and therefore their
+// shared neighbours, stay in one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
+    constexpr uint32_t NXCD = 8;
+    const uint32_t q = nblocks / NXCD, r = nblocks % NXCD;
+    const uint32_t x = bid % NXCD, local = bid / NXCD;
+    return x * q + (x < r ? x : r) + local;
+}
+
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }  // MathBasic.h:15-17
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }  // MathBasic.h:20-22
 
@@ -309,7 +319,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_keys(DevState s, Grid g, uint32_
     const int cls = h_class(s.d[H][i], g.hmin, g.nclass);
     int c[3];
     cell_of(g.c[cls], s.d[X][i], s.d[Y][i], s.d[Z][i], c);
-    keys[i] = (uint32_t(cls) << CLASS_SHIFT) | morton3(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
+    keys[i] = (uint32_t(cls) << g.shift) | morton3(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
     idx[i] = uint32_t(i);
 }
 
@@ -324,16 +334,17 @@ __global__ void __launch_bounds__(BLOCK) k_cell_ranges(const uint32_t* __restric
     const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t key = keys[k];
-    const uint32_t cell = g.c[key >> CLASS_SHIFT].base + (key & ((1u << CLASS_SHIFT) - 1u));
+    const uint32_t cell = g.c[key >> g.shift].base + (key & ((1u << g.shift) - 1u));
     if (k == 0 || keys[k - 1] != key) cstart[cell] = uint32_t(k);
     if (k == n - 1 || keys[k + 1] != key) cend[cell] = uint32_t(k + 1);
 }
 
 // Pack the neighbour-side data of particle order[k] into rec[k].
 __global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const uint32_t* __restrict__ order,
-                                                PairRec* __restrict__ rec, uint32_t* __restrict__ group, size_t n) {
+                                                PairRec* __restrict__ rec, double4* __restrict__ pos,
+                                                uint32_t* __restrict__ group, size_t n) {
 #pragma clang fp contract(off)
-    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t i = order[k];
     PairRec r;
@@ -360,6 +371,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const ui
         r.sxx = r.syy = r.sxy = r.sxz = r.syz = 0.0;
     }
     rec[k] = r;
+    pos[k] = make_double4(r.x, r.y, r.z, r.h);
     // DerivativeTemplate / AccelerationTemplate sum filter (DerivativeHelpers.h:132-139,262-269):
     // a pair enters the SUM_ONLY_UNDAMAGED sums iff both flags are equal and both reductions nonzero.
     uint32_t gid = 0;
@@ -382,53 +394,143 @@ __device__ __forceinline__ bool exact_pair(double xi, double yi, double zi, doub
 
 // One thread per particle (sorted position k).  For every smoothing-length class whose bounding box
 // lies within h_i + h_max(class), the cells of that class within that radius are scanned with the exact
-// pair test; neighbour positions go to list[c * n + k] (column-major, coalesced across the wavefront).
-__global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const PairRec* __restrict__ rec, Grid g,
+// pair test.  Neighbour positions go to the particle's row list[k * kmax + c] (kmax a multiple of 4),
+// buffered four at a time in registers and stored as one 16-byte write.
+//
+// With TILE (mixed precision), the block's neighbours inside its window of sorted positions are marked in
+// an LDS bitmap; after the scan a block prefix sum of the bitmap words numbers them in sorted order, the
+// first TILE_CAP become the block's tile (tile_idx), and every list entry in the tile is rewritten as
+// LOCAL_BIT | slot.  The block of logical index b covers sorted positions [b * 256, b * 256 + 256), the
+// same decomposition the tiled pair kernel uses.
+template <bool TILE>
+__global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restrict__ pos, Grid g,
                                                           const uint32_t* __restrict__ cstart,
                                                           const uint32_t* __restrict__ cend, uint32_t* __restrict__ list,
                                                           uint32_t* __restrict__ count, uint32_t kmax,
-                                                          uint32_t* overflow, size_t n) {
-    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double xi = rec[k].x, yi = rec[k].y, zi = rec[k].z, hi = rec[k].h;
-    const double p[3] = { xi, yi, zi };
+                                                          uint32_t* overflow, uint32_t* __restrict__ tile_idx,
+                                                          uint32_t* __restrict__ tile_cnt, size_t n) {
+    constexpr int WORDS = TILE_WINDOW / 32;
+    __shared__ uint32_t bits[TILE ? WORDS : 1];
+    __shared__ uint32_t scan[TILE ? BLOCK : 1];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const size_t k = size_t(blk) * blockDim.x + threadIdx.x;
+    // window [wlo, wlo + TILE_WINDOW) of sorted positions, as a signed offset from position 0
+    const long long wlo = (long long)blk * BLOCK - TILE_BEFORE;
+    if (TILE) {
+        for (int t = threadIdx.x; t < WORDS; t += blockDim.x) bits[t] = 0u;
+        __syncthreads();
+    }
     uint32_t cnt = 0;
-    for (int cls = 0; cls < g.nclass; ++cls) {
-        const GridClass& G = g.c[cls];
-        if (G.empty) continue;
-        const double R = hi + G.hmax;  // h_i + h_j <= R for every member j
-        double d2b = 0.0;
-        for (int a = 0; a < 3; ++a) {
-            const double t = dmax(G.lo[a] - p[a], p[a] - G.hi[a]);
-            if (t > 0.0) d2b += t * t;
-        }
-        if (d2b > R * R * (1.0 + 1e-6)) continue;
-        const int r = int(ceil(R * G.inv_cell * (1.0 + 1e-9)));
-        int lo[3], up[3];
-        for (int a = 0; a < 3; ++a) {
-            const int c = cell_coord(G, p[a], a);
-            lo[a] = c - r < 0 ? 0 : c - r;
-            up[a] = c + r >= G.dim[a] ? G.dim[a] - 1 : c + r;
-        }
-        for (int cz = lo[2]; cz <= up[2]; ++cz) {
-            for (int cy = lo[1]; cy <= up[1]; ++cy) {
-                for (int cx = lo[0]; cx <= up[0]; ++cx) {
-                    const uint32_t cell = G.base + morton3(uint32_t(cx), uint32_t(cy), uint32_t(cz));
-                    const uint32_t b = cstart[cell], e = cend[cell];
-                    for (uint32_t k2 = b; k2 < e; ++k2) {
-                        if (k2 == k) continue;
-                        const PairRec& o = rec[k2];
-                        if (exact_pair(xi, yi, zi, hi, o.x, o.y, o.z, o.h)) {
-                            if (cnt < kmax) list[size_t(cnt) * n + k] = k2;
-                            ++cnt;
+    if (k < n) {
+        const double4 me = pos[k];
+        const double p[3] = { me.x, me.y, me.z };
+        uint4* row = reinterpret_cast<uint4*>(list + size_t(k) * kmax);
+        uint32_t buf[4] = { 0, 0, 0, 0 };
+        for (int cls = 0; cls < g.nclass; ++cls) {
+            const GridClass& G = g.c[cls];
+            if (G.empty) continue;
+            const double R = me.w + G.hmax;  // h_i + h_j <= R for every member j
+            double d2b = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                const double t = dmax(G.lo[a] - p[a], p[a] - G.hi[a]);
+                if (t > 0.0) d2b += t * t;
+            }
+            if (d2b > R * R * (1.0 + 1e-6)) continue;
+            const int r = int(ceil(R * G.inv_cell * (1.0 + 1e-9)));
+            int lo[3], up[3];
+            for (int a = 0; a < 3; ++a) {
+                const int c = cell_coord(G, p[a], a);
+                lo[a] = c - r < 0 ? 0 : c - r;
+                up[a] = c + r >= G.dim[a] ? G.dim[a] - 1 : c + r;
+            }
+            // cells whose box lies farther than R (with a relative margin) hold no neighbour
+            const double R2 = R * R * (1.0 + 1e-6);
+            for (int cz = lo[2]; cz <= up[2]; ++cz) {
+                const double tz = dmax(dmax(G.lo[2] + cz * G.cell - p[2], p[2] - (G.lo[2] + (cz + 1) * G.cell)), 0.0);
+                for (int cy = lo[1]; cy <= up[1]; ++cy) {
+                    const double ty = dmax(dmax(G.lo[1] + cy * G.cell - p[1], p[1] - (G.lo[1] + (cy + 1) * G.cell)), 0.0);
+                    const double dzy = tz * tz + ty * ty;
+                    if (dzy > R2) continue;
+                    for (int cx = lo[0]; cx <= up[0]; ++cx) {
+                        const double tx =
+                            dmax(dmax(G.lo[0] + cx * G.cell - p[0], p[0] - (G.lo[0] + (cx + 1) * G.cell)), 0.0);
+                        if (dzy + tx * tx > R2) continue;
+                        const uint32_t cell = G.base + morton3(uint32_t(cx), uint32_t(cy), uint32_t(cz));
+                        const uint32_t b = cstart[cell], e = cend[cell];
+                        for (uint32_t k2 = b; k2 < e; ++k2) {
+                            if (k2 == k) continue;
+                            const double4 o = pos[k2];
+                            if (exact_pair(me.x, me.y, me.z, me.w, o.x, o.y, o.z, o.w)) {
+                                buf[cnt & 3u] = k2;
+                                ++cnt;
+                                if ((cnt & 3u) == 0 && cnt <= kmax) {
+                                    row[(cnt >> 2) - 1] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+                                }
+                                if (TILE) {
+                                    const long long off = (long long)k2 - wlo;
+                                    if (off >= 0 && off < TILE_WINDOW) {
+                                        atomicOr(&bits[off >> 5], 1u << (off & 31));
+                                    }
+                                }
+                            }
                         }
                     }
                 }
             }
         }
+        if ((cnt & 3u) != 0 && cnt < kmax) {
+            row[cnt >> 2] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+        }
+        count[k] = cnt;
+        if (cnt > kmax) atomicOr(overflow, 1u);
     }
-    count[k] = cnt;
-    if (cnt > kmax) atomicOr(overflow, 1u);
+    if (!TILE) return;
+    __syncthreads();
+    // number the marked positions in sorted order: exclusive prefix sum of the word popcounts
+    static_assert(WORDS == BLOCK, "one bitmap word per thread");
+    const uint32_t word = bits[threadIdx.x];
+    const uint32_t mine = __popc(word);
+    scan[threadIdx.x] = mine;
+    __syncthreads();
+    for (int off = 1; off < BLOCK; off <<= 1) {
+        const uint32_t add = threadIdx.x >= off ? scan[threadIdx.x - off] : 0u;
+        __syncthreads();
+        scan[threadIdx.x] += add;
+        __syncthreads();
+    }
+    const uint32_t base = scan[threadIdx.x] - mine;  // exclusive prefix of this word
+    {
+        uint32_t w = word, slot = base;
+        while (w) {
+            const int b = __ffs(w) - 1;
+            w &= w - 1u;
+            if (slot < uint32_t(TILE_CAP)) {
+                tile_idx[size_t(blk) * TILE_CAP + slot] = uint32_t(wlo + (long long)threadIdx.x * 32 + b);
+            }
+            ++slot;
+        }
+    }
+    if (threadIdx.x == BLOCK - 1) tile_cnt[blk] = scan[BLOCK - 1] < uint32_t(TILE_CAP) ? scan[BLOCK - 1] : TILE_CAP;
+    __syncthreads();
+    // turn scan[] into the exclusive prefix per word for the lookups below
+    const uint32_t excl = base;
+    __syncthreads();
+    scan[threadIdx.x] = excl;
+    __syncthreads();
+    // rewrite this particle's entries that live in the tile
+    if (k < n) {
+        const uint32_t listed = cnt < kmax ? cnt : kmax;
+        uint32_t* row = list + size_t(k) * kmax;
+        for (uint32_t c = 0; c < listed; ++c) {
+            const uint32_t k2 = row[c];
+            const long long off = (long long)k2 - wlo;
+            if (off >= 0 && off < TILE_WINDOW) {
+                const uint32_t wi = uint32_t(off >> 5), b = uint32_t(off & 31);
+                const uint32_t slot = scan[wi] + __popc(bits[wi] & ((1u << b) - 1u));
+                if (slot < uint32_t(TILE_CAP)) row[c] = LOCAL_BIT | slot;
+            }
+        }
+    }
 }
 
 // LutKernel<3>::gradImpl (core/sph/kernel/Kernel.h:129-144): linear interpolation in q^2.
@@ -456,7 +558,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
                                                       const uint32_t* __restrict__ list,
                                                       const uint32_t* __restrict__ count,
                                                       const double* __restrict__ lut, uint32_t kmax, size_t n) {
-    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const PairRec me = rec[k];
     const uint32_t gi = group[k];
@@ -466,8 +568,12 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
     const uint32_t cnt = count[k];
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
+    const uint4* row = reinterpret_cast<const uint4*>(list + size_t(k) * kmax);
+    uint4 quad = make_uint4(0, 0, 0, 0);
     for (uint32_t c = 0; c < listed; ++c) {
-        const uint32_t k2 = list[size_t(c) * n + k];
+        if ((c & 3u) == 0) quad = row[c >> 2];
+        const uint32_t sel = c & 3u;
+        const uint32_t k2 = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
         const PairRec o = rec[k2];
         const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
         const double hbar = 0.5 * (me.h + o.h);
@@ -528,6 +634,197 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
         s.d[GXY][i] = gxy;
         s.d[GXZ][i] = gxz;
         s.d[GYZ][i] = gyz;
+    }
+    s.u[NCNT][i] = cnt;
+}
+
+// ---- mixed precision ---------------------------------------------------------------------------------
+
+// Packs the mixed-precision neighbour record of particle order[k] (and the exact FP64 position used by
+// the neighbour-list build).
+__global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, Lattice L,
+                                                      const uint32_t* __restrict__ order, PairRecF* __restrict__ rec,
+                                                      double4* __restrict__ pos, uint32_t* __restrict__ group,
+                                                      size_t n) {
+    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = order[k];
+    const double x[3] = { s.d[X][i], s.d[Y][i], s.d[Z][i] };
+    const double h = s.d[H][i];
+    PairRecF r;
+    uint32_t c[3];
+    float d[3];
+    for (int a = 0; a < 3; ++a) {
+        double f = floor((x[a] - L.lo[a]) / L.S);
+        f = f < 0.0 ? 0.0 : (f > 1023.0 ? 1023.0 : f);
+        c[a] = uint32_t(f);
+        d[a] = float(x[a] - (L.lo[a] + f * L.S));
+    }
+    r.vx = s.d[VX][i];
+    r.vy = s.d[VY][i];
+    r.vz = s.d[VZ][i];
+    r.dx = d[0];
+    r.dy = d[1];
+    r.dz = d[2];
+    r.h = float(h);
+    r.m = float(s.d[M][i]);
+    const double rho = s.d[RHO][i];
+    const double rho2 = rho * rho;
+    r.rho = float(rho);
+    r.cs = float(s.d[CS][i]);
+    r.pr = float(s.d[P][i] / rho2);
+    if (prm.solid_stress) {
+        r.sxx = float(s.d[SXX][i] / rho2);
+        r.syy = float(s.d[SYY][i] / rho2);
+        r.sxy = float(s.d[SXY][i] / rho2);
+        r.sxz = float(s.d[SXZ][i] / rho2);
+        r.syz = float(s.d[SYZ][i] / rho2);
+    } else {
+        r.sxx = r.syy = r.sxy = r.sxz = r.syz = 0.f;
+    }
+    r.lat = c[0] | (c[1] << 10) | (c[2] << 20);
+    uint32_t gid = 0;
+    if (prm.sum_only_undamaged && prm.has_reduce) {
+        gid = (s.d[RED][i] == 0.0) ? NO_GROUP : s.u[FLAG][i];
+    }
+    r.group = gid;
+    r.pad0 = r.pad1 = r.pad2 = 0u;
+    rec[k] = r;
+    pos[k] = make_double4(x[0], x[1], x[2], h);
+    group[k] = gid;
+}
+
+// M4 gradient factor grad W / q of CubicSpline<3> (core/sph/kernel/Kernel.h:172-187) in closed form,
+// FP32: for q < 1 the bracket divided by q simplifies to (2.25 q - 3) / pi (finite at q = 0, where the
+// reference returns -3 / pi); for 1 <= q < 2 it is -0.75 (2 - q)^2 / (pi q).
+__device__ __forceinline__ float m4_grad_f(float q2) {
+    constexpr float INV_PI = 0.318309886183790671538f;
+    if (!(q2 < 4.0f)) return 0.0f;
+    const float q = sqrtf(q2);
+    if (q < 1.0f) return INV_PI * (2.25f * q - 3.0f);
+    const float t = 2.0f - q;
+    return -0.75f * INV_PI * (t * t) / q;
+}
+
+// Kahan-compensated FP32 accumulator.
+struct Kahan {
+    float sum = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float v) {
+        const float y = v - c;
+        const float t = sum + y;
+        c = (t - sum) - y;
+        sum = t;
+    }
+    __device__ __forceinline__ double value() const { return double(sum) - double(c); }
+};
+
+// Mixed-precision pair sums: the same terms and gather form as k_pairs_fp64, evaluated in FP32 on
+// lattice-relative positions with Kahan-compensated FP32 accumulators; results are written in FP64.
+// With TILE the block first stages its neighbour tile (built by the list kernel) in LDS; list entries with
+// LOCAL_BIT read the tile, the others read the record from global memory.
+template <bool TILE>
+__global__ void __launch_bounds__(TILE_BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
+                                                            const PairRecF* __restrict__ rec,
+                                                            const uint32_t* __restrict__ order,
+                                                            const uint32_t* __restrict__ list,
+                                                            const uint32_t* __restrict__ count,
+                                                            const uint32_t* __restrict__ tile_idx,
+                                                            const uint32_t* __restrict__ tile_cnt, uint32_t kmax,
+                                                            size_t n) {
+    __shared__ PairRecF tile[TILE ? TILE_CAP : 1];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    if (TILE) {
+        const uint32_t nt = tile_cnt[blk];
+        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) tile[t] = rec[tile_idx[size_t(blk) * TILE_CAP + t]];
+        __syncthreads();
+    }
+    const size_t k = size_t(blk) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const PairRecF me = rec[k];
+    const uint32_t gi = me.group;
+    const int lxi = int(me.lat & 1023u), lyi = int((me.lat >> 10) & 1023u), lzi = int(me.lat >> 20);
+    const float rhoInv = 1.0f / me.rho;
+    const float alpha = float(prm.av_alpha), beta = float(prm.av_beta);
+    Kahan ax, ay, az, du, divv, gxx, gyy, gzz, gxy, gxz, gyz;
+    const uint32_t cnt = count[k];
+    const uint32_t listed = cnt < kmax ? cnt : kmax;
+    const bool solid = prm.solid_stress != 0;
+    const uint4* row = reinterpret_cast<const uint4*>(list + size_t(k) * kmax);
+    // the next quad of list entries is loaded one quad ahead (hides the global-memory latency)
+    uint4 quad = listed > 0 ? row[0] : make_uint4(0, 0, 0, 0);
+    uint4 next = listed > 4 ? row[1] : make_uint4(0, 0, 0, 0);
+    for (uint32_t c = 0; c < listed; ++c) {
+        const uint32_t sel = c & 3u;
+        if (sel == 0 && c > 0) {
+            quad = next;
+            next = (c + 4 < listed) ? row[(c >> 2) + 1] : make_uint4(0, 0, 0, 0);
+        }
+        const uint32_t e = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
+        PairRecF o;
+        if (TILE && (e & LOCAL_BIT)) {
+            o = tile[e & ~LOCAL_BIT];
+        } else {
+            o = rec[e];
+        }
+        const int lxj = int(o.lat & 1023u), lyj = int((o.lat >> 10) & 1023u), lzj = int(o.lat >> 20);
+        const float rx = float(lxi - lxj) * L.Sf + (me.dx - o.dx);
+        const float ry = float(lyi - lyj) * L.Sf + (me.dy - o.dy);
+        const float rz = float(lzi - lzj) * L.Sf + (me.dz - o.dz);
+        const float hbar = 0.5f * (me.h + o.h);
+        const float hInv = 1.0f / hbar;
+        const float hInv2 = hInv * hInv;
+        const float d2 = rx * rx + ry * ry + rz * rz;
+        const float gk = m4_grad_f(d2 * hInv2);
+        const float h5g = (hInv2 * hInv2) * hInv * gk;
+        const float gx = rx * h5g, gy = ry * h5g, gz = rz * h5g;
+        const float wx = float(o.vx - me.vx), wy = float(o.vy - me.vy), wz = float(o.vz - me.vz);
+        const float mj = o.m;
+        const float mjr = mj * rhoInv;
+        const float wg = wx * gx + wy * gy + wz * gz;
+        divv.add(mjr * wg);
+        const float coef = me.pr + o.pr;
+        float fx = -(coef * gx), fy = -(coef * gy), fz = -(coef * gz);
+        if (solid && gi == o.group && gi != NO_GROUP) {
+            gxx.add(mjr * (wx * gx));
+            gyy.add(mjr * (wy * gy));
+            gzz.add(mjr * (wz * gz));
+            gxy.add(mjr * (0.5f * (wx * gy + wy * gx)));
+            gxz.add(mjr * (0.5f * (wx * gz + wz * gx)));
+            gyz.add(mjr * (0.5f * (wy * gz + wz * gy)));
+            const float xx = me.sxx + o.sxx, yy = me.syy + o.syy, xy = me.sxy + o.sxy;
+            const float xz = me.sxz + o.sxz, yz = me.syz + o.syz;
+            fx += xx * gx + xy * gy + xz * gz;
+            fy += xy * gx + yy * gy + yz * gz;
+            fz += xz * gx + yz * gy + (-xx - yy) * gz;
+        }
+        const float dvdr = -(wx * rx + wy * ry + wz * rz);
+        if (dvdr < 0.0f) {
+            const float rhobar = 0.5f * (me.rho + o.rho);
+            const float csbar = 0.5f * (me.cs + o.cs);
+            const float mu = hbar * dvdr / (d2 + 1.e-2f * (hbar * hbar));
+            const float av = (-alpha * csbar * mu + beta * (mu * mu)) / rhobar;
+            fx -= av * gx;
+            fy -= av * gy;
+            fz -= av * gz;
+            du.add(mj * (0.5f * av * -wg));
+        }
+        ax.add(mj * fx);
+        ay.add(mj * fy);
+        az.add(mj * fz);
+    }
+    const uint32_t i = order[k];
+    s.d[AX][i] = ax.value();
+    s.d[AY][i] = ay.value();
+    s.d[AZ][i] = az.value();
+    s.d[DU][i] = du.value();
+    s.d[DIVV][i] = divv.value();
+    if (solid) {
+        s.d[GXX][i] = gxx.value();
+        s.d[GYY][i] = gyy.value();
+        s.d[GZZ][i] = gzz.value();
+        s.d[GXY][i] = gxy.value();
+        s.d[GXZ][i] = gxz.value();
+        s.d[GYZ][i] = gyz.value();
     }
     s.u[NCNT][i] = cnt;
 }
@@ -870,23 +1167,54 @@ void launch_class_bounds(const Launch& l, DevState s, int nclass, double hmin, u
     k_class_bounds<<<g, BLOCK, 0, l.stream>>>(s, nclass, hmin, acc, n);
 }
 
-void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, uint32_t* group,
-                 size_t n) {
+void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, double4* pos,
+                 uint32_t* group, size_t n) {
     if (!n) return;
-    k_pack<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, order, rec, group, n);
+    k_pack<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, order, rec, pos, group, n);
 }
 
-void launch_neighbor_lists(const Launch& l, const PairRec* rec, Grid g, const uint32_t* cstart, const uint32_t* cend,
-                           uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow, size_t n) {
+void launch_neighbor_lists(const Launch& l, const double4* pos, Grid g, const uint32_t* cstart, const uint32_t* cend,
+                           uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow, uint32_t* tile_idx,
+                           uint32_t* tile_cnt, size_t n) {
     if (!n) return;
-    k_neighbor_lists<<<grid_for(n), BLOCK, 0, l.stream>>>(rec, g, cstart, cend, list, count, kmax, overflow, n);
+    if (tile_idx) {
+        // the tile decomposition fixes the block size
+        k_neighbor_lists<true><<<grid_for(n, TILE_BLOCK), TILE_BLOCK, 0, l.stream>>>(
+            pos, g, cstart, cend, list, count, kmax, overflow, tile_idx, tile_cnt, n);
+        return;
+    }
+    const int b = l.tune.lists_block;
+    k_neighbor_lists<false><<<grid_for(n, b), b, size_t(l.tune.lists_lds), l.stream>>>(
+        pos, g, cstart, cend, list, count, kmax, overflow, nullptr, nullptr, n);
 }
 
 void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
                        uint32_t kmax, size_t n) {
     if (!n) return;
-    k_pairs_fp64<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n);
+    const int b = l.tune.pairs_block;
+    k_pairs_fp64<<<grid_for(n, b), b, size_t(l.tune.pairs_lds), l.stream>>>(s, prm, rec, group, order, list, count,
+                                                                              lut_grads, kmax, n);
+}
+
+void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
+                       double4* pos, uint32_t* group, size_t n) {
+    if (!n) return;
+    k_pack_mixed<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, lat, order, rec, pos, group, n);
+}
+
+void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
+                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const uint32_t* tile_idx,
+                        const uint32_t* tile_cnt, uint32_t kmax, size_t n) {
+    if (!n) return;
+    if (tile_idx) {
+        k_pairs_mixed<true><<<grid_for(n, TILE_BLOCK), TILE_BLOCK, 0, l.stream>>>(s, prm, lat, rec, order, list, count,
+                                                                                  tile_idx, tile_cnt, kmax, n);
+    } else {
+        const int b = l.tune.pairs_block;
+        k_pairs_mixed<false><<<grid_for(n, b), b, size_t(l.tune.pairs_lds), l.stream>>>(
+            s, prm, lat, rec, order, list, count, nullptr, nullptr, kmax, n);
+    }
 }
 
 void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n) {
@@ -949,10 +1277,10 @@ size_t sort_pairs_temp_bytes(size_t n) {
 }
 
 void sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                const uint32_t* vals_in, uint32_t* vals_out, size_t n) {
+                const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits) {
     if (!n) return;
-    (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, int(n), 0, 32,
-                                             l.stream);
+    (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, int(n), 0,
+                                             key_bits, l.stream);
 }
 
 }  // namespace osph

@@ -28,7 +28,7 @@ PRECISION = {"fp64": 0, "mixed": 1}
 EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info", "osph_resize",
            "osph_set_quantity", "osph_get_quantity", "osph_zero_derivatives", "osph_build_neighbors",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
-           "osph_stream_copy",
+           "osph_stream_copy", "osph_tune",
            "osph_profile_enable", "osph_profile_read"]
 
 
@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_sync.argtypes = [vp]
     L.osph_kernel_lut.argtypes = [dp, dp]
     L.osph_stream_copy.argtypes = [sz, i, dp]
+    L.osph_tune.argtypes = [vp, C.c_char_p, C.c_longlong]
     L.osph_profile_enable.argtypes = [vp, i]
     L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
     _lib = L
@@ -257,6 +258,11 @@ class GpuSolver:
                                           pj.ctypes.data_as(C.POINTER(C.c_uint32)), cnt.value, C.byref(cnt)),
                     "osph_get_pairs")
         return pi, pj
+
+    def tune(self, **kw):
+        """Launch shapes of the list and pair kernels (see osph_tune)."""
+        for k, v in kw.items():
+            self._check(self.L.osph_tune(self.ctx, k.encode(), int(v)), f"osph_tune({k})")
 
     def sync(self):
         self._check(self.L.osph_sync(self.ctx), "osph_sync")

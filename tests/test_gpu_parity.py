@@ -47,14 +47,17 @@ def test_pairs_bit_exact(oracle, name):
     assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
 
 
-def test_pairs_mixed_smoothing_lengths(oracle):
-    """Four smoothing-length classes (h spans a factor 16): every class grid and cross-class search."""
+@pytest.mark.parametrize("divisor", [1, 2, 3])
+def test_pairs_mixed_smoothing_lengths(oracle, divisor):
+    """Four smoothing-length classes (h spans a factor 16): every class grid and cross-class search, with
+    neighbour cells of 2 h_max / divisor."""
     rng = np.random.default_rng(5)
     st, cfg = initial.fluid_sphere(n_requested=3000, radius=1000.0)
     st["r"][:, :3] += rng.normal(0, 10.0, size=(st.n, 3))
     st["r"][:, 3] *= np.exp2(rng.uniform(-2.0, 2.0, size=st.n))
-    _print_case("mixed_h", st)
+    _print_case(f"mixed_h/{divisor}", st)
     g = gpu.GpuSolver(st, cfg)
+    g.tune(cell_divisor=divisor)
     pi, pj = g.pairs()
     oi, oj = oracle.pairs(st["r"])
     assert len(pi) > 0
