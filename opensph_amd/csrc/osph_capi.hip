@@ -44,20 +44,21 @@ struct osph_ctx {
     size_t cell_cap = 0;
     PairRec* rec = nullptr;
     PairRecF* recf = nullptr;
-    uint32_t* tile_idx = nullptr;  // mixed precision: per-block neighbour tiles
-    uint32_t* tile_cnt = nullptr;
     Lattice lattice{};
     double4* pos = nullptr;
+    float4* posf = nullptr;  // origin-relative FP32 positions of the screened list kernel
     uint32_t* group = nullptr;
     uint32_t* list = nullptr;
     uint32_t* count = nullptr;
     uint32_t kmax = 96;
     size_t list_cap = 0;
-    unsigned long long* small = nullptr;     // [0..7] bounds, [8..11] criteria, [12] overflow, [16..47] classes
+    unsigned long long* small = nullptr;     // accumulators, layout SMALL_* in osph_internal.h
     unsigned long long* small_host = nullptr;  // pinned mirror
     double* lut = nullptr;
     float* lutf = nullptr;
     Grid grid{};
+    Screen screen{};
+    double href = 0.0;  // class reference h of the next neighbour search (0: measure it first)
     bool neighbors_valid = false;
 
     unsigned char* staging = nullptr;
@@ -106,9 +107,9 @@ void timed(osph_ctx* c, const char* name, F&& f) {
         return;
     }
     ProfEvent ev{ name, take_event(c), take_event(c) };
-    if (ev.start) hipEventRecord(ev.start, c->stream);
+    if (ev.start) (void)hipEventRecord(ev.start, c->stream);
     f();
-    if (ev.stop) hipEventRecord(ev.stop, c->stream);
+    if (ev.stop) (void)hipEventRecord(ev.stop, c->stream);
     c->events.push_back(ev);
 }
 
@@ -130,27 +131,26 @@ void resolve_events(osph_ctx* c) {
 Launch L(osph_ctx* c) { return Launch{ c->stream, c->tune }; }
 
 void free_state(osph_ctx* c) {
-    hipFree(c->dblock);
-    hipFree(c->ublock);
-    hipFree(c->keys);
-    hipFree(c->keys_sorted);
-    hipFree(c->iota);
-    hipFree(c->order);
-    hipFree(c->sort_temp);
-    hipFree(c->rec);
-    hipFree(c->recf);
-    hipFree(c->tile_idx);
-    hipFree(c->tile_cnt);
-    hipFree(c->pos);
-    hipFree(c->group);
-    hipFree(c->count);
+    (void)hipFree(c->dblock);
+    (void)hipFree(c->ublock);
+    (void)hipFree(c->keys);
+    (void)hipFree(c->keys_sorted);
+    (void)hipFree(c->iota);
+    (void)hipFree(c->order);
+    (void)hipFree(c->sort_temp);
+    (void)hipFree(c->rec);
+    (void)hipFree(c->recf);
+    (void)hipFree(c->posf);
+    (void)hipFree(c->pos);
+    (void)hipFree(c->group);
+    (void)hipFree(c->count);
     c->dblock = nullptr;
     c->ublock = nullptr;
     c->keys = c->keys_sorted = c->iota = c->order = nullptr;
     c->sort_temp = nullptr;
     c->rec = nullptr;
     c->recf = nullptr;
-    c->tile_idx = c->tile_cnt = nullptr;
+    c->posf = nullptr;
     c->pos = nullptr;
     c->group = c->count = nullptr;
     c->cap = 0;
@@ -158,7 +158,7 @@ void free_state(osph_ctx* c) {
 
 int ensure_staging(osph_ctx* c, size_t bytes) {
     if (bytes <= c->staging_cap) return OSPH_OK;
-    hipFree(c->staging);
+    (void)hipFree(c->staging);
     c->staging = nullptr;
     c->staging_cap = 0;
     HIPCHK(c, hipMalloc(&c->staging, bytes));
@@ -168,7 +168,7 @@ int ensure_staging(osph_ctx* c, size_t bytes) {
 
 int ensure_list(osph_ctx* c, size_t entries) {
     if (entries <= c->list_cap) return OSPH_OK;
-    hipFree(c->list);
+    (void)hipFree(c->list);
     c->list = nullptr;
     c->list_cap = 0;
     HIPCHK(c, hipMalloc(&c->list, entries * sizeof(uint32_t)));
@@ -178,8 +178,8 @@ int ensure_list(osph_ctx* c, size_t entries) {
 
 int ensure_cells(osph_ctx* c, size_t cells) {
     if (cells <= c->cell_cap) return OSPH_OK;
-    hipFree(c->cstart);
-    hipFree(c->cend);
+    (void)hipFree(c->cstart);
+    (void)hipFree(c->cend);
     c->cstart = c->cend = nullptr;
     c->cell_cap = 0;
     HIPCHK(c, hipMalloc(&c->cstart, cells * sizeof(uint32_t)));
@@ -322,69 +322,87 @@ int sync_check(osph_ctx* c) {
 }
 
 // Grid + sort + pack + neighbour lists for the current positions (ISymmetricFinder::build).
-int build_neighbors(osph_ctx* c) {
-    const size_t n = c->n;
-    c->neighbors_valid = false;
-    if (n == 0) {
-        c->neighbors_valid = true;
-        return OSPH_OK;
-    }
-    Launch l = L(c);
-    uint32_t* acc32 = reinterpret_cast<uint32_t*>(c->small);
-    timed(c, "init_small", [&] {
-        launch_clear_u32(l, acc32, 6, 0xffffffffu);        // min x, y, z
-        launch_clear_u32(l, acc32 + 6, 8, 0u);             // max x, y, z, max h
-        launch_clear_u32(l, acc32 + 14, 2, 0xffffffffu);   // min h
-        launch_clear_u32(l, acc32 + 24, 2, 0u);            // overflow flag
-    });
-    timed(c, "bounds", [&] { launch_bounds_reduce(l, c->s, c->small, n); });
-    HIPCHK(c, hipMemcpyAsync(c->small_host, c->small, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             c->stream));
-    if (int e = sync_check(c)) return e;
-    double lo[3], hi[3];
+// FP32 screen of the neighbour test.  Coordinates are rounded once to FP32 relative to lo, so each
+// carries an absolute error below eps D (D the largest extent, eps = 2^-24) and a difference below
+// 2 eps D + eps |dx|.  Near the cutoff (|d| ~ R >= 2 h_min) the squared distance then has a relative
+// error below 2 sqrt(3) eps (D / h_min + 1) plus a few eps of rounding, and the squared radius below
+// 6 eps.  The band is twice that sum; when it exceeds 1e-2 the screen is not worth running.
+Screen make_screen(const osph_ctx* c, const double lo[3], const double hi[3], double hmin) {
+    Screen sc{};
+    const int mode = c->tune.lists_screen;
+    sc.on = mode == 1 || (mode == -1 && c->cfg.precision == OSPH_PRECISION_MIXED);
+    double D = 0.0;
     for (int k = 0; k < 3; ++k) {
-        lo[k] = decode_ord(c->small_host[k]);
-        hi[k] = decode_ord(c->small_host[3 + k]);
-        if (!std::isfinite(lo[k]) || !std::isfinite(hi[k])) return c->fail(OSPH_ERR_STATE, "non-finite positions");
+        sc.o[k] = lo[k];
+        D = std::max(D, hi[k] - lo[k]);
     }
-    const double hmax = decode_ord(c->small_host[6]);
-    const double hmin = decode_ord(c->small_host[7]);
-    if (!(hmin > 0.0) || !std::isfinite(hmax)) return c->fail(OSPH_ERR_STATE, "smoothing lengths must be positive");
+    const double eps = std::ldexp(1.0, -24);
+    const double band = 2.0 * (2.0 * std::sqrt(3.0) * eps * (D / hmin + 1.0) + 12.0 * eps);
+    if (!(band < 1e-2)) sc.on = 0;
+    sc.lo = float(1.0 - band);
+    sc.hi = float(1.0 + band);
+    return sc;
+}
 
-    Grid g{};
-    g.hmin = hmin;
-    const double ratio = hmax / hmin;
-    g.nclass = ratio < 2.0 ? 1 : std::min(MAX_CLASSES, 1 + int(std::floor(std::log2(ratio))));
-    double clo[MAX_CLASSES][3], chi[MAX_CLASSES][3], chmax[MAX_CLASSES];
-    unsigned long long ccount[MAX_CLASSES] = { 0, 0, 0, 0 };
-    if (g.nclass == 1) {
-        for (int k = 0; k < 3; ++k) {
-            clo[0][k] = lo[k];
-            chi[0][k] = hi[k];
-        }
-        chmax[0] = hmax;
-        ccount[0] = n;
-    } else {
-        unsigned long long* cacc = c->small + 16;
-        uint32_t* cacc32 = reinterpret_cast<uint32_t*>(cacc);
-        for (int cl = 0; cl < MAX_CLASSES; ++cl) {
-            launch_clear_u32(l, cacc32 + 16 * cl, 6, 0xffffffffu);
-            launch_clear_u32(l, cacc32 + 16 * cl + 6, 10, 0u);
-        }
-        timed(c, "class_bounds", [&] { launch_class_bounds(l, c->s, g.nclass, hmin, cacc, n); });
-        HIPCHK(c, hipMemcpyAsync(c->small_host + 16, cacc, 8 * MAX_CLASSES * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, c->stream));
+// The pass before the neighbour search (k_advance: predictions for mode 2, the material initialize for
+// modes 1 and 2) with the per-class grid bounds, then the grid, the sort by cell, the cell ranges and
+// the packed neighbour records (ISymmetricFinder::build).  One host synchronisation reads the class
+// bounds; the first search after an upload adds one more for the class reference h.
+int search_grid(osph_ctx* c, int mode, double dt) {
+    const size_t n = c->n;
+    Launch l = L(c);
+    timed(c, "init_small", [&] { launch_init_small(l, c->small); });
+    if (!(c->href > 0.0)) {
+        timed(c, "bounds", [&] { launch_bounds_reduce(l, c->s, c->small, n); });
+        HIPCHK(c, hipMemcpyAsync(c->small_host, c->small, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 c->stream));
         if (int e = sync_check(c)) return e;
-        for (int cl = 0; cl < g.nclass; ++cl) {
-            const unsigned long long* a = c->small_host + 16 + 8 * cl;
-            for (int k = 0; k < 3; ++k) {
-                clo[cl][k] = decode_ord(a[k]);
-                chi[cl][k] = decode_ord(a[3 + k]);
-            }
-            chmax[cl] = decode_ord(a[6]);
-            ccount[cl] = a[7];
-        }
+        const double h0 = decode_ord(c->small_host[7]);
+        if (!(h0 > 0.0) || !std::isfinite(h0)) return c->fail(OSPH_ERR_STATE, "smoothing lengths must be positive");
+        c->href = h0;
     }
+    const double href = c->href;
+    unsigned long long* cacc = c->small + SMALL_CLASSES;
+    timed(c, mode == 2 ? "advance" : (mode == 1 ? "material_init" : "class_bounds"),
+          [&] { launch_advance(l, c->s, c->mats_dev, c->prm, mode, dt, href, cacc, n); });
+    HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CLASSES, cacc, CLASS_STRIDE * MAX_CLASSES * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, c->stream));
+    if (int e = sync_check(c)) return e;
+
+    double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    double hmin = INFINITY, hmax = 0.0;
+    double clo[MAX_CLASSES][3], chi[MAX_CLASSES][3], chmax[MAX_CLASSES];
+    unsigned long long ccount[MAX_CLASSES];
+    int last = 0;
+    for (int cl = 0; cl < MAX_CLASSES; ++cl) {
+        const unsigned long long* a = c->small_host + SMALL_CLASSES + CLASS_STRIDE * cl;
+        ccount[cl] = a[8];
+        if (!ccount[cl]) continue;
+        last = cl;
+        for (int k = 0; k < 3; ++k) {
+            clo[cl][k] = decode_ord(a[k]);
+            chi[cl][k] = decode_ord(a[3 + k]);
+            if (!std::isfinite(clo[cl][k]) || !std::isfinite(chi[cl][k])) {
+                return c->fail(OSPH_ERR_STATE, "non-finite positions");
+            }
+            lo[k] = std::min(lo[k], clo[cl][k]);
+            hi[k] = std::max(hi[k], chi[cl][k]);
+        }
+        chmax[cl] = decode_ord(a[6]);
+        const double chmin = decode_ord(a[7]);
+        if (!(chmin > 0.0) || !std::isfinite(chmax[cl])) {
+            return c->fail(OSPH_ERR_STATE, "smoothing lengths must be positive");
+        }
+        hmin = std::min(hmin, chmin);
+        hmax = std::max(hmax, chmax[cl]);
+    }
+    c->href = hmin;  // class reference of the next search
+
+    const Screen sc = make_screen(c, lo, hi, hmin);
+    c->screen = sc;
+    Grid g{};
+    g.hmin = href;
+    g.nclass = last + 1;
     uint32_t base = 0;
     for (int cl = 0; cl < g.nclass; ++cl) {
         GridClass& G = g.c[cl];
@@ -442,52 +460,88 @@ int build_neighbors(osph_ctx* c) {
         lat.S = S;
         lat.Sf = float(S);
         c->lattice = lat;
-        timed(c, "pack", [&] { launch_pack_mixed(l, c->s, c->prm, lat, c->order, c->recf, c->pos, c->group, n); });
-    } else {
-        timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->pos, c->group, n); });
-    }
-    uint32_t* overflow = acc32 + 24;
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        if (int e = ensure_list(c, n * size_t(c->kmax))) return e;
-        timed(c, "neighbor_lists", [&] {
-            const bool tiles = c->cfg.precision == OSPH_PRECISION_MIXED && c->tune.pair_tiles;
-            launch_neighbor_lists(l, c->pos, g, c->cstart, c->cend, c->list, c->count, c->kmax, overflow,
-                                  tiles ? c->tile_idx : nullptr, tiles ? c->tile_cnt : nullptr, n);
+        timed(c, "pack", [&] {
+            launch_pack_mixed(l, c->s, c->prm, lat, c->order, c->recf, c->pos, c->posf, sc, c->group, n);
         });
-        uint32_t flag = 0;
-        HIPCHK(c, hipMemcpyAsync(&c->small_host[12], overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    } else {
+        timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->pos, c->posf, sc, c->group, n); });
+    }
+    return OSPH_OK;
+}
+
+uint32_t* overflow_flag(osph_ctx* c) { return reinterpret_cast<uint32_t*>(c->small + SMALL_OVERFLOW); }
+
+int launch_lists(osph_ctx* c) {
+    if (int e = ensure_list(c, c->n * size_t(c->kmax))) return e;
+    timed(c, "neighbor_lists", [&] {
+        launch_neighbor_lists(L(c), c->pos, c->posf, c->screen, c->grid, c->cstart, c->cend, c->list, c->count,
+                              c->kmax, overflow_flag(c), c->n);
+    });
+    return OSPH_OK;
+}
+
+// After an overflow of the lists (a particle with more than kmax neighbours): grow kmax to fit the
+// largest count and clear the flag.  The stream must be idle.
+int grow_lists(osph_ctx* c) {
+    std::vector<uint32_t> cnt(c->n);
+    HIPCHK(c, hipMemcpy(cnt.data(), c->count, c->n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t mx = *std::max_element(cnt.begin(), cnt.end());
+    c->kmax = (std::max<uint32_t>(mx + 8, c->kmax * 2) + 3u) & ~3u;
+    launch_clear_u32(L(c), overflow_flag(c), 1, 0u);
+    return OSPH_OK;
+}
+
+bool overflowed(const osph_ctx* c) {
+    uint32_t flag;
+    std::memcpy(&flag, &c->small_host[SMALL_OVERFLOW], sizeof(uint32_t));
+    return flag != 0;
+}
+
+// Neighbour lists for the current grid, grown and rebuilt until every list fits.
+int build_lists(osph_ctx* c) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (int e = launch_lists(c)) return e;
+        HIPCHK(c, hipMemcpyAsync(&c->small_host[SMALL_OVERFLOW], overflow_flag(c), sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
         if (int e = sync_check(c)) return e;
-        std::memcpy(&flag, &c->small_host[12], sizeof(uint32_t));
-        if (!flag) {
+        if (!overflowed(c)) {
             c->neighbors_valid = true;
             return OSPH_OK;
         }
-        // a list overflowed: grow to the largest count and rebuild
-        std::vector<uint32_t> cnt(n);
-        HIPCHK(c, hipMemcpy(cnt.data(), c->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        const uint32_t mx = *std::max_element(cnt.begin(), cnt.end());
-        c->kmax = (std::max<uint32_t>(mx + 8, c->kmax * 2) + 3u) & ~3u;
-        launch_clear_u32(l, overflow, 1, 0u);
+        if (int e = grow_lists(c)) return e;
     }
     return c->fail(OSPH_ERR_STATE, "neighbour list did not fit after growing");
 }
 
-int integrate(osph_ctx* c, osph_stats* stats) {
-    const size_t n = c->n;
+int build_neighbors(osph_ctx* c, int mode) {
+    c->neighbors_valid = false;
+    if (c->n == 0) {
+        c->neighbors_valid = true;
+        return OSPH_OK;
+    }
+    if (int e = search_grid(c, mode, 0.0)) return e;
+    return build_lists(c);
+}
+
+void launch_pair_sums(osph_ctx* c) {
     Launch l = L(c);
-    timed(c, "material_init", [&] { launch_material_init(l, c->s, c->mats_dev, c->prm, n); });
-    if (int e = build_neighbors(c)) return e;
+    const size_t n = c->n;
     if (c->cfg.precision == OSPH_PRECISION_MIXED) {
         timed(c, "pairs_mixed", [&] {
-            const bool tiles = c->tune.pair_tiles != 0;
-            launch_pairs_mixed(l, c->s, c->prm, c->lattice, c->recf, c->order, c->list, c->count,
-                               tiles ? c->tile_idx : nullptr, tiles ? c->tile_cnt : nullptr, c->kmax, n);
+            launch_pairs_mixed(l, c->s, c->prm, c->lattice, c->recf, c->order, c->list, c->count, c->kmax, n);
         });
     } else {
         timed(c, "pairs_fp64", [&] {
             launch_pairs_fp64(l, c->s, c->prm, c->rec, c->group, c->order, c->list, c->count, c->lut, c->kmax, n);
         });
     }
+}
+
+int integrate(osph_ctx* c, osph_stats* stats) {
+    const size_t n = c->n;
+    Launch l = L(c);
+    if (int e = build_neighbors(c, 1)) return e;
+    launch_pair_sums(c);
     timed(c, "finalize", [&] { launch_finalize(l, c->s, c->mats_dev, c->prm, n); });
     if (stats) {
         std::vector<uint32_t> cnt(n);
@@ -555,20 +609,20 @@ int osph_stream_copy(size_t bytes, int iters, double* gbs) {
         hipEventCreate(&e1) == hipSuccess) {
         launch_clear_u32(Launch{ st }, static_cast<uint32_t*>(a), bytes / 4, 0x3ff00000u);
         launch_stream_copy(st, a, b, bytes);  // warm-up
-        hipEventRecord(e0, st);
+        (void)hipEventRecord(e0, st);
         for (int k = 0; k < iters; ++k) launch_stream_copy(st, a, b, bytes);
-        hipEventRecord(e1, st);
+        (void)hipEventRecord(e1, st);
         float ms = 0.f;
         if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.f) {
             *gbs = 2.0 * double(bytes) * iters / (double(ms) * 1e-3) / 1e9;
             rc = OSPH_OK;
         }
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (st) hipStreamDestroy(st);
-    hipFree(a);
-    hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return rc;
 }
 
@@ -595,8 +649,8 @@ int osph_create(osph_ctx** out, const osph_config* cfg, const osph_material* mat
     if (hipMalloc(&c->mats_dev, sizeof(osph_material) * n_materials) != hipSuccess) return bail(OSPH_ERR_HIP);
     if (hipMemcpy(c->mats_dev, materials, sizeof(osph_material) * n_materials, hipMemcpyHostToDevice) != hipSuccess)
         return bail(OSPH_ERR_HIP);
-    if (hipMalloc(&c->small, 64 * sizeof(unsigned long long)) != hipSuccess) return bail(OSPH_ERR_HIP);
-    if (hipHostMalloc(&c->small_host, 64 * sizeof(unsigned long long)) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (hipMalloc(&c->small, SMALL_WORDS * sizeof(unsigned long long)) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (hipHostMalloc(&c->small_host, SMALL_WORDS * sizeof(unsigned long long)) != hipSuccess) return bail(OSPH_ERR_HIP);
     std::vector<double> values(LUT_ENTRIES + 1), grads(LUT_ENTRIES + 1);
     build_lut(values.data(), grads.data());
     if (hipMalloc(&c->lut, grads.size() * sizeof(double)) != hipSuccess) return bail(OSPH_ERR_HIP);
@@ -612,20 +666,20 @@ int osph_create(osph_ctx** out, const osph_config* cfg, const osph_material* mat
 
 void osph_destroy(osph_ctx* c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     resolve_events(c);
-    for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     free_state(c);
-    hipFree(c->cstart);
-    hipFree(c->cend);
-    hipFree(c->list);
-    hipFree(c->staging);
-    hipFree(c->mats_dev);
-    hipFree(c->small);
-    hipFree(c->lut);
-    hipFree(c->lutf);
-    if (c->small_host) hipHostFree(c->small_host);
-    if (c->stream) hipStreamDestroy(c->stream);
+    (void)hipFree(c->cstart);
+    (void)hipFree(c->cend);
+    (void)hipFree(c->list);
+    (void)hipFree(c->staging);
+    (void)hipFree(c->mats_dev);
+    (void)hipFree(c->small);
+    (void)hipFree(c->lut);
+    (void)hipFree(c->lutf);
+    if (c->small_host) (void)hipHostFree(c->small_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -637,6 +691,7 @@ int osph_resize(osph_ctx* c, size_t n) {
     if (!c) return OSPH_ERR_ARG;
     if (n > 0x7fffffffull) return c->fail(OSPH_ERR_ARG, "particle count exceeds 2^31-1");
     c->neighbors_valid = false;
+    c->href = 0.0;
     if (n > c->cap) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         free_state(c);
@@ -649,13 +704,11 @@ int osph_resize(osph_ctx* c, size_t n) {
         HIPCHK(c, hipMalloc(&c->order, sizeof(uint32_t) * cap));
         if (c->cfg.precision == OSPH_PRECISION_MIXED) {
             HIPCHK(c, hipMalloc(&c->recf, sizeof(PairRecF) * cap));
-            const size_t blocks = (cap + TILE_BLOCK - 1) / TILE_BLOCK;
-            HIPCHK(c, hipMalloc(&c->tile_idx, sizeof(uint32_t) * blocks * TILE_CAP));
-            HIPCHK(c, hipMalloc(&c->tile_cnt, sizeof(uint32_t) * blocks));
         } else {
             HIPCHK(c, hipMalloc(&c->rec, sizeof(PairRec) * cap));
         }
         HIPCHK(c, hipMalloc(&c->pos, sizeof(double4) * cap));
+        HIPCHK(c, hipMalloc(&c->posf, sizeof(float4) * cap));
         HIPCHK(c, hipMalloc(&c->group, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->count, sizeof(uint32_t) * cap));
         c->sort_temp_bytes = sort_pairs_temp_bytes(cap);
@@ -697,6 +750,7 @@ int osph_set_quantity(osph_ctx* c, int q, int order, const void* host, size_t st
     plane_ptrs(c, d, dst);
     launch_stage_to_planes(L(c), c->staging, stride, d.ncomp, d.off, d.elem, dst, n);
     c->neighbors_valid = false;
+    c->href = 0.0;
     return sync_check(c);
 }
 
@@ -726,7 +780,7 @@ int osph_zero_derivatives(osph_ctx* c) {
 
 int osph_build_neighbors(osph_ctx* c) {
     if (!c) return OSPH_ERR_ARG;
-    if (int e = build_neighbors(c)) return e;
+    if (int e = build_neighbors(c, 0)) return e;
     return sync_check(c);
 }
 
@@ -734,24 +788,17 @@ int osph_get_pairs(osph_ctx* c, uint32_t* pi, uint32_t* pj, size_t cap, size_t* 
     if (!c || !count) return OSPH_ERR_ARG;
     if (!c->neighbors_valid) return c->fail(OSPH_ERR_STATE, "neighbours not built");
     const size_t n = c->n;
-    std::vector<uint32_t> cnt(n), order(n), list(n * size_t(c->kmax)), tiles;
-    const size_t blocks = (n + TILE_BLOCK - 1) / TILE_BLOCK;
+    std::vector<uint32_t> cnt(n), order(n), list(n * size_t(c->kmax));
     if (n) {
         HIPCHK(c, hipMemcpy(cnt.data(), c->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(order.data(), c->order, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(list.data(), c->list, list.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        if (c->tile_idx && c->tune.pair_tiles) {
-            tiles.resize(blocks * TILE_CAP);
-            HIPCHK(c, hipMemcpy(tiles.data(), c->tile_idx, tiles.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        }
     }
     std::vector<uint64_t> keys;
     for (size_t k = 0; k < n; ++k) {
         const uint64_t i = order[k];
         for (uint32_t m = 0; m < cnt[k] && m < c->kmax; ++m) {
-            uint32_t e = list[size_t(k) * c->kmax + m];
-            if (!tiles.empty() && (e & LOCAL_BIT)) e = tiles[(k / TILE_BLOCK) * TILE_CAP + (e & ~LOCAL_BIT)];
-            const uint64_t j = order[e];
+            const uint64_t j = order[list[size_t(k) * c->kmax + m]];
             if (i < j) keys.push_back((i << 32) | j);
         }
     }
@@ -772,31 +819,50 @@ int osph_integrate(osph_ctx* c, osph_stats* stats) {
 
 int osph_timestep(osph_ctx* c, double* dt, int* crit) {
     if (!c) return OSPH_ERR_ARG;
-    uint32_t* m32 = reinterpret_cast<uint32_t*>(c->small + 8);
+    uint32_t* m32 = reinterpret_cast<uint32_t*>(c->small + SMALL_CRITERIA);
     launch_clear_u32(L(c), m32, 8, 0xffffffffu);
-    timed(c, "criteria", [&] { launch_criteria(L(c), c->s, c->mats_dev, c->prm, c->small + 8, c->n); });
-    HIPCHK(c, hipMemcpyAsync(c->small_host + 8, c->small + 8, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             c->stream));
+    timed(c, "criteria", [&] { launch_criteria(L(c), c->s, c->mats_dev, c->prm, c->small + SMALL_CRITERIA, c->n); });
+    HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CRITERIA, c->small + SMALL_CRITERIA,
+                             4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
     if (int e = sync_check(c)) return e;
-    select_timestep(c, c->small_host + 8, dt, crit);
+    select_timestep(c, c->small_host + SMALL_CRITERIA, dt, crit);
     return OSPH_OK;
 }
 
+// One PredictorCorrector step with the state resident: the fused predict pass and neighbour search,
+// the lists, the pair sums and the fused finalize + correct + criteria.  The list overflow flag is read
+// with the criteria at the end of the step; after an overflow the finalize pass has left the predicted
+// state untouched, so the lists are grown and the step's second half is run again.
 int osph_step_pc(osph_ctx* c, double dt, double* next_dt, int* crit) {
     if (!c) return OSPH_ERR_ARG;
     if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(OSPH_ERR_ARG, "time step must be positive");
     const size_t n = c->n;
     Launch l = L(c);
-    timed(c, "predict", [&] { launch_predict(l, c->s, c->mats_dev, c->prm, dt, n); });
-    if (int e = integrate(c, nullptr)) return e;
-    uint32_t* m32 = reinterpret_cast<uint32_t*>(c->small + 8);
-    launch_clear_u32(l, m32, 8, 0xffffffffu);
-    timed(c, "correct", [&] { launch_correct(l, c->s, c->mats_dev, c->prm, dt, c->small + 8, n); });
-    HIPCHK(c, hipMemcpyAsync(c->small_host + 8, c->small + 8, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             c->stream));
-    if (int e = sync_check(c)) return e;
-    select_timestep(c, c->small_host + 8, next_dt, crit);
-    return OSPH_OK;
+    c->neighbors_valid = false;
+    if (n == 0) {
+        unsigned long long none[4] = { ~0ull, ~0ull, ~0ull, ~0ull };
+        select_timestep(c, none, next_dt, crit);
+        return OSPH_OK;
+    }
+    if (int e = search_grid(c, 2, dt)) return e;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (int e = launch_lists(c)) return e;
+        launch_pair_sums(c);
+        timed(c, "finalize_correct", [&] {
+            launch_finalize_correct(l, c->s, c->mats_dev, c->prm, dt, c->small + SMALL_CRITERIA, overflow_flag(c), n);
+        });
+        HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CRITERIA, c->small + SMALL_CRITERIA,
+                                 (SMALL_OVERFLOW + 1 - SMALL_CRITERIA) * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, c->stream));
+        if (int e = sync_check(c)) return e;
+        if (!overflowed(c)) {
+            c->neighbors_valid = true;
+            select_timestep(c, c->small_host + SMALL_CRITERIA, next_dt, crit);
+            return OSPH_OK;
+        }
+        if (int e = grow_lists(c)) return e;
+    }
+    return c->fail(OSPH_ERR_STATE, "neighbour list did not fit after growing");
 }
 
 int osph_tune(osph_ctx* c, const char* key, long long value) {
@@ -814,8 +880,8 @@ int osph_tune(osph_ctx* c, const char* key, long long value) {
         c->tune.lists_lds = int(value);
     } else if (k == "cell_divisor" && (value == 1 || value == 2 || value == 3)) {
         c->tune.cell_divisor = int(value);
-    } else if (k == "pair_tiles" && (value == 0 || value == 1)) {
-        c->tune.pair_tiles = int(value);
+    } else if (k == "lists_screen" && (value == -1 || value == 0 || value == 1)) {
+        c->tune.lists_screen = int(value);
     } else {
         return c->fail(OSPH_ERR_ARG, "unknown tuning key or value out of range: " + k);
     }

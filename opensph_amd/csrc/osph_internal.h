@@ -55,15 +55,13 @@ struct alignas(16) PairRecF {
 };
 static_assert(sizeof(PairRecF) == 96, "PairRecF must be 96 bytes");
 
-// Neighbour tiles of the mixed-precision path: every block of TILE_BLOCK consecutive sorted particles
-// stages in LDS the records of its neighbours that lie within a window of TILE_WINDOW sorted positions
-// around the block (up to TILE_CAP of them, in sorted order).  A list entry with LOCAL_BIT set is a slot
-// in that tile; otherwise it is a sorted position read from global memory.
-constexpr int TILE_BLOCK = 256;
-constexpr int TILE_CAP = 1280;
-constexpr int TILE_WINDOW = 8192;                          // 256 bitmap words
-constexpr int TILE_BEFORE = (TILE_WINDOW - TILE_BLOCK) / 2;  // window start = block start - TILE_BEFORE
-constexpr uint32_t LOCAL_BIT = 0x80000000u;
+// FP32 screen of the neighbour test: positions relative to the origin o, and the band [lo, hi] = [1 - b,
+// 1 + b] around the squared radius outside which the FP32 comparison equals the exact FP64 one.
+struct Screen {
+    double o[3];
+    float lo, hi;
+    int on;
+};
 
 struct Lattice {
     double lo[3];
@@ -128,7 +126,7 @@ struct Tuning {
     int lists_block = 256;
     int lists_lds = 0;
     int cell_divisor = 1;  // cells of 2 h_max(class) / divisor
-    int pair_tiles = 0;    // mixed precision: stage neighbour tiles in LDS
+    int lists_screen = -1; // FP32 screen in the list kernel: 1 on, 0 off, -1 on for mixed precision only
 };
 
 struct Launch {
@@ -136,33 +134,46 @@ struct Launch {
     Tuning tune{};
 };
 
-void launch_material_init(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n);
+// Accumulators of one step (64 words of device memory): SMALL_BOUNDS + 0..7 global bounds,
+// SMALL_CRITERIA + 0..3 the criterion minima, SMALL_OVERFLOW the list overflow flag (low 32 bits),
+// SMALL_CLASSES + CLASS_STRIDE * c + 0..8 the bounds of class c.
+constexpr int SMALL_BOUNDS = 0;
+constexpr int SMALL_CRITERIA = 8;
+constexpr int SMALL_OVERFLOW = 12;
+constexpr int SMALL_CLASSES = 16;
+constexpr int CLASS_STRIDE = 10;
+constexpr int SMALL_WORDS = 64;
+static_assert(SMALL_CLASSES + CLASS_STRIDE * MAX_CLASSES <= SMALL_WORDS, "accumulator layout");
+
+// Clears every accumulator (min slots to all ones, max and count slots to zero) in one launch.
+void launch_init_small(const Launch& l, unsigned long long* small);
+// acc[0..2] = min x, y, z; acc[3..5] = max x, y, z; acc[6] = max h; acc[7] = min h (order-encoded).
 void launch_bounds_reduce(const Launch& l, DevState s, unsigned long long* acc, size_t n);
-void launch_class_bounds(const Launch& l, DevState s, int nclass, double hmin, unsigned long long* acc, size_t n);
+// mode 2: predictions + material initialize; 1: material initialize; 0: neither.  Then the per-class
+// grid bounds (class of h relative to href) into acc[CLASS_STRIDE * c + 0..8].
+void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
+                    double href, unsigned long long* acc, size_t n);
 void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n);
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value);
 void launch_cell_ranges(const Launch& l, const uint32_t* keys, Grid g, uint32_t* cstart, uint32_t* cend, size_t n);
 void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, double4* pos,
-                 uint32_t* group, size_t n);
+                 float4* posf, Screen sc, uint32_t* group, size_t n);
 // Neighbour lists are row-major: list[k * kmax + c], kmax a multiple of 4.
-// With tiles != nullptr (mixed precision), also builds the per-block neighbour tiles: tile_idx[b * TILE_CAP + t]
-// holds the sorted position of slot t of block b, tile_cnt[b] the number of slots used.
-void launch_neighbor_lists(const Launch& l, const double4* pos, Grid g, const uint32_t* cstart, const uint32_t* cend,
-                           uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow, uint32_t* tile_idx,
-                           uint32_t* tile_cnt, size_t n);
+void launch_neighbor_lists(const Launch& l, const double4* pos, const float4* posf, Screen sc, Grid g,
+                           const uint32_t* cstart, const uint32_t* cend, uint32_t* list, uint32_t* count, uint32_t kmax,
+                           uint32_t* overflow, size_t n);
 void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
                        uint32_t kmax, size_t n);
 void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
-                       double4* pos, uint32_t* group, size_t n);
+                       double4* pos, float4* posf, Screen sc, uint32_t* group, size_t n);
 void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
-                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const uint32_t* tile_idx,
-                        const uint32_t* tile_cnt, uint32_t kmax, size_t n);
+                        const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n);
 void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n);
 void launch_zero_derivatives(const Launch& l, DevState s, size_t n);
-void launch_predict(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt, size_t n);
-void launch_correct(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt,
-                    unsigned long long* mins, size_t n);
+// Finalize + corrections + criteria of a predictor-corrector step; a no-op while *overflow != 0.
+void launch_finalize_correct(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt,
+                             unsigned long long* mins, const uint32_t* overflow, size_t n);
 void launch_criteria(const Launch& l, DevState s, const osph_material* mats, Params prm,
                      unsigned long long* mins, size_t n);
 void launch_stage_to_planes(const Launch& l, const unsigned char* staging, size_t stride, int ncomp,

@@ -122,12 +122,8 @@ __device__ void eigenvalues(const double t[6], double out[3]) {
 // EosMaterial::initialize (core/sph/Materials.cpp:47-58), VonMisesRheology::initialize
 // (core/physics/Rheology.cpp:36-83) and the h clamp of AdaptiveSmoothingLength::initialize
 // (core/sph/equations/EquationTerm.cpp:356-364).
-__global__ void __launch_bounds__(BLOCK) k_material_init(DevState s, const osph_material* __restrict__ mats,
-                                                         Params prm, size_t n) {
+__device__ void material_init_at(DevState& s, const osph_material& m, const Params& prm, size_t i) {
 #pragma clang fp contract(off)
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const osph_material m = mats[s.u[MAT][i]];
     double p, cs;
     tillotson(m, s.d[RHO][i], s.d[U][i], p, cs);
     s.d[CS][i] = cs;
@@ -215,6 +211,20 @@ __device__ void block_reduce_ord(unsigned long long (&v)[NV], const bool (&is_mi
     }
 }
 
+__global__ void __launch_bounds__(SMALL_WORDS) k_init_small(unsigned long long* a) {
+    const int t = threadIdx.x;
+    bool is_min;
+    if (t >= SMALL_CLASSES && t < SMALL_CLASSES + CLASS_STRIDE * MAX_CLASSES) {
+        const int k = (t - SMALL_CLASSES) % CLASS_STRIDE;
+        is_min = k < 3 || k == 7;
+    } else if (t >= SMALL_CRITERIA && t < SMALL_CRITERIA + 4) {
+        is_min = true;
+    } else {
+        is_min = t < SMALL_BOUNDS + 3 || t == SMALL_BOUNDS + 7;
+    }
+    a[t] = is_min ? ~0ull : 0ull;
+}
+
 // acc[0..2] = min x, y, z; acc[3..5] = max x, y, z; acc[6] = max h; acc[7] = min h (order-encoded).
 __global__ void __launch_bounds__(BLOCK) k_bounds(DevState s, unsigned long long* acc, size_t n) {
     unsigned long long v[8] = { ~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull, 0ull, ~0ull };
@@ -238,67 +248,86 @@ __device__ __forceinline__ int h_class(double h, double hmin, int nclass) {
     return c < 0 ? 0 : (c >= nclass ? nclass - 1 : c);
 }
 
-// Per class c: acc[8c + 0..2] min x, y, z; [3..5] max x, y, z; [6] max h (order-encoded); [7] count.
-__global__ void __launch_bounds__(BLOCK) k_class_bounds(DevState s, int nclass, double hmin, unsigned long long* acc,
-                                                        size_t n) {
-    unsigned long long v[MAX_CLASSES * 7];
-    bool is_min[MAX_CLASSES * 7];
-    unsigned long long cnt[MAX_CLASSES] = { 0, 0, 0, 0 };
-    for (int c = 0; c < MAX_CLASSES; ++c) {
-        for (int k = 0; k < 7; ++k) {
-            is_min[7 * c + k] = k < 3;
-            v[7 * c + k] = k < 3 ? ~0ull : 0ull;
+// Per-class bounds of the neighbour grid, accumulated per thread and reduced per block into
+// acc[CLASS_STRIDE * c + ...]: [0..2] min x, y, z; [3..5] max x, y, z; [6] max h; [7] min h (all
+// order-encoded); [8] count.  Classes of no lane of a wave skip the wave reduction.
+struct ClassAcc {
+    unsigned long long v[MAX_CLASSES][8];
+    unsigned long long cnt[MAX_CLASSES];
+
+    __device__ void init() {
+#pragma unroll
+        for (int c = 0; c < MAX_CLASSES; ++c) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[c][k] = (k < 3 || k == 7) ? ~0ull : 0ull;
+            cnt[c] = 0;
         }
     }
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-        const double h = s.d[H][i];
-        const int c = h_class(h, hmin, nclass);
-        for (int cc = 0; cc < MAX_CLASSES; ++cc) {
-            if (cc != c) continue;  // keep the register arrays statically indexed
+
+    __device__ void add(int cls, double x, double y, double z, double h) {
+        const unsigned long long b[4] = { ord_bits(x), ord_bits(y), ord_bits(z), ord_bits(h) };
+#pragma unroll
+        for (int c = 0; c < MAX_CLASSES; ++c) {
+            if (c != cls) continue;  // keeps the register arrays statically indexed
+#pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const unsigned long long b = ord_bits(s.d[X + k][i]);
-                v[7 * cc + k] = b < v[7 * cc + k] ? b : v[7 * cc + k];
-                v[7 * cc + 3 + k] = b > v[7 * cc + 3 + k] ? b : v[7 * cc + 3 + k];
+                v[c][k] = b[k] < v[c][k] ? b[k] : v[c][k];
+                v[c][3 + k] = b[k] > v[c][3 + k] ? b[k] : v[c][3 + k];
             }
-            const unsigned long long hb = ord_bits(h);
-            v[7 * cc + 6] = hb > v[7 * cc + 6] ? hb : v[7 * cc + 6];
-            cnt[cc] += 1;
+            v[c][6] = b[3] > v[c][6] ? b[3] : v[c][6];
+            v[c][7] = b[3] < v[c][7] ? b[3] : v[c][7];
+            cnt[c] += 1;
         }
     }
-    __shared__ unsigned long long red[BLOCK / 64][MAX_CLASSES * 7 + MAX_CLASSES];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    // Whole block: every thread must call it.
+    __device__ void reduce(unsigned long long* acc) {
+        constexpr int NV = 9;
+        __shared__ unsigned long long red[BLOCK / 64][MAX_CLASSES][NV];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < MAX_CLASSES * 7; ++k) {
-        const unsigned long long r = is_min[k] ? wave_min(v[k]) : wave_max(v[k]);
-        if (lane == 0) red[wave][k] = r;
-    }
+        for (int c = 0; c < MAX_CLASSES; ++c) {
+            if (__any(cnt[c] != 0)) {
 #pragma unroll
-    for (int c = 0; c < MAX_CLASSES; ++c) {
-        unsigned long long t = cnt[c];
-        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-        if (lane == 0) red[wave][MAX_CLASSES * 7 + c] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < MAX_CLASSES * 7) {
-        const int k = threadIdx.x;
-        unsigned long long r = red[0][k];
-        for (int w = 1; w < BLOCK / 64; ++w) {
-            const unsigned long long x = red[w][k];
-            r = (k % 7) < 3 ? (x < r ? x : r) : (x > r ? x : r);
+                for (int k = 0; k < 8; ++k) {
+                    const unsigned long long r = (k < 3 || k == 7) ? wave_min(v[c][k]) : wave_max(v[c][k]);
+                    if (lane == 0) red[wave][c][k] = r;
+                }
+                unsigned long long t = cnt[c];
+                for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+                if (lane == 0) red[wave][c][8] = t;
+            } else if (lane == 0) {
+                red[wave][c][8] = 0;
+            }
         }
-        unsigned long long* dst = &acc[8 * (k / 7) + (k % 7)];
-        if ((k % 7) < 3) {
-            atomicMin(dst, r);
-        } else {
-            atomicMax(dst, r);
+        __syncthreads();
+        if (threadIdx.x < MAX_CLASSES * NV) {
+            const int c = threadIdx.x / NV, k = threadIdx.x % NV;
+            unsigned long long r = k == 8 ? 0ull : ((k < 3 || k == 7) ? ~0ull : 0ull);
+            bool any = false;
+            for (int w = 0; w < BLOCK / 64; ++w) {
+                if (red[w][c][8] == 0) continue;
+                any = true;
+                const unsigned long long x = red[w][c][k];
+                if (k == 8) {
+                    r += x;
+                } else {
+                    r = (k < 3 || k == 7) ? (x < r ? x : r) : (x > r ? x : r);
+                }
+            }
+            if (any) {
+                unsigned long long* dst = &acc[CLASS_STRIDE * c + k];
+                if (k == 8) {
+                    atomicAdd(dst, r);
+                } else if (k < 3 || k == 7) {
+                    atomicMin(dst, r);
+                } else {
+                    atomicMax(dst, r);
+                }
+            }
         }
-    } else if (threadIdx.x < MAX_CLASSES * 8) {
-        const int c = threadIdx.x - MAX_CLASSES * 7;
-        unsigned long long t = 0;
-        for (int w = 0; w < BLOCK / 64; ++w) t += red[w][MAX_CLASSES * 7 + c];
-        if (t) atomicAdd(&acc[8 * c + 7], t);
     }
-}
+};
 
 __device__ __forceinline__ int cell_coord(const GridClass& g, double x, int k) {
     return int(floor((x - g.lo[k]) * g.inv_cell));
@@ -340,9 +369,15 @@ __global__ void __launch_bounds__(BLOCK) k_cell_ranges(const uint32_t* __restric
 }
 
 // Pack the neighbour-side data of particle order[k] into rec[k].
+// Origin-relative FP32 position of the screened neighbour test (one rounding of the FP64 offset).
+__device__ __forceinline__ float4 screen_pos(const Screen& sc, double x, double y, double z, double h) {
+    return make_float4(float(x - sc.o[0]), float(y - sc.o[1]), float(z - sc.o[2]), float(h));
+}
+
 __global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const uint32_t* __restrict__ order,
                                                 PairRec* __restrict__ rec, double4* __restrict__ pos,
-                                                uint32_t* __restrict__ group, size_t n) {
+                                                float4* __restrict__ posf, Screen sc, uint32_t* __restrict__ group,
+                                                size_t n) {
 #pragma clang fp contract(off)
     const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -372,6 +407,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const ui
     }
     rec[k] = r;
     pos[k] = make_double4(r.x, r.y, r.z, r.h);
+    if (sc.on) posf[k] = screen_pos(sc, r.x, r.y, r.z, r.h);
     // DerivativeTemplate / AccelerationTemplate sum filter (DerivativeHelpers.h:132-139,262-269):
     // a pair enters the SUM_ONLY_UNDAMAGED sums iff both flags are equal and both reductions nonzero.
     uint32_t gid = 0;
@@ -397,140 +433,90 @@ __device__ __forceinline__ bool exact_pair(double xi, double yi, double zi, doub
 // pair test.  Neighbour positions go to the particle's row list[k * kmax + c] (kmax a multiple of 4),
 // buffered four at a time in registers and stored as one 16-byte write.
 //
-// With TILE (mixed precision), the block's neighbours inside its window of sorted positions are marked in
-// an LDS bitmap; after the scan a block prefix sum of the bitmap words numbers them in sorted order, the
-// first TILE_CAP become the block's tile (tile_idx), and every list entry in the tile is rewritten as
-// LOCAL_BIT | slot.  The block of logical index b covers sorted positions [b * 256, b * 256 + 256), the
-// same decomposition the tiled pair kernel uses.
-template <bool TILE>
-__global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restrict__ pos, Grid g,
+// With SCREEN, candidates are first tested in FP32 on origin-relative positions (16 bytes per candidate
+// instead of 32).  The FP32 squared distance and radius carry a relative error below sc.band (bounded on
+// the host from the domain extent and h_min), so d2 < R2 (1 - band) and d2 > R2 (1 + band) decide the
+// exact test; only candidates inside the band are re-tested in FP64, and the pair set is unchanged.
+template <bool SCREEN>
+__global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restrict__ pos,
+                                                          const float4* __restrict__ posf, Screen sc, Grid g,
                                                           const uint32_t* __restrict__ cstart,
                                                           const uint32_t* __restrict__ cend, uint32_t* __restrict__ list,
                                                           uint32_t* __restrict__ count, uint32_t kmax,
-                                                          uint32_t* overflow, uint32_t* __restrict__ tile_idx,
-                                                          uint32_t* __restrict__ tile_cnt, size_t n) {
-    constexpr int WORDS = TILE_WINDOW / 32;
-    __shared__ uint32_t bits[TILE ? WORDS : 1];
-    __shared__ uint32_t scan[TILE ? BLOCK : 1];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const size_t k = size_t(blk) * blockDim.x + threadIdx.x;
-    // window [wlo, wlo + TILE_WINDOW) of sorted positions, as a signed offset from position 0
-    const long long wlo = (long long)blk * BLOCK - TILE_BEFORE;
-    if (TILE) {
-        for (int t = threadIdx.x; t < WORDS; t += blockDim.x) bits[t] = 0u;
-        __syncthreads();
-    }
+                                                          uint32_t* overflow, size_t n) {
+    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double4 me = pos[k];
+    const float4 mf = SCREEN ? posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const double p[3] = { me.x, me.y, me.z };
+    uint4* row = reinterpret_cast<uint4*>(list + size_t(k) * kmax);
+    uint32_t buf[4] = { 0, 0, 0, 0 };
     uint32_t cnt = 0;
-    if (k < n) {
-        const double4 me = pos[k];
-        const double p[3] = { me.x, me.y, me.z };
-        uint4* row = reinterpret_cast<uint4*>(list + size_t(k) * kmax);
-        uint32_t buf[4] = { 0, 0, 0, 0 };
-        for (int cls = 0; cls < g.nclass; ++cls) {
-            const GridClass& G = g.c[cls];
-            if (G.empty) continue;
-            const double R = me.w + G.hmax;  // h_i + h_j <= R for every member j
-            double d2b = 0.0;
-            for (int a = 0; a < 3; ++a) {
-                const double t = dmax(G.lo[a] - p[a], p[a] - G.hi[a]);
-                if (t > 0.0) d2b += t * t;
-            }
-            if (d2b > R * R * (1.0 + 1e-6)) continue;
-            const int r = int(ceil(R * G.inv_cell * (1.0 + 1e-9)));
-            int lo[3], up[3];
-            for (int a = 0; a < 3; ++a) {
-                const int c = cell_coord(G, p[a], a);
-                lo[a] = c - r < 0 ? 0 : c - r;
-                up[a] = c + r >= G.dim[a] ? G.dim[a] - 1 : c + r;
-            }
-            // cells whose box lies farther than R (with a relative margin) hold no neighbour
-            const double R2 = R * R * (1.0 + 1e-6);
-            for (int cz = lo[2]; cz <= up[2]; ++cz) {
-                const double tz = dmax(dmax(G.lo[2] + cz * G.cell - p[2], p[2] - (G.lo[2] + (cz + 1) * G.cell)), 0.0);
-                for (int cy = lo[1]; cy <= up[1]; ++cy) {
-                    const double ty = dmax(dmax(G.lo[1] + cy * G.cell - p[1], p[1] - (G.lo[1] + (cy + 1) * G.cell)), 0.0);
-                    const double dzy = tz * tz + ty * ty;
-                    if (dzy > R2) continue;
-                    for (int cx = lo[0]; cx <= up[0]; ++cx) {
-                        const double tx =
-                            dmax(dmax(G.lo[0] + cx * G.cell - p[0], p[0] - (G.lo[0] + (cx + 1) * G.cell)), 0.0);
-                        if (dzy + tx * tx > R2) continue;
-                        const uint32_t cell = G.base + morton3(uint32_t(cx), uint32_t(cy), uint32_t(cz));
-                        const uint32_t b = cstart[cell], e = cend[cell];
-                        for (uint32_t k2 = b; k2 < e; ++k2) {
-                            if (k2 == k) continue;
+    for (int cls = 0; cls < g.nclass; ++cls) {
+        const GridClass& G = g.c[cls];
+        if (G.empty) continue;
+        const double R = me.w + G.hmax;  // h_i + h_j <= R for every member j
+        double d2b = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const double t = dmax(G.lo[a] - p[a], p[a] - G.hi[a]);
+            if (t > 0.0) d2b += t * t;
+        }
+        if (d2b > R * R * (1.0 + 1e-6)) continue;
+        const int r = int(ceil(R * G.inv_cell * (1.0 + 1e-9)));
+        int lo[3], up[3];
+        for (int a = 0; a < 3; ++a) {
+            const int c = cell_coord(G, p[a], a);
+            lo[a] = c - r < 0 ? 0 : c - r;
+            up[a] = c + r >= G.dim[a] ? G.dim[a] - 1 : c + r;
+        }
+        // cells whose box lies farther than R (with a relative margin) hold no neighbour
+        const double R2 = R * R * (1.0 + 1e-6);
+        for (int cz = lo[2]; cz <= up[2]; ++cz) {
+            const double tz = dmax(dmax(G.lo[2] + cz * G.cell - p[2], p[2] - (G.lo[2] + (cz + 1) * G.cell)), 0.0);
+            for (int cy = lo[1]; cy <= up[1]; ++cy) {
+                const double ty = dmax(dmax(G.lo[1] + cy * G.cell - p[1], p[1] - (G.lo[1] + (cy + 1) * G.cell)), 0.0);
+                const double dzy = tz * tz + ty * ty;
+                if (dzy > R2) continue;
+                for (int cx = lo[0]; cx <= up[0]; ++cx) {
+                    const double tx =
+                        dmax(dmax(G.lo[0] + cx * G.cell - p[0], p[0] - (G.lo[0] + (cx + 1) * G.cell)), 0.0);
+                    if (dzy + tx * tx > R2) continue;
+                    const uint32_t cell = G.base + morton3(uint32_t(cx), uint32_t(cy), uint32_t(cz));
+                    const uint32_t b = cstart[cell], e = cend[cell];
+                    for (uint32_t k2 = b; k2 < e; ++k2) {
+                        bool hit;
+                        if (SCREEN) {
+                            const float4 o = posf[k2];
+                            const float dx = mf.x - o.x, dy = mf.y - o.y, dz = mf.z - o.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            const float hs = mf.w + o.w;
+                            const float rr = hs * hs;
+                            hit = d2 < rr * sc.lo;
+                            if (!hit && !(d2 > rr * sc.hi)) {
+                                const double4 od = pos[k2];
+                                hit = exact_pair(me.x, me.y, me.z, me.w, od.x, od.y, od.z, od.w);
+                            }
+                        } else {
                             const double4 o = pos[k2];
-                            if (exact_pair(me.x, me.y, me.z, me.w, o.x, o.y, o.z, o.w)) {
-                                buf[cnt & 3u] = k2;
-                                ++cnt;
-                                if ((cnt & 3u) == 0 && cnt <= kmax) {
-                                    row[(cnt >> 2) - 1] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
-                                }
-                                if (TILE) {
-                                    const long long off = (long long)k2 - wlo;
-                                    if (off >= 0 && off < TILE_WINDOW) {
-                                        atomicOr(&bits[off >> 5], 1u << (off & 31));
-                                    }
-                                }
+                            hit = exact_pair(me.x, me.y, me.z, me.w, o.x, o.y, o.z, o.w);
+                        }
+                        if (hit && k2 != k) {
+                            buf[cnt & 3u] = k2;
+                            ++cnt;
+                            if ((cnt & 3u) == 0 && cnt <= kmax) {
+                                row[(cnt >> 2) - 1] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
                             }
                         }
                     }
                 }
             }
         }
-        if ((cnt & 3u) != 0 && cnt < kmax) {
-            row[cnt >> 2] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
-        }
-        count[k] = cnt;
-        if (cnt > kmax) atomicOr(overflow, 1u);
     }
-    if (!TILE) return;
-    __syncthreads();
-    // number the marked positions in sorted order: exclusive prefix sum of the word popcounts
-    static_assert(WORDS == BLOCK, "one bitmap word per thread");
-    const uint32_t word = bits[threadIdx.x];
-    const uint32_t mine = __popc(word);
-    scan[threadIdx.x] = mine;
-    __syncthreads();
-    for (int off = 1; off < BLOCK; off <<= 1) {
-        const uint32_t add = threadIdx.x >= off ? scan[threadIdx.x - off] : 0u;
-        __syncthreads();
-        scan[threadIdx.x] += add;
-        __syncthreads();
+    if ((cnt & 3u) != 0 && cnt < kmax) {
+        row[cnt >> 2] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
     }
-    const uint32_t base = scan[threadIdx.x] - mine;  // exclusive prefix of this word
-    {
-        uint32_t w = word, slot = base;
-        while (w) {
-            const int b = __ffs(w) - 1;
-            w &= w - 1u;
-            if (slot < uint32_t(TILE_CAP)) {
-                tile_idx[size_t(blk) * TILE_CAP + slot] = uint32_t(wlo + (long long)threadIdx.x * 32 + b);
-            }
-            ++slot;
-        }
-    }
-    if (threadIdx.x == BLOCK - 1) tile_cnt[blk] = scan[BLOCK - 1] < uint32_t(TILE_CAP) ? scan[BLOCK - 1] : TILE_CAP;
-    __syncthreads();
-    // turn scan[] into the exclusive prefix per word for the lookups below
-    const uint32_t excl = base;
-    __syncthreads();
-    scan[threadIdx.x] = excl;
-    __syncthreads();
-    // rewrite this particle's entries that live in the tile
-    if (k < n) {
-        const uint32_t listed = cnt < kmax ? cnt : kmax;
-        uint32_t* row = list + size_t(k) * kmax;
-        for (uint32_t c = 0; c < listed; ++c) {
-            const uint32_t k2 = row[c];
-            const long long off = (long long)k2 - wlo;
-            if (off >= 0 && off < TILE_WINDOW) {
-                const uint32_t wi = uint32_t(off >> 5), b = uint32_t(off & 31);
-                const uint32_t slot = scan[wi] + __popc(bits[wi] & ((1u << b) - 1u));
-                if (slot < uint32_t(TILE_CAP)) row[c] = LOCAL_BIT | slot;
-            }
-        }
-    }
+    count[k] = cnt;
+    if (cnt > kmax) atomicOr(overflow, 1u);
 }
 
 // LutKernel<3>::gradImpl (core/sph/kernel/Kernel.h:129-144): linear interpolation in q^2.
@@ -644,7 +630,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
 // the neighbour-list build).
 __global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, Lattice L,
                                                       const uint32_t* __restrict__ order, PairRecF* __restrict__ rec,
-                                                      double4* __restrict__ pos, uint32_t* __restrict__ group,
+                                                      double4* __restrict__ pos, float4* __restrict__ posf,
+                                                      Screen sc, uint32_t* __restrict__ group,
                                                       size_t n) {
     const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -691,6 +678,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, La
     r.pad0 = r.pad1 = r.pad2 = 0u;
     rec[k] = r;
     pos[k] = make_double4(x[0], x[1], x[2], h);
+    if (sc.on) posf[k] = screen_pos(sc, x[0], x[1], x[2], h);
     group[k] = gid;
 }
 
@@ -720,25 +708,12 @@ struct Kahan {
 
 // Mixed-precision pair sums: the same terms and gather form as k_pairs_fp64, evaluated in FP32 on
 // lattice-relative positions with Kahan-compensated FP32 accumulators; results are written in FP64.
-// With TILE the block first stages its neighbour tile (built by the list kernel) in LDS; list entries with
-// LOCAL_BIT read the tile, the others read the record from global memory.
-template <bool TILE>
-__global__ void __launch_bounds__(TILE_BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
-                                                            const PairRecF* __restrict__ rec,
-                                                            const uint32_t* __restrict__ order,
-                                                            const uint32_t* __restrict__ list,
-                                                            const uint32_t* __restrict__ count,
-                                                            const uint32_t* __restrict__ tile_idx,
-                                                            const uint32_t* __restrict__ tile_cnt, uint32_t kmax,
-                                                            size_t n) {
-    __shared__ PairRecF tile[TILE ? TILE_CAP : 1];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    if (TILE) {
-        const uint32_t nt = tile_cnt[blk];
-        for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) tile[t] = rec[tile_idx[size_t(blk) * TILE_CAP + t]];
-        __syncthreads();
-    }
-    const size_t k = size_t(blk) * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
+                                                       const PairRecF* __restrict__ rec,
+                                                       const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ count, uint32_t kmax, size_t n) {
+    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const PairRecF me = rec[k];
     const uint32_t gi = me.group;
@@ -760,12 +735,7 @@ __global__ void __launch_bounds__(TILE_BLOCK) k_pairs_mixed(DevState s, Params p
             next = (c + 4 < listed) ? row[(c >> 2) + 1] : make_uint4(0, 0, 0, 0);
         }
         const uint32_t e = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
-        PairRecF o;
-        if (TILE && (e & LOCAL_BIT)) {
-            o = tile[e & ~LOCAL_BIT];
-        } else {
-            o = rec[e];
-        }
+        const PairRecF o = rec[e];
         const int lxj = int(o.lat & 1023u), lyj = int((o.lat >> 10) & 1023u), lzj = int(o.lat >> 20);
         const float rx = float(lxi - lxj) * L.Sf + (me.dx - o.dx);
         const float ry = float(lyi - lyj) * L.Sf + (me.dy - o.dy);
@@ -854,12 +824,8 @@ __device__ void damage_rate(DevState& s, const osph_material& m, size_t i) {
 // EquationHolder::finalize in reverse term order (EquationTerm.h:293-297): AdaptiveSmoothingLength
 // (EquationTerm.cpp:366-387) or ConstSmoothingLength (:427-434), ContinuityEquation (:289-299),
 // SolidStressForce (:177-203), PressureForce (:90-99); then the material finalize (damage).
-__global__ void __launch_bounds__(BLOCK) k_finalize(DevState s, const osph_material* __restrict__ mats, Params prm,
-                                                    size_t n) {
+__device__ void finalize_at(DevState& s, const osph_material& m, const Params& prm, size_t i) {
 #pragma clang fp contract(off)
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const osph_material m = mats[s.u[MAT][i]];
     const double rho = s.d[RHO][i], divv = s.d[DIVV][i];
     const double h = s.d[H][i];
     s.d[VH][i] = (prm.adaptive_h && h > 2.0 * prm.h_lo) ? h / 3.0 * divv : 0.0;
@@ -885,6 +851,13 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(DevState s, const osph_mater
     if (prm.has_damage && m.damage == 1) damage_rate(s, m, i);
 }
 
+__global__ void __launch_bounds__(BLOCK) k_finalize(DevState s, const osph_material* __restrict__ mats, Params prm,
+                                                    size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    finalize_at(s, mats[s.u[MAT][i]], prm, i);
+}
+
 // Storage::zeroHighestDerivatives (core/quantities/Storage.cpp:594-602)
 __global__ void __launch_bounds__(BLOCK) k_zero_derivatives(DevState s, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -903,11 +876,8 @@ __device__ __forceinline__ void clamp_deriv(double& v, double& dv, double lo, do
 
 // PredictorCorrector::makePredictions (core/timestepping/TimeStepping.cpp:286-300), then the swap of
 // the highest derivatives into the predictions storage and Storage::zeroHighestDerivatives (:331-334).
-__global__ void __launch_bounds__(BLOCK) k_predict(DevState s, const osph_material* __restrict__ mats, Params prm,
-                                                   double dt, size_t n) {
+__device__ void predict_at(DevState& s, const osph_material& m, const Params& prm, double dt, size_t i) {
 #pragma clang fp contract(off)
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     const double dt2 = 0.5 * (dt * dt);
     for (int k = 0; k < 3; ++k) {
         const double a = s.d[AX + k][i];
@@ -918,7 +888,6 @@ __global__ void __launch_bounds__(BLOCK) k_predict(DevState s, const osph_materi
         s.d[AX + k][i] = 0.0;
     }
     s.d[H][i] += s.d[VH][i] * dt + 0.0 * dt2;
-    const osph_material m = mats[s.u[MAT][i]];
     {
         double x = s.d[RHO][i], dx = s.d[DRHO][i];
         x += dx * dt;
@@ -1019,13 +988,10 @@ __device__ void reduce_criteria(double v[4], unsigned long long* mins) {
 }
 
 // PredictorCorrector::makeCorrections (TimeStepping.cpp:302-322): pdv is the new derivative, cdv the
-// saved prediction; the clamp zeroes the new derivative at the bounds.  Fused with the criteria.
-__global__ void __launch_bounds__(BLOCK) k_correct(DevState s, const osph_material* __restrict__ mats, Params prm,
-                                                   double dt, unsigned long long* mins, size_t n) {
+// saved prediction; the clamp zeroes the new derivative at the bounds.
+__device__ void correct_at(DevState& s, const osph_material& m, const Params& prm, double dt, size_t i) {
 #pragma clang fp contract(off)
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    double v[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
-    if (i < n) {
+    {
         const double dt2 = 0.5 * (dt * dt);
         const double ca = 1.0 / 3.0, cb = 0.5;
         for (int k = 0; k < 3; ++k) {
@@ -1033,7 +999,6 @@ __global__ void __launch_bounds__(BLOCK) k_correct(DevState s, const osph_materi
             s.d[X + k][i] -= ca * (cdv - pdv) * dt2;
             s.d[VX + k][i] -= cb * (cdv - pdv) * dt;
         }
-        const osph_material m = mats[s.u[MAT][i]];
         {
             double x = s.d[RHO][i], px = s.d[DRHO][i];
             x -= 0.5 * (s.d[PDRHO][i] - px) * dt;
@@ -1060,9 +1025,46 @@ __global__ void __launch_bounds__(BLOCK) k_correct(DevState s, const osph_materi
             s.d[D][i] = x;
             s.d[DD][i] = px;
         }
+    }
+}
+
+// The end of a predictor-corrector step for particle i: EquationHolder::finalize and the material
+// finalize, the corrections, then the criteria of the corrected state, reduced into mins[0..3].  When
+// the neighbour lists overflowed (*overflow != 0) the kernel changes nothing: the step is repeated with
+// larger lists from the same predicted state.
+__global__ void __launch_bounds__(BLOCK) k_finalize_correct(DevState s, const osph_material* __restrict__ mats,
+                                                            Params prm, double dt, unsigned long long* mins,
+                                                            const uint32_t* overflow, size_t n) {
+    if (*overflow) return;  // uniform over the grid
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double v[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
+    if (i < n) {
+        const osph_material m = mats[s.u[MAT][i]];
+        finalize_at(s, m, prm, i);
+        correct_at(s, m, prm, dt, i);
         criteria_values(s, m, prm, i, v);
     }
     reduce_criteria(v, mins);
+}
+
+// One pass before the neighbour search, MODE 2: predictions (predict_at) then the material initialize;
+// MODE 1: the material initialize only; MODE 0: neither.  Every mode accumulates the per-class grid
+// bounds of the resulting positions, with the class of h relative to href.
+template <int MODE>
+__global__ void __launch_bounds__(BLOCK) k_advance(DevState s, const osph_material* __restrict__ mats, Params prm,
+                                                   double dt, double href, unsigned long long* acc, size_t n) {
+    ClassAcc ca;
+    ca.init();
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        if (MODE > 0) {
+            const osph_material m = mats[s.u[MAT][i]];
+            if (MODE > 1) predict_at(s, m, prm, dt, i);
+            material_init_at(s, m, prm, i);
+        }
+        const double h = s.d[H][i];
+        ca.add(h_class(h, href, MAX_CLASSES), s.d[X][i], s.d[Y][i], s.d[Z][i], h);
+    }
+    ca.reduce(acc);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_criteria(DevState s, const osph_material* __restrict__ mats, Params prm,
@@ -1131,9 +1133,8 @@ void launch_stream_copy(hipStream_t stream, const void* a, void* b, size_t bytes
 
 // ---- launchers ----------------------------------------------------------------------------------------
 
-void launch_material_init(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n) {
-    if (!n) return;
-    k_material_init<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, n);
+void launch_init_small(const Launch& l, unsigned long long* small) {
+    k_init_small<<<1, SMALL_WORDS, 0, l.stream>>>(small);
 }
 
 void launch_bounds_reduce(const Launch& l, DevState s, unsigned long long* acc, size_t n) {
@@ -1160,32 +1161,38 @@ void launch_cell_ranges(const Launch& l, const uint32_t* keys, Grid g, uint32_t*
     k_cell_ranges<<<grid_for(n), BLOCK, 0, l.stream>>>(keys, g, cstart, cend, n);
 }
 
-void launch_class_bounds(const Launch& l, DevState s, int nclass, double hmin, unsigned long long* acc, size_t n) {
+void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
+                    double href, unsigned long long* acc, size_t n) {
     if (!n) return;
     unsigned g = grid_for(n);
-    g = g > 1024 ? 1024 : g;
-    k_class_bounds<<<g, BLOCK, 0, l.stream>>>(s, nclass, hmin, acc, n);
+    g = g > 2048 ? 2048 : g;
+    if (mode == 2) {
+        k_advance<2><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, acc, n);
+    } else if (mode == 1) {
+        k_advance<1><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, acc, n);
+    } else {
+        k_advance<0><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, acc, n);
+    }
 }
 
 void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, double4* pos,
-                 uint32_t* group, size_t n) {
+                 float4* posf, Screen sc, uint32_t* group, size_t n) {
     if (!n) return;
-    k_pack<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, order, rec, pos, group, n);
+    k_pack<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, order, rec, pos, posf, sc, group, n);
 }
 
-void launch_neighbor_lists(const Launch& l, const double4* pos, Grid g, const uint32_t* cstart, const uint32_t* cend,
-                           uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow, uint32_t* tile_idx,
-                           uint32_t* tile_cnt, size_t n) {
+void launch_neighbor_lists(const Launch& l, const double4* pos, const float4* posf, Screen sc, Grid g,
+                           const uint32_t* cstart, const uint32_t* cend, uint32_t* list, uint32_t* count, uint32_t kmax,
+                           uint32_t* overflow, size_t n) {
     if (!n) return;
-    if (tile_idx) {
-        // the tile decomposition fixes the block size
-        k_neighbor_lists<true><<<grid_for(n, TILE_BLOCK), TILE_BLOCK, 0, l.stream>>>(
-            pos, g, cstart, cend, list, count, kmax, overflow, tile_idx, tile_cnt, n);
-        return;
-    }
     const int b = l.tune.lists_block;
-    k_neighbor_lists<false><<<grid_for(n, b), b, size_t(l.tune.lists_lds), l.stream>>>(
-        pos, g, cstart, cend, list, count, kmax, overflow, nullptr, nullptr, n);
+    if (sc.on) {
+        k_neighbor_lists<true><<<grid_for(n, b), b, size_t(l.tune.lists_lds), l.stream>>>(
+            pos, posf, sc, g, cstart, cend, list, count, kmax, overflow, n);
+    } else {
+        k_neighbor_lists<false><<<grid_for(n, b), b, size_t(l.tune.lists_lds), l.stream>>>(
+            pos, posf, sc, g, cstart, cend, list, count, kmax, overflow, n);
+    }
 }
 
 void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
@@ -1198,23 +1205,17 @@ void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* r
 }
 
 void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
-                       double4* pos, uint32_t* group, size_t n) {
+                       double4* pos, float4* posf, Screen sc, uint32_t* group, size_t n) {
     if (!n) return;
-    k_pack_mixed<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, lat, order, rec, pos, group, n);
+    k_pack_mixed<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, lat, order, rec, pos, posf, sc, group, n);
 }
 
 void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
-                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const uint32_t* tile_idx,
-                        const uint32_t* tile_cnt, uint32_t kmax, size_t n) {
+                        const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n) {
     if (!n) return;
-    if (tile_idx) {
-        k_pairs_mixed<true><<<grid_for(n, TILE_BLOCK), TILE_BLOCK, 0, l.stream>>>(s, prm, lat, rec, order, list, count,
-                                                                                  tile_idx, tile_cnt, kmax, n);
-    } else {
-        const int b = l.tune.pairs_block;
-        k_pairs_mixed<false><<<grid_for(n, b), b, size_t(l.tune.pairs_lds), l.stream>>>(
-            s, prm, lat, rec, order, list, count, nullptr, nullptr, kmax, n);
-    }
+    const int b = l.tune.pairs_block;
+    k_pairs_mixed<<<grid_for(n, b), b, size_t(l.tune.pairs_lds), l.stream>>>(s, prm, lat, rec, order, list, count,
+                                                                              kmax, n);
 }
 
 void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n) {
@@ -1227,15 +1228,10 @@ void launch_zero_derivatives(const Launch& l, DevState s, size_t n) {
     k_zero_derivatives<<<grid_for(n), BLOCK, 0, l.stream>>>(s, n);
 }
 
-void launch_predict(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt, size_t n) {
+void launch_finalize_correct(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt,
+                             unsigned long long* mins, const uint32_t* overflow, size_t n) {
     if (!n) return;
-    k_predict<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, n);
-}
-
-void launch_correct(const Launch& l, DevState s, const osph_material* mats, Params prm, double dt,
-                    unsigned long long* mins, size_t n) {
-    if (!n) return;
-    k_correct<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, mins, n);
+    k_finalize_correct<<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, mins, overflow, n);
 }
 
 void launch_criteria(const Launch& l, DevState s, const osph_material* mats, Params prm, unsigned long long* mins,

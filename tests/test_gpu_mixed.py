@@ -19,8 +19,7 @@ def total_energy(st):
 
 
 @pytest.mark.parametrize("name", ["impact_exercised", "fluid_exercised"])
-@pytest.mark.parametrize("tiles", [0, 1])
-def test_mixed_derivatives_match_fp64(name, tiles):
+def test_mixed_derivatives_match_fp64(name):
     from cases import exercised_fluid, exercised_impact
 
     st, cfg = {"impact_exercised": exercised_impact, "fluid_exercised": exercised_fluid}[name]()
@@ -29,7 +28,6 @@ def test_mixed_derivatives_match_fp64(name, tiles):
     a.integrate()
     da = a.download(st)
     b = gpu.GpuSolver(st, cfg, precision="mixed")
-    b.tune(pair_tiles=tiles)
     b.integrate()
     db = b.download(st)
     assert np.array_equal(da["ncnt"], db["ncnt"])
@@ -57,9 +55,11 @@ def test_mixed_impact_energy_over_100_steps():
 
 
 @pytest.mark.parametrize("name", ["impact_exercised", "mixed_h"])
-@pytest.mark.parametrize("tiles", [0, 1])
-def test_mixed_neighbour_tiles_keep_pairs_exact(oracle, name, tiles):
-    """The mixed path's lists refer to LDS tile slots; decoded, they are the exact pair set."""
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("screen", [0, 1])
+def test_screened_lists_keep_pairs_exact(oracle, name, precision, screen):
+    """The FP32-screened neighbour test (with FP64 re-tests inside its error band) gives the exact pair
+    set in both precisions."""
     from cases import exercised_impact
 
     if name == "mixed_h":
@@ -68,9 +68,9 @@ def test_mixed_neighbour_tiles_keep_pairs_exact(oracle, name, tiles):
         st["r"][:, 3] *= np.exp2(rng.uniform(-2.0, 2.0, size=st.n))
     else:
         st, cfg = exercised_impact()
-    print(f"[case] mixed tiles {name} n={st.n}", flush=True)
-    g = gpu.GpuSolver(st, cfg, precision="mixed")
-    g.tune(pair_tiles=tiles)
+    print(f"[case] screened lists {name} {precision} screen={screen} n={st.n}", flush=True)
+    g = gpu.GpuSolver(st, cfg, precision=precision)
+    g.tune(lists_screen=screen)
     pi, pj = g.pairs()
     oi, oj = oracle.pairs(st["r"])
     assert np.array_equal(pi, oi) and np.array_equal(pj, oj)

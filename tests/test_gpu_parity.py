@@ -143,19 +143,46 @@ def test_empty_and_single_particle():
     assert g.pairs()[0].size == 0
 
 
-def test_pair_cutoff_boundary(oracle):
-    """Two particles exactly at |r| = h_i + h_j are not neighbours; one ulp closer they are."""
+@pytest.mark.parametrize("screen", [0, 1])
+def test_pair_cutoff_boundary(oracle, screen):
+    """Two particles exactly at |r| = h_i + h_j are not neighbours; one ulp closer they are (also
+    through the FP32-screened list kernel, where both cases fall inside its error band)."""
     st, cfg = initial.fluid_sphere(n_requested=200, radius=100.0)
     two = st.subset([0, 1])
     two["r"][:] = [[0.0, 0.0, 0.0, 1.0], [2.0, 0.0, 0.0, 1.0]]
-    _print_case("boundary", two)
+    _print_case(f"boundary/screen={screen}", two)
     g = gpu.GpuSolver(two, cfg)
+    g.tune(lists_screen=screen)
     assert g.pairs()[0].size == 0
     two["r"][1, 0] = np.nextafter(2.0, 0.0)
     g.upload(two)
     pi, pj = g.pairs()
     assert list(zip(pi, pj)) == [(0, 1)]
     assert oracle.pairs(two["r"])[0].tolist() == [0]
+
+
+@pytest.mark.parametrize("origin", [0.0, 3.0e4])
+def test_pairs_lattice_at_cutoff(oracle, origin):
+    """A cubic lattice of spacing 1 with h = 1/2 puts every nearest-neighbour pair exactly at the cutoff;
+    perturbations of a few ulps make about half of them neighbours.  Far from the origin the FP32 screen
+    band widens with the domain-to-h ratio.  Both list kernels must give the oracle's set."""
+    rng = np.random.default_rng(31)
+    m = 22
+    g1 = np.arange(m, dtype=np.float64)
+    xyz = np.stack(np.meshgrid(g1, g1, g1, indexing="ij"), axis=-1).reshape(-1, 3) + origin
+    st, cfg = initial.fluid_sphere(n_requested=200, radius=100.0)
+    lat = st.subset(np.zeros(len(xyz), dtype=np.int64))
+    ulps = rng.integers(-3, 4, size=xyz.shape)
+    lat["r"][:, :3] = xyz + ulps * np.spacing(np.maximum(np.abs(xyz), 1.0))
+    lat["r"][:, 3] = 0.5
+    oi, oj = oracle.pairs(lat["r"])
+    assert 0.2 * 3 * len(xyz) < len(oi) < 0.8 * 3 * len(xyz)
+    for screen in (0, 1):
+        _print_case(f"lattice_cutoff/{origin}/screen={screen}", lat)
+        g = gpu.GpuSolver(lat, cfg)
+        g.tune(lists_screen=screen)
+        pi, pj = g.pairs()
+        assert np.array_equal(pi, oi) and np.array_equal(pj, oj), screen
 
 
 @pytest.mark.slow
@@ -170,6 +197,10 @@ def test_full_size_impact_exact_pairs_and_derivatives(oracle):
     pi, pj = g.pairs()
     oi, oj = oracle.pairs(st["r"])
     assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
+    g.tune(lists_screen=1)
+    si, sj = g.pairs()
+    assert np.array_equal(si, oi) and np.array_equal(sj, oj)
+    g.tune(lists_screen=-1)
     g.upload(st)
     g.integrate()
     a = g.download(st)
