@@ -1,0 +1,1536 @@
+// Self-gravity of OpenSPH on gfx950 (MI355X): BarnesHut (core/gravity/BarnesHut.cpp) on a binary radix
+// tree, BruteForceGravity (core/gravity/BruteForceGravity.h), and resident LeapFrog steps with the
+// gravity-only HardSphereSolver (core/timestepping/TimeStepping.cpp:352-391, core/gravity/NBodySolver.cpp:97-117).
+//
+// Build (IGravity::build): the particles are sorted by the 63-bit Morton code of their position; the
+// binary radix tree of the sorted codes (internal node i, 0 <= i < n-1, with one end of its range at
+// sorted position i; particle k is node n-1+k) is built level-free in one pass, then the node moments
+// are accumulated bottom-up.  Each node carries its box, centre of mass, opening radius
+// r_open = 2/sqrt(3) |r_max| / theta (BarnesHut.cpp:411-414), and the traceless quadrupole and octupole
+// about its centre of mass, combined from its two children with the parallel-axis theorem
+// (BarnesHut.cpp:433-486, Moments.h:176-280).  A node of at most leaf_size particles is a leaf: it is
+// never opened, and its particles are summed exactly when it is not approximated.
+//
+// Evaluation (IGravity::evalSelfGravity): every leaf (a node of <= leaf_size particles whose parent is
+// larger) is a target.  Its interaction lists follow the reference's check-list walk (BarnesHut.cpp:212-312):
+// a node whose opening ball does not overlap the target's box goes to the node list; a leaf whose ball
+// overlaps goes to the particle list; other nodes are opened.  Nodes containing the target are always
+// opened and the target's own pairs are summed separately, as the reference's walk never holds a node
+// together with its ancestors.  The walk runs one lane per target over skip links (no stack); the sums
+// run one wavefront per target, its lanes split between the target's particles and the list entries.
+#include "osph_device.h"
+#include "osph_gpu.h"
+#include "osph_gravity.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace osph {
+namespace {
+
+constexpr uint32_t END = 0xffffffffu;
+constexpr int LUT_ENTRIES = 40000;  // LutKernel<3>::NEntries (core/sph/kernel/Kernel.h:53)
+// Bound on the depth of the radix tree: 63 bits of Morton code plus 31 bits of position tie-break.
+constexpr int MAX_DEPTH = 96;
+
+// Plane indices of the resident state (storage order).
+enum GPlane : int { GX, GY, GZ, GH, GVX, GVY, GVZ, GVH, GAX, GAY, GAZ, GM, NGPLANES };
+
+// Traceless moments about the node's centre of mass; q2 = { xx, xy, xz, yy, yz } and
+// q3 = { xxx, xxy, xxz, xyy, xyz, yyy, yyz } (the other components follow from the vanishing traces).
+struct NodeMom {
+    double M;
+    double q2[5];
+    double q3[7];
+    double pad[3];
+};
+static_assert(sizeof(NodeMom) == 128, "NodeMom is 128 bytes");
+
+struct Tree {
+    uint32_t n;        // particles; internal nodes 0..n-2, particle k is node n-1+k
+    uint32_t leaf;     // leaf size
+    uint32_t* left;    // [n-1]
+    uint32_t* right;   // [n-1]
+    uint32_t* parent;  // [2n-1]
+    uint32_t* skip;    // [2n-1]: the node after this subtree in depth-first order
+    uint32_t* first;   // [n-1]
+    uint32_t* count;   // [n-1]
+    double4* geo;      // [n-1]: centre of mass, r_open
+    double4* lo;       // [n-1]: box
+    double4* hi;
+    NodeMom* mom;      // [n-1]
+    uint32_t* flag;    // [n-1]: arrivals of the bottom-up pass
+    const double4* pos;  // sorted x, y, z, h
+    const double* gm;    // sorted G m
+    double thetaInv;
+    uint32_t* error;     // bit 0: a climb exceeded MAX_DEPTH; bit 1: a walk exceeded 2n visits
+};
+
+struct Planes {
+    double* d[NGPLANES];
+};
+
+__device__ __forceinline__ bool is_particle(const Tree& t, uint32_t id) { return id >= t.n - 1; }
+
+// ---- build ------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLOCK) k_bounds3(Planes s, unsigned long long* acc, size_t n) {
+    unsigned long long v[6] = { ~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull };
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            const unsigned long long b = ord_bits(s.d[GX + k][i]);
+            v[k] = b < v[k] ? b : v[k];
+            v[3 + k] = b > v[3 + k] ? b : v[3 + k];
+        }
+    }
+    __shared__ unsigned long long red[BLOCK / 64][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 6; ++k) {
+        const unsigned long long r = k < 3 ? wave_min(v[k]) : wave_max(v[k]);
+        if (lane == 0) red[wave][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        unsigned long long r = red[0][k];
+        for (int w = 1; w < BLOCK / 64; ++w) r = k < 3 ? (red[w][k] < r ? red[w][k] : r) : (red[w][k] > r ? red[w][k] : r);
+        if (k < 3) {
+            atomicMin(&acc[k], r);
+        } else {
+            atomicMax(&acc[k], r);
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t spread21(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_morton(Planes s, double lo0, double lo1, double lo2, double scale,
+                                                  uint64_t* keys, uint32_t* idx, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double lo[3] = { lo0, lo1, lo2 };
+    uint64_t key = 0;
+    for (int k = 0; k < 3; ++k) {
+        double q = floor((s.d[GX + k][i] - lo[k]) * scale);
+        q = q < 0.0 ? 0.0 : (q > 2097151.0 ? 2097151.0 : q);
+        key |= spread21(uint64_t(q)) << k;
+    }
+    keys[i] = key;
+    idx[i] = uint32_t(i);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_gather(Planes s, const uint32_t* __restrict__ order, double G,
+                                                  double4* __restrict__ pos, double* __restrict__ gm, size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = order[k];
+    pos[k] = make_double4(s.d[GX][i], s.d[GY][i], s.d[GZ][i], s.d[GH][i]);
+    gm[k] = G * s.d[GM][i];  // BarnesHut::build premultiplies the masses (BarnesHut.cpp:58-60)
+}
+
+// Common-prefix length of sorted codes i and j; equal codes are told apart by their positions.
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ keys, long long i, long long j, long long n) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = keys[i], b = keys[j];
+    if (a == b) return 64 + __clz(uint32_t(i) ^ uint32_t(j));
+    return __clzll(a ^ b);
+}
+
+// Internal node i of the binary radix tree of the sorted codes: its range, split and children.
+__global__ void __launch_bounds__(BLOCK) k_radix_tree(Tree t, const uint64_t* __restrict__ keys) {
+    const long long n = t.n;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta(keys, i, i + 1, n) - delta(keys, i, i - 1, n)) >= 0 ? 1 : -1;
+    const int dmin_ = delta(keys, i, i - d, n);
+    long long lmax = 2;
+    while (delta(keys, i, i + lmax * d, n) > dmin_) lmax <<= 1;
+    long long l = 0;
+    for (long long s = lmax >> 1; s >= 1; s >>= 1) {
+        if (delta(keys, i, i + (l + s) * d, n) > dmin_) l += s;
+    }
+    const long long j = i + l * d;
+    const int dnode = delta(keys, i, j, n);
+    long long split = 0;
+    for (long long div = 2;; div <<= 1) {
+        const long long s = (l + div - 1) / div;
+        if (delta(keys, i, i + (split + s) * d, n) > dnode) split += s;
+        if (s == 1) break;
+    }
+    const long long gamma = i + split * d + (d < 0 ? -1 : 0);
+    const long long lo = i < j ? i : j, hi = i < j ? j : i;
+    const uint32_t lc = uint32_t(lo == gamma ? (n - 1 + gamma) : gamma);
+    const uint32_t rc = uint32_t(hi == gamma + 1 ? (n - 1 + gamma + 1) : gamma + 1);
+    t.left[i] = lc;
+    t.right[i] = rc;
+    t.first[i] = uint32_t(lo);
+    t.count[i] = uint32_t(hi - lo + 1);
+    t.parent[lc] = uint32_t(i);
+    t.parent[rc] = uint32_t(i);
+    if (i == 0) t.parent[0] = END;
+    t.flag[i] = 0u;
+}
+
+// Depth-first successor of each node: the right sibling of the nearest ancestor-or-self that is a left
+// child, or END.
+__global__ void __launch_bounds__(BLOCK) k_skip(Tree t) {
+    const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t total = 2 * size_t(t.n) - 1;
+    if (id >= total) return;
+    uint32_t c = uint32_t(id), next = END;
+    for (int level = 0; c != 0u && level < MAX_DEPTH; ++level) {
+        const uint32_t p = t.parent[c];
+        if (t.left[p] == c) {
+            next = t.right[p];
+            break;
+        }
+        c = p;
+    }
+    t.skip[id] = next;
+}
+
+struct Moments {
+    double M, c[3], lo[3], hi[3], q2[5], q3[7];
+};
+
+__device__ void load_node(const Tree& t, uint32_t id, Moments& m) {
+    if (is_particle(t, id)) {
+        const double4 p = t.pos[id - (t.n - 1)];
+        m.M = t.gm[id - (t.n - 1)];
+        m.c[0] = m.lo[0] = m.hi[0] = p.x;
+        m.c[1] = m.lo[1] = m.hi[1] = p.y;
+        m.c[2] = m.lo[2] = m.hi[2] = p.z;
+        for (int k = 0; k < 5; ++k) m.q2[k] = 0.0;
+        for (int k = 0; k < 7; ++k) m.q3[k] = 0.0;
+        return;
+    }
+    const double4 g = t.geo[id], lo = t.lo[id], hi = t.hi[id];
+    const NodeMom& q = t.mom[id];
+    m.M = q.M;
+    m.c[0] = g.x;
+    m.c[1] = g.y;
+    m.c[2] = g.z;
+    m.lo[0] = lo.x;
+    m.lo[1] = lo.y;
+    m.lo[2] = lo.z;
+    m.hi[0] = hi.x;
+    m.hi[1] = hi.y;
+    m.hi[2] = hi.z;
+    for (int k = 0; k < 5; ++k) m.q2[k] = q.q2[k];
+    for (int k = 0; k < 7; ++k) m.q3[k] = q.q3[k];
+}
+
+// Adds the moments of child m (shifted by d = com_child - com_parent) to q2 and q3: the parallel-axis
+// theorem for traceless moments (Moments.h:176-186, 262-280).  With t = q2 d:
+//   q2 += q2c + M (d d - |d|^2/3 I)
+//   q3 += q3c + M (d d d - |d|^2/5 sym(I d)) + sym(q2c d) - 2/5 sym(I t)
+// where sym sums the three distinct index placements.
+__device__ void add_shifted(const Moments& m, const double d[3], double q2[5], double q3[7]) {
+    const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double M = m.M;
+    q2[0] += m.q2[0] + M * (d[0] * d[0] - dd / 3.0);
+    q2[1] += m.q2[1] + M * (d[0] * d[1]);
+    q2[2] += m.q2[2] + M * (d[0] * d[2]);
+    q2[3] += m.q2[3] + M * (d[1] * d[1] - dd / 3.0);
+    q2[4] += m.q2[4] + M * (d[1] * d[2]);
+    // full child quadrupole
+    const double cxx = m.q2[0], cxy = m.q2[1], cxz = m.q2[2], cyy = m.q2[3], cyz = m.q2[4], czz = -cxx - cyy;
+    const double Q[3][3] = { { cxx, cxy, cxz }, { cxy, cyy, cyz }, { cxz, cyz, czz } };
+    double tv[3];
+    for (int a = 0; a < 3; ++a) tv[a] = Q[a][0] * d[0] + Q[a][1] * d[1] + Q[a][2] * d[2];
+    // independent components (i, j, k) of q3
+    const int I3[7][3] = { { 0, 0, 0 }, { 0, 0, 1 }, { 0, 0, 2 }, { 0, 1, 1 }, { 0, 1, 2 }, { 1, 1, 1 }, { 1, 1, 2 } };
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+        const int i = I3[c][0], j = I3[c][1], k = I3[c][2];
+        const double dij = i == j ? 1.0 : 0.0, dik = i == k ? 1.0 : 0.0, djk = j == k ? 1.0 : 0.0;
+        const double red = d[i] * d[j] * d[k] - dd / 5.0 * (dij * d[k] + dik * d[j] + djk * d[i]);
+        const double perm = Q[i][j] * d[k] + Q[i][k] * d[j] + Q[j][k] * d[i];
+        const double permt = dij * tv[k] + dik * tv[j] + djk * tv[i];
+        q3[c] += m.q3[c] + M * red + perm - 0.4 * permt;
+    }
+}
+
+// Bottom-up moments: one thread per particle climbs while it is the second child to arrive.
+__global__ void __launch_bounds__(BLOCK) k_moments(Tree t) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= t.n) return;
+    uint32_t node = t.parent[t.n - 1 + k];
+    for (int level = 0; node != END; ++level) {
+        if (level >= MAX_DEPTH) {
+            atomicOr(t.error, 1u);
+            return;
+        }
+        __threadfence();
+        if (atomicAdd(&t.flag[node], 1u) == 0u) return;
+        __threadfence();
+        Moments a, b;
+        load_node(t, t.left[node], a);
+        load_node(t, t.right[node], b);
+        const double M = a.M + b.M;
+        double c[3], lo[3], hi[3];
+        for (int q = 0; q < 3; ++q) {
+            lo[q] = dmin(a.lo[q], b.lo[q]);
+            hi[q] = dmax(a.hi[q], b.hi[q]);
+        }
+        NodeMom out{};
+        double r_open = 0.0;
+        if (M == 0.0) {
+            c[0] = c[1] = c[2] = 0.0;
+        } else {
+            // BarnesHut::buildInner (BarnesHut.cpp:447-468)
+            for (int q = 0; q < 3; ++q) c[q] = (a.M * a.c[q] + b.M * b.c[q]) / M;
+            double rm[3];
+            for (int q = 0; q < 3; ++q) rm[q] = dmax(c[q] - lo[q], hi[q] - c[q]);
+            r_open = 2.0 / sqrt(3.0) * t.thetaInv * sqrt(rm[0] * rm[0] + rm[1] * rm[1] + rm[2] * rm[2]);
+            out.M = M;
+            double da[3], db[3];
+            for (int q = 0; q < 3; ++q) {
+                da[q] = a.c[q] - c[q];
+                db[q] = b.c[q] - c[q];
+            }
+            add_shifted(a, da, out.q2, out.q3);
+            add_shifted(b, db, out.q2, out.q3);
+        }
+        t.geo[node] = make_double4(c[0], c[1], c[2], r_open);
+        t.lo[node] = make_double4(lo[0], lo[1], lo[2], 0.0);
+        t.hi[node] = make_double4(hi[0], hi[1], hi[2], 0.0);
+        t.mom[node] = out;
+        node = t.parent[node];
+    }
+}
+
+// Leaves (targets): nodes of at most leaf_size particles whose parent is larger, and particles whose
+// parent is larger.  leaf_at[first] = node id.
+__global__ void __launch_bounds__(BLOCK) k_mark_leaves(Tree t, uint32_t* leaf_at) {
+    const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t total = 2 * size_t(t.n) - 1;
+    if (id >= total) return;
+    const uint32_t p = id == 0 ? END : t.parent[id];
+    const uint32_t pc = p == END ? 0xffffffffu : t.count[p];
+    if (is_particle(t, uint32_t(id))) {
+        if (pc > t.leaf) leaf_at[id - (t.n - 1)] = uint32_t(id);
+    } else if (t.count[id] <= t.leaf && pc > t.leaf) {
+        leaf_at[t.first[id]] = uint32_t(id);
+    }
+}
+
+struct NotEnd {
+    __host__ __device__ bool operator()(uint32_t v) const { return v != END; }
+};
+
+// Sphere::overlaps (core/objects/geometry/Sphere.h:73-79): the ball (c, r) and the box share a point.
+__device__ __forceinline__ bool ball_overlaps_box(double4 g, const double lo[3], const double hi[3]) {
+#pragma clang fp contract(off)
+    const double c[3] = { g.x, g.y, g.z };
+    double l2 = 0.0, r2 = 0.0;
+    double lv[3], rv[3];
+    for (int k = 0; k < 3; ++k) {
+        lv[k] = dmax(lo[k] - c[k], 0.0);
+        rv[k] = dmax(c[k] - hi[k], 0.0);
+    }
+    l2 = lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2];
+    r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+    return g.w * g.w - l2 - r2 > 0.0;
+}
+
+__device__ __forceinline__ void node_range(const Tree& t, uint32_t id, uint32_t& f, uint32_t& c) {
+    if (is_particle(t, id)) {
+        f = id - (t.n - 1);
+        c = 1;
+    } else {
+        f = t.first[id];
+        c = t.count[id];
+    }
+}
+
+// Interaction lists of one target per lane (BarnesHut::evalNode, BarnesHut.cpp:212-312, applied to the
+// target leaf's box).  Lists of target q live at nlist[q * cap_n ...] and xlist[q * cap_x ...]; counts
+// beyond the capacity are reported so that the host can grow the lists and walk again.
+__global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restrict__ targets, uint32_t ntargets,
+                                                uint32_t* __restrict__ nlist, uint32_t cap_n,
+                                                uint32_t* __restrict__ xlist, uint32_t cap_x,
+                                                uint32_t* __restrict__ ncount, uint32_t* __restrict__ xcount,
+                                                uint32_t* need) {
+    const uint32_t q = uint32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= ntargets) return;
+    const uint32_t T = targets[q];
+    uint32_t tf, tc;
+    node_range(t, T, tf, tc);
+    double lo[3], hi[3];
+    if (is_particle(t, T)) {
+        const double4 p = t.pos[tf];
+        lo[0] = hi[0] = p.x;
+        lo[1] = hi[1] = p.y;
+        lo[2] = hi[2] = p.z;
+    } else {
+        const double4 a = t.lo[T], b = t.hi[T];
+        lo[0] = a.x;
+        lo[1] = a.y;
+        lo[2] = a.z;
+        hi[0] = b.x;
+        hi[1] = b.y;
+        hi[2] = b.z;
+    }
+    uint32_t nn = 0, nx = 0;
+    uint32_t* nl = nlist + size_t(q) * cap_n;
+    uint32_t* xl = xlist + size_t(q) * cap_x;
+    uint32_t id = 0;  // the root: internal node 0, or particle 0 when n == 1
+    // a depth-first walk visits each of the 2n - 1 nodes at most once
+    for (size_t visits = 0; id != END; ++visits) {
+        if (visits > 2 * size_t(t.n)) {
+            atomicOr(t.error, 2u);
+            break;
+        }
+        if (id == T) {
+            id = t.skip[id];
+            continue;
+        }
+        if (is_particle(t, id)) {
+            // a single-particle leaf has r_open = 0: always summed pair by pair (BarnesHut.cpp:237-243)
+            if (nx < cap_x) xl[nx] = id;
+            ++nx;
+            id = t.skip[id];
+            continue;
+        }
+        const uint32_t f = t.first[id], c = t.count[id];
+        if (f <= tf && tf + tc <= f + c) {  // an ancestor of the target
+            id = t.left[id];
+            continue;
+        }
+        const bool is_leaf = c <= t.leaf;
+        const double4 g = t.geo[id];
+        if (is_leaf && g.w == 0.0) {
+            if (nx < cap_x) xl[nx] = id;
+            ++nx;
+            id = t.skip[id];
+        } else if (g.w == 0.0) {
+            // more than leaf_size coincident particles: no opening ball; their expansion is singular
+            // for a target at the same point, so the node is opened down to its leaves
+            id = t.left[id];
+        } else if (!ball_overlaps_box(g, lo, hi)) {
+            if (nn < cap_n) nl[nn] = id;
+            ++nn;
+            id = t.skip[id];
+        } else if (is_leaf) {
+            if (nx < cap_x) xl[nx] = id;
+            ++nx;
+            id = t.skip[id];
+        } else {
+            id = t.left[id];
+        }
+    }
+    ncount[q] = nn;
+    xcount[q] = nx;
+    if (nn > cap_n) atomicMax(&need[0], nn);
+    if (nx > cap_x) atomicMax(&need[1], nx);
+}
+
+// ---- evaluation ---------------------------------------------------------------------------------------
+
+// GravityLutKernel::grad (core/sph/kernel/GravityKernel.h:60-70) times the source mass, added to a.
+// grads == nullptr is the point-particle kernel (radius 0).
+__device__ __forceinline__ void add_pair(const double* __restrict__ grads, double rx, double ry, double rz, double h,
+                                         double m, double a[3]) {
+#pragma clang fp contract(off)
+    if (grads) {
+        const double hInv = 1.0 / h;
+        const double sx = rx * hInv, sy = ry * hInv, sz = rz * hInv;
+        const double qSqr = sx * sx + sy * sy + sz * sz;
+        if (!(qSqr + EPS_D >= 4.0)) {
+            // LutKernel<3>::gradImpl (core/sph/kernel/Kernel.h:129-144)
+            const double fi = double(LUT_ENTRIES / 4) * qSqr;
+            const uint32_t i1 = uint32_t(fi);
+            const double ratio = fi - double(i1);
+            const double gr = grads[i1] * (1.0 - ratio) + grads[i1 + 1] * ratio;
+            const double h3 = hInv * hInv * hInv;
+            a[0] += m * ((h3 * rx) * gr);
+            a[1] += m * ((h3 * ry) * gr);
+            a[2] += m * ((h3 * rz) * gr);
+            return;
+        }
+    }
+    const double l = sqrt(rx * rx + ry * ry + rz * rz);
+    const double l3 = l * l * l;
+    a[0] += m * (rx / l3);
+    a[1] += m * (ry / l3);
+    a[2] += m * (rz / l3);
+}
+
+// evaluateGravity (core/gravity/Moments.h:318-360) of node moments at dr = r - com, added to a.
+__device__ __forceinline__ void add_multipole(const NodeMom& q, double drx, double dry, double drz, int order,
+                                              double a[3]) {
+    const double inv = 1.0 / (drx * drx + dry * dry + drz * drz);
+    const double g0 = -sqrt(inv);
+    const double g1 = -1.0 * inv * g0;
+    const double g2 = -3.0 * inv * g1;
+    const double g3 = -5.0 * inv * g2;
+    const double g4 = -7.0 * inv * g3;
+    const double d0 = -drx, d1 = -dry, d2 = -drz;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (order >= 3) {
+        const double xxx = q.q3[0], xxy = q.q3[1], xxz = q.q3[2], xyy = q.q3[3], xyz = q.q3[4], yyy = q.q3[5],
+                     yyz = q.q3[6];
+        const double xzz = -xxx - xyy, yzz = -xxy - yyy, zzz = -xxz - yyz;
+        // Q1_i = q_ijk d_j d_k
+        const double Q1x = xxx * d0 * d0 + xyy * d1 * d1 + xzz * d2 * d2 + 2.0 * (xxy * d0 * d1 + xxz * d0 * d2 + xyz * d1 * d2);
+        const double Q1y = xxy * d0 * d0 + yyy * d1 * d1 + yzz * d2 * d2 + 2.0 * (xyy * d0 * d1 + xyz * d0 * d2 + yyz * d1 * d2);
+        const double Q1z = xxz * d0 * d0 + yyz * d1 * d1 + zzz * d2 * d2 + 2.0 * (xyz * d0 * d1 + xzz * d0 * d2 + yzz * d1 * d2);
+        const double Q0 = (Q1x * d0 + Q1y * d1 + Q1z * d2) * (1.0 / 6.0);
+        ax += g4 * d0 * Q0 + g3 * (0.5 * Q1x);
+        ay += g4 * d1 * Q0 + g3 * (0.5 * Q1y);
+        az += g4 * d2 * Q0 + g3 * (0.5 * Q1z);
+    }
+    if (order >= 2) {
+        const double xx = q.q2[0], xy = q.q2[1], xz = q.q2[2], yy = q.q2[3], yz = q.q2[4], zz = -xx - yy;
+        const double Q1x = xx * d0 + xy * d1 + xz * d2;
+        const double Q1y = xy * d0 + yy * d1 + yz * d2;
+        const double Q1z = xz * d0 + yz * d1 + zz * d2;
+        const double Q0 = 0.5 * (Q1x * d0 + Q1y * d1 + Q1z * d2);
+        ax += g3 * d0 * Q0 + g2 * Q1x;
+        ay += g3 * d1 * Q0 + g2 * Q1y;
+        az += g3 * d2 * Q0 + g2 * Q1z;
+    }
+    ax += g1 * d0 * q.M;
+    ay += g1 * d1 * q.M;
+    az += g1 * d2 * q.M;
+    a[0] += ax;
+    a[1] += ay;
+    a[2] += az;
+}
+
+// One wavefront per target leaf.  Lanes are (particle p, split s) with P = pow2 >= count particles and
+// S = 64 / P splits; split s takes every S-th entry of the lists and of the target's own pairs, and the
+// splits are summed with cross-lane adds at the end.  Accelerations are added to the state planes.
+__global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t* __restrict__ order,
+                                                const uint32_t* __restrict__ targets, uint32_t ntargets,
+                                                const uint32_t* __restrict__ nlist, uint32_t cap_n,
+                                                const uint32_t* __restrict__ xlist, uint32_t cap_x,
+                                                const uint32_t* __restrict__ ncount,
+                                                const uint32_t* __restrict__ xcount, const double* __restrict__ grads,
+                                                int order_m) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t q = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64)) + wave;
+    if (q >= ntargets) return;
+    const uint32_t T = targets[q];
+    uint32_t tf, tc;
+    node_range(t, T, tf, tc);
+    uint32_t P = 1;
+    while (P < tc) P <<= 1;
+    const uint32_t S = 64 / P;
+    const uint32_t p = lane & (P - 1), sp = lane / P;
+    const bool active = p < tc;
+    const uint32_t k = tf + (active ? p : 0);
+    const double4 me = t.pos[k];
+    double a[3] = { 0.0, 0.0, 0.0 };
+    const uint32_t nn = min(ncount[q], cap_n), nx = min(xcount[q], cap_x);
+    const uint32_t* nl = nlist + size_t(q) * cap_n;
+    const uint32_t* xl = xlist + size_t(q) * cap_x;
+    for (uint32_t c = sp; c < nn; c += S) {
+        const uint32_t id = nl[c];
+        const double4 g = t.geo[id];
+        add_multipole(t.mom[id], me.x - g.x, me.y - g.y, me.z - g.z, order_m, a);
+    }
+    for (uint32_t c = sp; c < nx; c += S) {
+        uint32_t f, cnt;
+        node_range(t, xl[c], f, cnt);
+        for (uint32_t j = f; j < f + cnt; ++j) {
+            const double4 o = t.pos[j];
+            add_pair(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), t.gm[j], a);
+        }
+    }
+    for (uint32_t j = tf + sp; j < tf + tc; j += S) {
+        if (j == k) continue;
+        const double4 o = t.pos[j];
+        add_pair(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), t.gm[j], a);
+    }
+    for (uint32_t o = P; o < 64; o <<= 1) {
+        for (int c = 0; c < 3; ++c) a[c] += __shfl_xor(a[c], int(o), 64);
+    }
+    if (active && sp == 0) {
+        const uint32_t i = order[k];
+        s.d[GAX][i] += a[0];
+        s.d[GAY][i] += a[1];
+        s.d[GAZ][i] += a[2];
+    }
+}
+
+// BarnesHut::evalImpl (BarnesHut.cpp:136-178) at arbitrary points: a node is approximated when the point
+// lies outside its box and size^2 / (distance^2 + EPS) < theta^2; leaves are summed with the unsymmetrised
+// kernel of the source particle.
+__global__ void __launch_bounds__(BLOCK) k_points(Tree t, const double4* __restrict__ pts, double4* __restrict__ acc,
+                                                  size_t np, const double* __restrict__ grads, int order_m) {
+    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const double4 r0 = pts[q];
+    double a[3] = { 0.0, 0.0, 0.0 };
+    const double th2 = 1.0 / (t.thetaInv * t.thetaInv);
+    uint32_t id = 0;
+    for (size_t visits = 0; id != END; ++visits) {
+        if (visits > 2 * size_t(t.n)) {
+            atomicOr(t.error, 2u);
+            break;
+        }
+        uint32_t f, c;
+        node_range(t, id, f, c);
+        if (!is_particle(t, id)) {
+            const double4 lo = t.lo[id], hi = t.hi[id], g = t.geo[id];
+            const bool inside = !(r0.x < lo.x || r0.x > hi.x || r0.y < lo.y || r0.y > hi.y || r0.z < lo.z || r0.z > hi.z);
+            const double sx = hi.x - lo.x, sy = hi.y - lo.y, sz = hi.z - lo.z;
+            const double cx = 0.5 * (lo.x + hi.x) - r0.x, cy = 0.5 * (lo.y + hi.y) - r0.y, cz = 0.5 * (lo.z + hi.z) - r0.z;
+            const double size2 = sx * sx + sy * sy + sz * sz;
+            const double dist2 = cx * cx + cy * cy + cz * cz;
+            if (!inside && size2 > 0.0 && size2 / (dist2 + EPS_D) < th2) {
+                add_multipole(t.mom[id], r0.x - g.x, r0.y - g.y, r0.z - g.z, order_m, a);
+                id = t.skip[id];
+                continue;
+            }
+            if (c > t.leaf) {
+                id = t.left[id];
+                continue;
+            }
+        }
+        // BarnesHut::evalExact (BarnesHut.cpp:376-387)
+        for (uint32_t j = f; j < f + c; ++j) {
+            const double4 o = t.pos[j];
+            add_pair(grads, o.x - r0.x, o.y - r0.y, o.z - r0.z, o.w, t.gm[j], a);
+        }
+        id = t.skip[id];
+    }
+    acc[q] = make_double4(a[0], a[1], a[2], 0.0);
+}
+
+// BruteForceGravity::evalImpl (BruteForceGravity.h:112-131) for targets tg[0..nt): sources in storage
+// order, staged through LDS; the per-target sum runs in source order like the reference's two loops.
+__global__ void __launch_bounds__(BLOCK) k_direct(Planes s, size_t n, const uint32_t* __restrict__ tg, size_t nt,
+                                                  const double* __restrict__ grads, double G, double4* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double4 sp[BLOCK];
+    __shared__ double sm[BLOCK];
+    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool active = q < nt;
+    const uint32_t i = active ? tg[q] : 0u;
+    const double xi = s.d[GX][i], yi = s.d[GY][i], zi = s.d[GZ][i], hi = s.d[GH][i];
+    double a[3] = { 0.0, 0.0, 0.0 };
+    for (size_t base = 0; base < n; base += BLOCK) {
+        const size_t j = base + threadIdx.x;
+        if (j < n) {
+            sp[threadIdx.x] = make_double4(s.d[GX][j], s.d[GY][j], s.d[GZ][j], s.d[GH][j]);
+            sm[threadIdx.x] = s.d[GM][j];
+        }
+        __syncthreads();
+        const size_t cnt = n - base < size_t(BLOCK) ? n - base : size_t(BLOCK);
+        if (active) {
+            for (size_t c = 0; c < cnt; ++c) {
+                if (base + c == i) continue;
+                const double4 o = sp[c];
+                add_pair(grads, o.x - xi, o.y - yi, o.z - zi, 0.5 * (o.w + hi), sm[c], a);
+            }
+        }
+        __syncthreads();
+    }
+    if (active) out[q] = make_double4(G * a[0], G * a[1], G * a[2], 0.0);
+}
+
+// BruteForceGravity::evalEnergy (BruteForceGravity.h:103-115): per-particle sums m_i m_j value.
+__global__ void __launch_bounds__(BLOCK) k_energy(Planes s, size_t n, const double* __restrict__ values,
+                                                  double* __restrict__ e_out) {
+#pragma clang fp contract(off)
+    __shared__ double4 sp[BLOCK];
+    __shared__ double sm[BLOCK];
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool active = i < n;
+    const size_t ii = active ? i : 0;
+    const double xi = s.d[GX][ii], yi = s.d[GY][ii], zi = s.d[GZ][ii], hi = s.d[GH][ii], mi = s.d[GM][ii];
+    double e = 0.0;
+    for (size_t base = 0; base < n; base += BLOCK) {
+        const size_t j = base + threadIdx.x;
+        if (j < n) {
+            sp[threadIdx.x] = make_double4(s.d[GX][j], s.d[GY][j], s.d[GZ][j], s.d[GH][j]);
+            sm[threadIdx.x] = s.d[GM][j];
+        }
+        __syncthreads();
+        const size_t cnt = n - base < size_t(BLOCK) ? n - base : size_t(BLOCK);
+        if (active) {
+            for (size_t c = 0; c < cnt; ++c) {
+                if (base + c == i) continue;
+                const double4 o = sp[c];
+                const double rx = xi - o.x, ry = yi - o.y, rz = zi - o.z, h = 0.5 * (hi + o.w);
+                double v = 0.0;
+                bool outer = true;
+                if (values) {
+                    // GravityLutKernel::value (GravityKernel.h:45-58)
+                    const double hInv = 1.0 / h;
+                    const double sx = rx * hInv, sy = ry * hInv, sz = rz * hInv;
+                    const double qSqr = sx * sx + sy * sy + sz * sz;
+                    if (!(qSqr + EPS_D >= 4.0)) {
+                        const double fi = double(LUT_ENTRIES / 4) * qSqr;
+                        const uint32_t i1 = uint32_t(fi);
+                        const double ratio = fi - double(i1);
+                        v = hInv * (values[i1] * (1.0 - ratio) + values[i1 + 1] * ratio);
+                        outer = false;
+                    }
+                }
+                if (outer) v = -1.0 / sqrt(rx * rx + ry * ry + rz * rz);
+                e += mi * sm[c] * v;
+            }
+        }
+        __syncthreads();
+    }
+    if (active) e_out[i] = e;
+}
+
+// Deterministic reductions: per-block partial sums in a fixed tree order.
+template <int NV>
+__device__ void block_sum(double (&v)[NV], double* __restrict__ out) {
+    __shared__ double red[BLOCK / 64][NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) red[wave][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double x = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) x += red[w][threadIdx.x];
+        out[size_t(blockIdx.x) * NV + threadIdx.x] = x;
+    }
+}
+
+// Integrals (core/physics/Integrals.cpp:24-180): kinetic energy, momentum, angular momentum about the
+// origin, mass-weighted position, total mass.
+__global__ void __launch_bounds__(BLOCK) k_integrals(Planes s, size_t n, double* __restrict__ partial) {
+    double v[11] = { 0.0 };
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const double m = s.d[GM][i];
+        const double x = s.d[GX][i], y = s.d[GY][i], z = s.d[GZ][i];
+        const double vx = s.d[GVX][i], vy = s.d[GVY][i], vz = s.d[GVZ][i];
+        v[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        v[1] += m * vx;
+        v[2] += m * vy;
+        v[3] += m * vz;
+        v[4] += m * (y * vz - z * vy);
+        v[5] += m * (z * vx - x * vz);
+        v[6] += m * (x * vy - y * vx);
+        v[7] += m * x;
+        v[8] += m * y;
+        v[9] += m * z;
+        v[10] += m;
+    }
+    block_sum<11>(v, partial);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sum_partials(const double* __restrict__ partial, int nblocks, int nv,
+                                                        double* __restrict__ out) {
+    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+        double x = 0.0;
+        for (int b = 0; b < nblocks; ++b) x += partial[size_t(b) * nv + k];
+        out[k] = x;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sum_vector(const double* __restrict__ v, size_t n, double* __restrict__ partial) {
+    double x[1] = { 0.0 };
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) x[0] += v[i];
+    block_sum<1>(x, partial);
+}
+
+// LeapFrog drift (TimeStepping.cpp:356-360,382-386): r += v dt / 2 in all four lanes.
+__global__ void __launch_bounds__(BLOCK) k_drift(Planes s, double dt, size_t n) {
+#pragma clang fp contract(off)
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 4; ++k) s.d[GX + k][i] += s.d[GVX + k][i] * 0.5 * dt;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_zero_acc(Planes s, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s.d[GAX][i] = 0.0;
+    s.d[GAY][i] = 0.0;
+    s.d[GAZ][i] = 0.0;
+}
+
+// HardSphereSolver zeroes the h lanes of v and dv (NBodySolver.cpp:110-114); then the kick v += dv dt
+// (TimeStepping.cpp:372-376) and the second drift.
+__global__ void __launch_bounds__(BLOCK) k_kick_drift(Planes s, double dt, size_t n) {
+#pragma clang fp contract(off)
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s.d[GVH][i] = 0.0;
+    for (int k = 0; k < 3; ++k) s.d[GVX + k][i] += s.d[GAX + k][i] * dt;
+    for (int k = 0; k < 4; ++k) s.d[GX + k][i] += s.d[GVX + k][i] * 0.5 * dt;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill_u32g(uint32_t* p, size_t count, uint32_t value) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x) {
+        p[i] = value;
+    }
+}
+
+// small[0..5] bounds, [6] list sizes needed (2 x u32), [8] traversal error flags.
+__global__ void k_init_bounds(unsigned long long* a) {
+    const int t = threadIdx.x;
+    if (t < 6) a[t] = t < 3 ? ~0ull : 0ull;
+    if (t == 6 || t == 8) a[t] = 0ull;
+}
+
+// Strided element layouts <-> planes.
+__global__ void __launch_bounds__(BLOCK) k_unstage(const unsigned char* __restrict__ st, size_t stride, int ncomp,
+                                                   Planes s, int plane0, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* e = reinterpret_cast<const double*>(st + i * stride);
+    for (int c = 0; c < ncomp; ++c) s.d[plane0 + c][i] = e[c];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_stage(unsigned char* __restrict__ st, size_t stride, int ncomp, Planes s,
+                                                 int plane0, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* e = reinterpret_cast<double*>(st + i * stride);
+    for (int c = 0; c < ncomp; ++c) e[c] = s.d[plane0 + c][i];
+}
+
+// ---- host-side gravity LUT ------------------------------------------------------------------------------
+
+// GravityKernel<CubicSpline<3>>::gradImpl / valueImpl (core/sph/kernel/GravityKernel.h:82-107) sampled
+// by the LutKernel<3> constructor (core/sph/kernel/Kernel.h:83-101).
+#pragma STDC FP_CONTRACT OFF
+double gk_value(double qSqr) {
+    const double q = std::sqrt(qSqr);
+    if (q < 1.0) {
+        const double q4 = (q * q) * (q * q), q5 = ((q * q) * (q * q)) * q;
+        return 2.0 / 3.0 * qSqr - 3.0 / 10.0 * q4 + 1.0 / 10.0 * q5 - 7.0 / 5.0;
+    }
+    const double q3 = q * q * q, q5 = ((q * q) * (q * q)) * q;
+    return 4.0 / 3.0 * qSqr - q3 + 3.0 / 10.0 * (qSqr * qSqr) - 1.0 / 30.0 * q5 - 8.0 / 5.0 + 1.0 / (15.0 * q);
+}
+
+double gk_grad(double qSqr) {
+    const double q = std::sqrt(qSqr);
+    if (q == 0.0) return 4.0 / 3.0;
+    if (q < 1.0) return 1.0 / q * (4.0 / 3.0 * q - 6.0 / 5.0 * (q * q * q) + 1.0 / 2.0 * (qSqr * qSqr));
+    return 1.0 / q * (8.0 / 3.0 * q - 3.0 * qSqr + 6.0 / 5.0 * (q * q * q) - 1.0 / 6.0 * (qSqr * qSqr) -
+                      1.0 / (15.0 * qSqr));
+}
+
+}  // namespace
+}  // namespace osph
+
+using namespace osph;
+
+struct osph_gravity {
+    osph_gravity_config cfg{};
+    hipStream_t stream = nullptr;
+    size_t n = 0, cap = 0;
+    double* dblock = nullptr;
+    Planes s{};
+    // tree
+    uint64_t *keys = nullptr, *keys_sorted = nullptr;
+    uint32_t *iota = nullptr, *order = nullptr;
+    void* sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    double4* pos = nullptr;
+    double* gm = nullptr;
+    uint32_t *left = nullptr, *right = nullptr, *parent = nullptr, *skip = nullptr, *first = nullptr, *count = nullptr,
+             *flag = nullptr;
+    double4 *geo = nullptr, *lo = nullptr, *hi = nullptr;
+    NodeMom* mom = nullptr;
+    uint32_t *leaf_at = nullptr, *targets = nullptr, *ntargets_dev = nullptr;
+    void* select_temp = nullptr;
+    size_t select_temp_bytes = 0;
+    uint32_t ntargets = 0;
+    // interaction lists
+    uint32_t *nlist = nullptr, *xlist = nullptr, *ncount = nullptr, *xcount = nullptr;
+    uint32_t cap_n = 256, cap_x = 64;
+    size_t list_targets = 0;
+    unsigned long long* small = nullptr;  // [0..5] bounds, [6] need (2 x u32)
+    unsigned long long* small_host = nullptr;
+    double *lut_grads = nullptr, *lut_values = nullptr;
+    double* scratch = nullptr;
+    size_t scratch_cap = 0;
+    unsigned char* staging = nullptr;
+    size_t staging_cap = 0;
+    bool built = false;
+    // profiling
+    bool prof = false;
+    struct Ev {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> events;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, uint64_t>> acc;
+    std::string err;
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+};
+
+namespace {
+
+#define GCHK(g, expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return (g)->fail(OSPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int OK = OSPH_OK, ERR_ARG = OSPH_ERR_ARG, ERR_HIP = OSPH_ERR_HIP, ERR_STATE = OSPH_ERR_STATE;
+
+hipEvent_t take_event(osph_gravity* g) {
+    if (!g->pool.empty()) {
+        hipEvent_t e = g->pool.back();
+        g->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+template <typename F>
+void timed(osph_gravity* g, const char* name, F&& f) {
+    if (!g->prof) {
+        f();
+        return;
+    }
+    osph_gravity::Ev ev{ name, take_event(g), take_event(g) };
+    if (ev.a) (void)hipEventRecord(ev.a, g->stream);
+    f();
+    if (ev.b) (void)hipEventRecord(ev.b, g->stream);
+    g->events.push_back(ev);
+}
+
+void resolve(osph_gravity* g) {
+    for (auto& e : g->events) {
+        float ms = 0.f;
+        if (e.a && e.b && hipEventSynchronize(e.b) == hipSuccess && hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            auto& a = g->acc[e.name];
+            a.first += ms;
+            a.second += 1;
+        }
+        if (e.a) g->pool.push_back(e.a);
+        if (e.b) g->pool.push_back(e.b);
+    }
+    g->events.clear();
+}
+
+int sync(osph_gravity* g) {
+    GCHK(g, hipStreamSynchronize(g->stream));
+    GCHK(g, hipGetLastError());
+    return OK;
+}
+
+// Synchronises and fails if a tree climb or walk ran past its bound (a malformed tree).
+int check_traversal(osph_gravity* g) {
+    GCHK(g, hipMemcpyAsync(g->small_host + 8, g->small + 8, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           g->stream));
+    if (int e = sync(g)) return e;
+    if (g->small_host[8] != 0) return g->fail(ERR_STATE, "tree traversal exceeded its bound");
+    return OK;
+}
+
+void free_particles(osph_gravity* g) {
+    void* ptrs[] = { g->dblock, g->keys, g->keys_sorted, g->iota, g->order, g->sort_temp, g->pos, g->gm, g->left,
+                     g->right, g->parent, g->skip, g->first, g->count, g->flag, g->geo, g->lo, g->hi, g->mom,
+                     g->leaf_at, g->targets, g->ntargets_dev, g->select_temp };
+    for (void* p : ptrs) (void)hipFree(p);
+    g->dblock = nullptr;
+    g->keys = g->keys_sorted = nullptr;
+    g->iota = g->order = nullptr;
+    g->sort_temp = g->select_temp = nullptr;
+    g->pos = nullptr;
+    g->gm = nullptr;
+    g->left = g->right = g->parent = g->skip = g->first = g->count = g->flag = nullptr;
+    g->geo = g->lo = g->hi = nullptr;
+    g->mom = nullptr;
+    g->leaf_at = g->targets = g->ntargets_dev = nullptr;
+    g->cap = 0;
+}
+
+int ensure_particles(osph_gravity* g, size_t n) {
+    if (n <= g->cap) return OK;
+    GCHK(g, hipStreamSynchronize(g->stream));
+    free_particles(g);
+    const size_t c = n, nodes = c > 1 ? c - 1 : 1, ids = 2 * c;
+    GCHK(g, hipMalloc(&g->dblock, sizeof(double) * NGPLANES * c));
+    GCHK(g, hipMalloc(&g->keys, sizeof(uint64_t) * c));
+    GCHK(g, hipMalloc(&g->keys_sorted, sizeof(uint64_t) * c));
+    GCHK(g, hipMalloc(&g->iota, sizeof(uint32_t) * c));
+    GCHK(g, hipMalloc(&g->order, sizeof(uint32_t) * c));
+    GCHK(g, hipMalloc(&g->pos, sizeof(double4) * c));
+    GCHK(g, hipMalloc(&g->gm, sizeof(double) * c));
+    GCHK(g, hipMalloc(&g->left, sizeof(uint32_t) * nodes));
+    GCHK(g, hipMalloc(&g->right, sizeof(uint32_t) * nodes));
+    GCHK(g, hipMalloc(&g->first, sizeof(uint32_t) * nodes));
+    GCHK(g, hipMalloc(&g->count, sizeof(uint32_t) * nodes));
+    GCHK(g, hipMalloc(&g->flag, sizeof(uint32_t) * nodes));
+    GCHK(g, hipMalloc(&g->geo, sizeof(double4) * nodes));
+    GCHK(g, hipMalloc(&g->lo, sizeof(double4) * nodes));
+    GCHK(g, hipMalloc(&g->hi, sizeof(double4) * nodes));
+    GCHK(g, hipMalloc(&g->mom, sizeof(NodeMom) * nodes));
+    GCHK(g, hipMalloc(&g->parent, sizeof(uint32_t) * ids));
+    GCHK(g, hipMalloc(&g->skip, sizeof(uint32_t) * ids));
+    GCHK(g, hipMalloc(&g->leaf_at, sizeof(uint32_t) * c));
+    GCHK(g, hipMalloc(&g->targets, sizeof(uint32_t) * c));
+    GCHK(g, hipMalloc(&g->ntargets_dev, sizeof(uint32_t)));
+    size_t b1 = 0, b2 = 0;
+    GCHK(g, hipcub::DeviceRadixSort::SortPairs(nullptr, b1, g->keys, g->keys_sorted, g->iota, g->order, int(c), 0, 63,
+                                               g->stream));
+    GCHK(g, hipcub::DeviceSelect::If(nullptr, b2, g->leaf_at, g->targets, g->ntargets_dev, int(c), NotEnd(), g->stream));
+    g->sort_temp_bytes = b1;
+    g->select_temp_bytes = b2;
+    GCHK(g, hipMalloc(&g->sort_temp, b1 > 0 ? b1 : 1));
+    GCHK(g, hipMalloc(&g->select_temp, b2 > 0 ? b2 : 1));
+    for (int k = 0; k < NGPLANES; ++k) g->s.d[k] = g->dblock + size_t(k) * c;
+    g->cap = c;
+    return OK;
+}
+
+int ensure_staging(osph_gravity* g, size_t bytes) {
+    if (bytes <= g->staging_cap) return OK;
+    (void)hipFree(g->staging);
+    g->staging = nullptr;
+    g->staging_cap = 0;
+    GCHK(g, hipMalloc(&g->staging, bytes));
+    g->staging_cap = bytes;
+    return OK;
+}
+
+int ensure_scratch(osph_gravity* g, size_t doubles) {
+    if (doubles <= g->scratch_cap) return OK;
+    (void)hipFree(g->scratch);
+    g->scratch = nullptr;
+    g->scratch_cap = 0;
+    GCHK(g, hipMalloc(&g->scratch, doubles * sizeof(double)));
+    g->scratch_cap = doubles;
+    return OK;
+}
+
+int ensure_lists(osph_gravity* g, size_t targets) {
+    const size_t need = targets;
+    if (need <= g->list_targets && g->nlist) return OK;
+    (void)hipFree(g->nlist);
+    (void)hipFree(g->xlist);
+    (void)hipFree(g->ncount);
+    (void)hipFree(g->xcount);
+    g->nlist = g->xlist = g->ncount = g->xcount = nullptr;
+    g->list_targets = 0;
+    const size_t t = need > 0 ? need : 1;
+    GCHK(g, hipMalloc(&g->nlist, sizeof(uint32_t) * t * g->cap_n));
+    GCHK(g, hipMalloc(&g->xlist, sizeof(uint32_t) * t * g->cap_x));
+    GCHK(g, hipMalloc(&g->ncount, sizeof(uint32_t) * t));
+    GCHK(g, hipMalloc(&g->xcount, sizeof(uint32_t) * t));
+    g->list_targets = t;
+    return OK;
+}
+
+Tree tree_of(osph_gravity* g) {
+    Tree t{};
+    t.n = uint32_t(g->n);
+    t.leaf = uint32_t(g->cfg.leaf_size);
+    t.left = g->left;
+    t.right = g->right;
+    t.parent = g->parent;
+    t.skip = g->skip;
+    t.first = g->first;
+    t.count = g->count;
+    t.geo = g->geo;
+    t.lo = g->lo;
+    t.hi = g->hi;
+    t.mom = g->mom;
+    t.flag = g->flag;
+    t.pos = g->pos;
+    t.gm = g->gm;
+    t.thetaInv = 1.0 / g->cfg.theta;
+    t.error = reinterpret_cast<uint32_t*>(g->small + 8);
+    return t;
+}
+
+double decode(unsigned long long e) { return decode_ord_host(e); }
+
+int build_tree(osph_gravity* g) {
+    const size_t n = g->n;
+    g->built = false;
+    if (n == 0) {
+        g->built = true;
+        g->ntargets = 0;
+        return OK;
+    }
+    hipStream_t st = g->stream;
+    timed(g, "bounds", [&] {
+        k_init_bounds<<<1, 64, 0, st>>>(g->small);
+        unsigned gb = grid_for(n);
+        gb = gb > 1024 ? 1024 : gb;
+        k_bounds3<<<gb, BLOCK, 0, st>>>(g->s, g->small, n);
+    });
+    GCHK(g, hipMemcpyAsync(g->small_host, g->small, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    if (int e = sync(g)) return e;
+    double lo[3], ext = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = decode(g->small_host[k]);
+        const double hi = decode(g->small_host[3 + k]);
+        if (!std::isfinite(lo[k]) || !std::isfinite(hi)) return g->fail(ERR_STATE, "non-finite positions");
+        ext = std::max(ext, hi - lo[k]);
+    }
+    const double scale = ext > 0.0 ? 2097151.0 / ext : 0.0;
+    Tree t = tree_of(g);
+    timed(g, "morton", [&] { k_morton<<<grid_for(n), BLOCK, 0, st>>>(g->s, lo[0], lo[1], lo[2], scale, g->keys, g->iota, n); });
+    timed(g, "sort", [&] {
+        (void)hipcub::DeviceRadixSort::SortPairs(g->sort_temp, g->sort_temp_bytes, g->keys, g->keys_sorted, g->iota,
+                                                 g->order, int(n), 0, 63, st);
+    });
+    timed(g, "gather", [&] { k_gather<<<grid_for(n), BLOCK, 0, st>>>(g->s, g->order, g->cfg.G, g->pos, g->gm, n); });
+    if (n > 1) {
+        timed(g, "radix_tree", [&] { k_radix_tree<<<grid_for(n - 1), BLOCK, 0, st>>>(t, g->keys_sorted); });
+        timed(g, "skip", [&] { k_skip<<<grid_for(2 * n - 1), BLOCK, 0, st>>>(t); });
+        timed(g, "moments", [&] { k_moments<<<grid_for(n), BLOCK, 0, st>>>(t); });
+    } else {
+        const uint32_t endv = END;
+        GCHK(g, hipMemcpyAsync(g->parent, &endv, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        GCHK(g, hipMemcpyAsync(g->skip, &endv, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
+    timed(g, "leaves", [&] {
+        k_fill_u32g<<<std::min<unsigned>(grid_for(n), 4096), BLOCK, 0, st>>>(g->leaf_at, n, END);
+        k_mark_leaves<<<grid_for(2 * n - 1), BLOCK, 0, st>>>(t, g->leaf_at);
+        (void)hipcub::DeviceSelect::If(g->select_temp, g->select_temp_bytes, g->leaf_at, g->targets, g->ntargets_dev,
+                                       int(n), NotEnd(), st);
+    });
+    uint32_t nt = 0;
+    GCHK(g, hipMemcpyAsync(g->small_host + 7, g->ntargets_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (int e = check_traversal(g)) return e;
+    std::memcpy(&nt, g->small_host + 7, sizeof(uint32_t));
+    g->ntargets = nt;
+    g->built = true;
+    return OK;
+}
+
+// Barnes-Hut interaction lists and sums, adding to the state's accelerations.
+int eval_bh(osph_gravity* g, osph_gravity_stats* stats) {
+    const size_t n = g->n;
+    if (!n) return OK;
+    hipStream_t st = g->stream;
+    Tree t = tree_of(g);
+    const uint32_t nt = g->ntargets;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (int e = ensure_lists(g, nt)) return e;
+        uint32_t* need = reinterpret_cast<uint32_t*>(g->small + 6);
+        timed(g, "walk", [&] {
+            k_fill_u32g<<<1, 64, 0, st>>>(need, 2, 0u);
+            k_walk<<<grid_for(nt), BLOCK, 0, st>>>(t, g->targets, nt, g->nlist, g->cap_n, g->xlist, g->cap_x, g->ncount,
+                                                   g->xcount, need);
+        });
+        GCHK(g, hipMemcpyAsync(g->small_host + 6, need, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        if (int e = check_traversal(g)) return e;
+        uint32_t nd[2];
+        std::memcpy(nd, g->small_host + 6, sizeof(nd));
+        if (nd[0] == 0 && nd[1] == 0) break;
+        if (attempt == 3) return g->fail(ERR_STATE, "interaction lists did not fit after growing");
+        // grow to the largest list and walk again
+        (void)hipFree(g->nlist);
+        g->nlist = nullptr;
+        if (nd[0]) g->cap_n = (nd[0] + 63u) & ~63u;
+        if (nd[1]) g->cap_x = (nd[1] + 15u) & ~15u;
+        g->list_targets = 0;
+    }
+    const int ord = g->cfg.order;
+    const double* grads = g->cfg.kernel == OSPH_GRAVITY_KERNEL_M4 ? g->lut_grads : nullptr;
+    const unsigned blocks = unsigned((nt + (BLOCK / 64) - 1) / (BLOCK / 64));
+    timed(g, "eval", [&] {
+        k_eval<<<blocks > 0 ? blocks : 1, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist,
+                                                         g->cap_x, g->ncount, g->xcount, grads, ord);
+    });
+    if (stats) {
+        std::vector<uint32_t> a(nt), b(nt);
+        if (nt) {
+            GCHK(g, hipMemcpyAsync(a.data(), g->ncount, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            GCHK(g, hipMemcpyAsync(b.data(), g->xcount, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        }
+        if (int e = sync(g)) return e;
+        stats->nodes_approx = stats->nodes_exact = 0;
+        for (uint32_t k = 0; k < nt; ++k) {
+            stats->nodes_approx += a[k];
+            stats->nodes_exact += b[k];
+        }
+        stats->node_count = 2 * uint64_t(n) - 1;
+    }
+    return OK;
+}
+
+// Direct sum for targets (device indices) into out (double4 per target), G applied.
+int eval_direct(osph_gravity* g, const uint32_t* tg_dev, size_t nt, double4* out) {
+    if (!nt) return OK;
+    const double* grads = g->cfg.kernel == OSPH_GRAVITY_KERNEL_M4 ? g->lut_grads : nullptr;
+    timed(g, "direct", [&] {
+        k_direct<<<grid_for(nt), BLOCK, 0, g->stream>>>(g->s, g->n, tg_dev, nt, grads, g->cfg.G, out);
+    });
+    return OK;
+}
+
+__global__ void k_add_direct(Planes s, const uint32_t* tg, const double4* a, size_t nt) {
+    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= nt) return;
+    const uint32_t i = tg[q];
+    s.d[GAX][i] += a[q].x;
+    s.d[GAY][i] += a[q].y;
+    s.d[GAZ][i] += a[q].z;
+}
+
+__global__ void k_iota(uint32_t* p, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = uint32_t(i);
+}
+
+// Self-gravity of every particle added to the state's accelerations.
+int eval_self(osph_gravity* g, osph_gravity_stats* stats) {
+    if (g->cfg.solver == OSPH_GRAVITY_BARNES_HUT) {
+        if (!g->built) return g->fail(ERR_STATE, "gravity not built");
+        return eval_bh(g, stats);
+    }
+    const size_t n = g->n;
+    if (!n) return OK;
+    if (int e = ensure_scratch(g, 4 * n + n / 2 + 1)) return e;
+    double4* out = reinterpret_cast<double4*>(g->scratch);
+    uint32_t* tg = reinterpret_cast<uint32_t*>(g->scratch + 4 * n);
+    k_iota<<<grid_for(n), BLOCK, 0, g->stream>>>(tg, n);
+    if (int e = eval_direct(g, tg, n, out)) return e;
+    k_add_direct<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, tg, out, n);
+    if (stats) *stats = osph_gravity_stats{ 0, 0, 0 };
+    return OK;
+}
+
+int upload_vectors(osph_gravity* g, const double* host, size_t stride, int plane0, int ncomp, size_t n) {
+    if (!n) return OK;
+    if (int e = ensure_staging(g, n * stride)) return e;
+    GCHK(g, hipMemcpyAsync(g->staging, host, n * stride, hipMemcpyHostToDevice, g->stream));
+    k_unstage<<<grid_for(n), BLOCK, 0, g->stream>>>(g->staging, stride, ncomp, g->s, plane0, n);
+    return OK;
+}
+
+int download_vectors(osph_gravity* g, double* host, size_t stride, int plane0, int ncomp, size_t n) {
+    if (!n) return OK;
+    if (int e = ensure_staging(g, n * stride)) return e;
+    if (stride > size_t(ncomp) * sizeof(double)) {
+        // keep the caller's other lanes: start from their current values
+        GCHK(g, hipMemcpyAsync(g->staging, host, n * stride, hipMemcpyHostToDevice, g->stream));
+    }
+    k_stage<<<grid_for(n), BLOCK, 0, g->stream>>>(g->staging, stride, ncomp, g->s, plane0, n);
+    GCHK(g, hipMemcpyAsync(host, g->staging, n * stride, hipMemcpyDeviceToHost, g->stream));
+    return sync(g);
+}
+
+}  // namespace
+
+extern "C" {
+
+int osph_gravity_create(osph_gravity** out, const osph_gravity_config* cfg) {
+    if (!out || !cfg) return ERR_ARG;
+    *out = nullptr;
+    if (cfg->solver != OSPH_GRAVITY_BRUTE_FORCE && cfg->solver != OSPH_GRAVITY_BARNES_HUT) return ERR_ARG;
+    if (cfg->kernel != OSPH_GRAVITY_KERNEL_POINT && cfg->kernel != OSPH_GRAVITY_KERNEL_M4) return ERR_ARG;
+    if (cfg->solver == OSPH_GRAVITY_BARNES_HUT) {
+        if (!(cfg->theta > 0.0) || !std::isfinite(cfg->theta)) return ERR_ARG;
+        if (cfg->order != 0 && cfg->order != 2 && cfg->order != 3) return ERR_ARG;
+        if (cfg->leaf_size < 1 || cfg->leaf_size > 64) return ERR_ARG;
+    }
+    osph_gravity* g = new osph_gravity();
+    g->cfg = *cfg;
+    auto bail = [&](int code) {
+        osph_gravity_destroy(g);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) return bail(ERR_HIP);
+    if (hipMalloc(&g->small, 16 * sizeof(unsigned long long)) != hipSuccess) return bail(ERR_HIP);
+    if (hipHostMalloc(&g->small_host, 16 * sizeof(unsigned long long)) != hipSuccess) return bail(ERR_HIP);
+    std::vector<double> grads(LUT_ENTRIES + 1), values(LUT_ENTRIES + 1);
+    const double qSqrToIdx = double(LUT_ENTRIES) * (1.0 / (2.0 * 2.0));
+    for (int i = 0; i <= LUT_ENTRIES; ++i) {
+        const double qSqr = double(i) / qSqrToIdx;
+        grads[i] = gk_grad(qSqr);
+        values[i] = gk_value(qSqr);
+    }
+    if (hipMalloc(&g->lut_grads, grads.size() * sizeof(double)) != hipSuccess) return bail(ERR_HIP);
+    if (hipMalloc(&g->lut_values, values.size() * sizeof(double)) != hipSuccess) return bail(ERR_HIP);
+    if (hipMemcpy(g->lut_grads, grads.data(), grads.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->lut_values, values.data(), values.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        return bail(ERR_HIP);
+    }
+    *out = g;
+    return OK;
+}
+
+void osph_gravity_destroy(osph_gravity* g) {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    resolve(g);
+    for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
+    free_particles(g);
+    void* ptrs[] = { g->nlist, g->xlist, g->ncount, g->xcount, g->small, g->lut_grads, g->lut_values, g->scratch,
+                     g->staging };
+    for (void* p : ptrs) (void)hipFree(p);
+    if (g->small_host) (void)hipHostFree(g->small_host);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+const char* osph_gravity_last_error(const osph_gravity* g) { return g ? g->err.c_str() : "null context"; }
+
+int osph_gravity_set_particles(osph_gravity* g, size_t n, const double* r, const double* v, size_t vec_stride,
+                               const double* m) {
+    if (!g || (n && (!r || !m))) return ERR_ARG;
+    if (n >= (size_t(1) << 31)) return g->fail(ERR_ARG, "particle count must be below 2^31");
+    const size_t stride = vec_stride ? vec_stride : 4 * sizeof(double);
+    if (stride < 4 * sizeof(double) || stride % sizeof(double)) return g->fail(ERR_ARG, "bad vector stride");
+    if (int e = ensure_particles(g, n > 0 ? n : 1)) return e;
+    g->n = n;
+    g->built = false;
+    if (!n) return sync(g);
+    if (int e = upload_vectors(g, r, stride, GX, 4, n)) return e;
+    if (v) {
+        if (int e = upload_vectors(g, v, stride, GVX, 4, n)) return e;
+        if (int e = sync(g)) return e;  // the staging buffer is reused below
+    } else {
+        GCHK(g, hipMemsetAsync(g->s.d[GVX], 0, 4 * sizeof(double) * g->cap, g->stream));
+    }
+    GCHK(g, hipMemsetAsync(g->s.d[GAX], 0, 3 * sizeof(double) * g->cap, g->stream));
+    GCHK(g, hipMemcpyAsync(g->s.d[GM], m, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    return sync(g);
+}
+
+int osph_gravity_get(osph_gravity* g, int what, double* out, size_t vec_stride) {
+    if (!g || !out || what < 0 || what > 2) return ERR_ARG;
+    const size_t stride = vec_stride ? vec_stride : 4 * sizeof(double);
+    if (stride < 4 * sizeof(double) || stride % sizeof(double)) return g->fail(ERR_ARG, "bad vector stride");
+    const int plane0 = what == 0 ? GX : (what == 1 ? GVX : GAX);
+    const int ncomp = what == 2 ? 3 : 4;
+    if (what == 2) {
+        for (size_t i = 0; i < g->n; ++i) out[i * (stride / sizeof(double)) + 3] = 0.0;
+    }
+    return download_vectors(g, out, stride, plane0, ncomp, g->n);
+}
+
+int osph_gravity_build(osph_gravity* g) {
+    if (!g) return ERR_ARG;
+    if (g->cfg.solver == OSPH_GRAVITY_BRUTE_FORCE) {
+        g->built = true;
+        return OK;
+    }
+    return build_tree(g);
+}
+
+int osph_gravity_eval_self(osph_gravity* g, double* dv, size_t vec_stride, osph_gravity_stats* stats) {
+    if (!g) return ERR_ARG;
+    const size_t stride = vec_stride ? vec_stride : 4 * sizeof(double);
+    if (dv) {
+        if (stride < 4 * sizeof(double) || stride % sizeof(double)) return g->fail(ERR_ARG, "bad vector stride");
+        if (int e = upload_vectors(g, dv, stride, GAX, 3, g->n)) return e;
+        if (int e = sync(g)) return e;
+    }
+    if (int e = eval_self(g, stats)) return e;
+    if (dv) return download_vectors(g, dv, stride, GAX, 3, g->n);
+    return sync(g);
+}
+
+int osph_gravity_eval_points(osph_gravity* g, const double* p, size_t np, double* a, size_t vec_stride) {
+    if (!g || (np && (!p || !a))) return ERR_ARG;
+    const size_t stride = vec_stride ? vec_stride : 4 * sizeof(double);
+    if (stride < 4 * sizeof(double) || stride % sizeof(double)) return g->fail(ERR_ARG, "bad vector stride");
+    if (!np) return OK;
+    if (g->cfg.solver != OSPH_GRAVITY_BARNES_HUT) {
+        return g->fail(ERR_STATE, "point evaluation is provided by the Barnes-Hut solver");
+    }
+    if (!g->built) return g->fail(ERR_STATE, "gravity not built");
+    if (int e = ensure_scratch(g, 8 * np)) return e;
+    double4* pts = reinterpret_cast<double4*>(g->scratch);
+    double4* acc = pts + np;
+    std::vector<double4> hp(np), ha(np);
+    for (size_t k = 0; k < np; ++k) {
+        const double* e = p + k * (stride / sizeof(double));
+        hp[k] = make_double4(e[0], e[1], e[2], e[3]);
+    }
+    GCHK(g, hipMemcpyAsync(pts, hp.data(), np * sizeof(double4), hipMemcpyHostToDevice, g->stream));
+    if (g->n == 0) {
+        for (size_t k = 0; k < np; ++k) {
+            double* e = a + k * (stride / sizeof(double));
+            e[0] = e[1] = e[2] = e[3] = 0.0;
+        }
+        return sync(g);
+    }
+    const double* grads = g->cfg.kernel == OSPH_GRAVITY_KERNEL_M4 ? g->lut_grads : nullptr;
+    Tree t = tree_of(g);
+    timed(g, "points", [&] { k_points<<<grid_for(np), BLOCK, 0, g->stream>>>(t, pts, acc, np, grads, g->cfg.order); });
+    GCHK(g, hipMemcpyAsync(ha.data(), acc, np * sizeof(double4), hipMemcpyDeviceToHost, g->stream));
+    if (int e = check_traversal(g)) return e;
+    for (size_t k = 0; k < np; ++k) {
+        double* e = a + k * (stride / sizeof(double));
+        e[0] = ha[k].x;
+        e[1] = ha[k].y;
+        e[2] = ha[k].z;
+        e[3] = 0.0;
+    }
+    return OK;
+}
+
+int osph_gravity_direct(osph_gravity* g, const uint32_t* t, size_t nt, double* a, size_t vec_stride) {
+    if (!g || (nt && (!t || !a))) return ERR_ARG;
+    const size_t stride = vec_stride ? vec_stride : 4 * sizeof(double);
+    if (stride < 4 * sizeof(double) || stride % sizeof(double)) return g->fail(ERR_ARG, "bad vector stride");
+    for (size_t k = 0; k < nt; ++k) {
+        if (t[k] >= g->n) return g->fail(ERR_ARG, "target index out of range");
+    }
+    if (!nt) return OK;
+    if (int e = ensure_scratch(g, 4 * nt + nt / 2 + 1)) return e;
+    double4* out = reinterpret_cast<double4*>(g->scratch);
+    uint32_t* tg = reinterpret_cast<uint32_t*>(g->scratch + 4 * nt);
+    GCHK(g, hipMemcpyAsync(tg, t, nt * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+    if (int e = eval_direct(g, tg, nt, out)) return e;
+    std::vector<double4> ha(nt);
+    GCHK(g, hipMemcpyAsync(ha.data(), out, nt * sizeof(double4), hipMemcpyDeviceToHost, g->stream));
+    if (int e = sync(g)) return e;
+    for (size_t k = 0; k < nt; ++k) {
+        double* e = a + k * (stride / sizeof(double));
+        e[0] = ha[k].x;
+        e[1] = ha[k].y;
+        e[2] = ha[k].z;
+        e[3] = 0.0;
+    }
+    return OK;
+}
+
+int osph_gravity_energy(osph_gravity* g, double* energy) {
+    if (!g || !energy) return ERR_ARG;
+    const size_t n = g->n;
+    *energy = 0.0;
+    if (!n) return OK;
+    const unsigned blocks = std::min<unsigned>(grid_for(n), 1024);
+    if (int e = ensure_scratch(g, n + blocks + 1)) return e;
+    const double* values = g->cfg.kernel == OSPH_GRAVITY_KERNEL_M4 ? g->lut_values : nullptr;
+    timed(g, "energy", [&] {
+        k_energy<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, n, values, g->scratch);
+        k_sum_vector<<<blocks, BLOCK, 0, g->stream>>>(g->scratch, n, g->scratch + n);
+        k_sum_partials<<<1, 64, 0, g->stream>>>(g->scratch + n, int(blocks), 1, g->scratch + n + blocks);
+    });
+    double sum = 0.0;
+    GCHK(g, hipMemcpyAsync(&sum, g->scratch + n + blocks, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    if (int e = sync(g)) return e;
+    *energy = 0.5 * g->cfg.G * sum;
+    return OK;
+}
+
+int osph_gravity_moments(osph_gravity* g, double out[16]) {
+    if (!g || !out) return ERR_ARG;
+    if (g->cfg.solver != OSPH_GRAVITY_BARNES_HUT || !g->built) return g->fail(ERR_STATE, "no Barnes-Hut tree built");
+    std::memset(out, 0, 16 * sizeof(double));
+    if (g->n == 0) return OK;
+    if (g->n == 1) {
+        double4 p;
+        double m;
+        GCHK(g, hipMemcpy(&p, g->pos, sizeof(p), hipMemcpyDeviceToHost));
+        GCHK(g, hipMemcpy(&m, g->gm, sizeof(m), hipMemcpyDeviceToHost));
+        out[0] = m / g->cfg.G;
+        out[1] = p.x;
+        out[2] = p.y;
+        out[3] = p.z;
+        return OK;
+    }
+    double4 geo;
+    NodeMom mom;
+    GCHK(g, hipMemcpy(&geo, g->geo, sizeof(geo), hipMemcpyDeviceToHost));
+    GCHK(g, hipMemcpy(&mom, g->mom, sizeof(mom), hipMemcpyDeviceToHost));
+    const double inv = 1.0 / g->cfg.G;  // BarnesHut::getMoments divides the premultiplied moments by G
+    out[0] = mom.M * inv;
+    out[1] = geo.x;
+    out[2] = geo.y;
+    out[3] = geo.z;
+    for (int k = 0; k < 5; ++k) out[4 + k] = mom.q2[k] * inv;
+    for (int k = 0; k < 7; ++k) out[9 + k] = mom.q3[k] * inv;
+    return OK;
+}
+
+int osph_gravity_step_leapfrog(osph_gravity* g, double dt, int steps) {
+    if (!g) return ERR_ARG;
+    if (!(dt > 0.0) || !std::isfinite(dt) || steps < 0) return g->fail(ERR_ARG, "bad time step or step count");
+    const size_t n = g->n;
+    if (!n) return OK;
+    for (int s = 0; s < steps; ++s) {
+        timed(g, "drift", [&] { k_drift<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, dt, n); });
+        k_zero_acc<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, n);
+        if (g->cfg.solver == OSPH_GRAVITY_BARNES_HUT) {
+            if (int e = build_tree(g)) return e;
+        }
+        if (int e = eval_self(g, nullptr)) return e;
+        timed(g, "kick_drift", [&] { k_kick_drift<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, dt, n); });
+    }
+    return sync(g);
+}
+
+int osph_gravity_integrals(osph_gravity* g, double out[11]) {
+    if (!g || !out) return ERR_ARG;
+    std::memset(out, 0, 11 * sizeof(double));
+    const size_t n = g->n;
+    if (!n) return OK;
+    const unsigned blocks = std::min<unsigned>(grid_for(n), 1024);
+    if (int e = ensure_scratch(g, size_t(blocks) * 11 + 11)) return e;
+    k_integrals<<<blocks, BLOCK, 0, g->stream>>>(g->s, n, g->scratch);
+    k_sum_partials<<<1, 64, 0, g->stream>>>(g->scratch, int(blocks), 11, g->scratch + size_t(blocks) * 11);
+    GCHK(g, hipMemcpyAsync(out, g->scratch + size_t(blocks) * 11, 11 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    if (int e = sync(g)) return e;
+    // CenterOfMass::evaluate (Integrals.cpp:155-180): sum m r / sum m
+    for (int k = 7; k < 10; ++k) out[k] /= out[10];
+    return OK;
+}
+
+int osph_gravity_sync(osph_gravity* g) {
+    if (!g) return ERR_ARG;
+    return sync(g);
+}
+
+int osph_gravity_profile_enable(osph_gravity* g, int enable) {
+    if (!g) return ERR_ARG;
+    GCHK(g, hipStreamSynchronize(g->stream));
+    resolve(g);
+    g->prof = enable != 0;
+    g->acc.clear();
+    return OK;
+}
+
+int osph_gravity_profile_read(osph_gravity* g, char* names, size_t name_cap, double* ms, uint64_t* launches, int cap,
+                              int* count) {
+    if (!g || !count) return ERR_ARG;
+    GCHK(g, hipStreamSynchronize(g->stream));
+    resolve(g);
+    int k = 0;
+    for (const auto& kv : g->acc) {
+        if (k < cap) {
+            if (names && name_cap) {
+                std::strncpy(names + size_t(k) * name_cap, kv.first.c_str(), name_cap - 1);
+                names[size_t(k) * name_cap + name_cap - 1] = '\0';
+            }
+            if (ms) ms[k] = kv.second.first;
+            if (launches) launches[k] = kv.second.second;
+        }
+        ++k;
+    }
+    *count = k;
+    return OK;
+}
+
+}  // extern "C"

@@ -5,6 +5,7 @@
 // per-neighbour data into 128-byte records in that order, builds neighbour lists with the exact FP64
 // pair test of SymmetricSolver<3>::loop, and evaluates all pair sums in gather form: each particle
 // sums over its own neighbours, so every sum is written by one thread, in a fixed order, without atomics.
+#include "osph_device.h"
 #include "osph_internal.h"
 
 #include <hipcub/hipcub.hpp>
@@ -13,30 +14,6 @@ namespace osph {
 
 namespace {
 
-constexpr int BLOCK = 256;
-
-inline unsigned grid_for(size_t n, int block = BLOCK) {
-    size_t g = (n + block - 1) / block;
-    return unsigned(g > 0 ? g : 1);
-}
-
-// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Renumber them so that XCD x
-// processes one contiguous range of logical blocks: consecutive sorted particles, and therefore their
-// shared neighbours, stay in one L2.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
-    constexpr uint32_t NXCD = 8;
-    const uint32_t q = nblocks / NXCD, r = nblocks % NXCD;
-    const uint32_t x = bid % NXCD, local = bid / NXCD;
-    return x * q + (x < r ? x : r) + local;
-}
-
-__device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }  // MathBasic.h:15-17
-__device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }  // MathBasic.h:20-22
-
-// EPS and LARGE are float literals widened to double (core/math/MathUtils.h:26-34).
-__device__ constexpr double EPS_D = double(1.e-12f);
-__device__ constexpr double LARGE_D = double(1.e20f);
-__device__ constexpr double PI_D = 3.14159265358979323846264338327950288;
 
 // ---- material physics (per particle; compiled without FMA contraction so that branch decisions
 // ---- follow the reference's rounding) -------------------------------------------------------------
@@ -163,28 +140,6 @@ __device__ void material_init_at(DevState& s, const osph_material& m, const Para
 }
 
 // ---- grid ------------------------------------------------------------------------------------------
-
-// Order-preserving map of doubles onto unsigned 64-bit integers (for atomic min/max).
-__device__ __forceinline__ unsigned long long ord_bits(double v) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-__device__ __forceinline__ unsigned long long wave_min(unsigned long long v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
 
 // Block reduction of NV order-encoded values; is_min[k] selects min or max for value k.
 template <int NV>

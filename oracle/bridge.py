@@ -208,6 +208,10 @@ class RefRun:
             L.ref_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t]
             L.ref_pairs.restype = C.c_size_t
             L.ref_pairs.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_size_t]
+            dp, i = C.POINTER(C.c_double), C.c_int
+            L.ref_gravity.argtypes = [i, i, C.c_double, i, i, C.c_double, C.c_size_t, dp, dp, dp, dp]
+            L.ref_gravity_points.argtypes = [i, i, C.c_double, i, i, C.c_double, C.c_size_t, dp, dp, dp, C.c_size_t,
+                                             dp]
             cls._lib = L
         return cls._lib
 
@@ -253,3 +257,102 @@ class RefRun:
             if cnt <= cap:
                 return pi[:cnt], pj[:cnt]
             cap = cnt
+
+
+class GravityOracle:
+    """The C restatement of the reference's self-gravity (osph_oracle_gravity.c)."""
+
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run make -C oracle)")
+        L = self.lib = C.CDLL(path)
+        dp, sz = C.POINTER(C.c_double), C.c_size_t
+        L.orc_gravity_lut.argtypes = [dp, dp]
+        L.orc_gravity_direct.argtypes = [sz] + [dp] * 6 + [C.c_double, C.POINTER(C.c_uint32), sz, dp]
+        L.orc_gravity_points.argtypes = [sz] + [dp] * 6 + [C.c_double, dp, sz, dp]
+        L.orc_gravity_energy.restype = C.c_double
+        L.orc_gravity_energy.argtypes = [sz] + [dp] * 6 + [C.c_double]
+        L.orc_gravity_moments.argtypes = [sz] + [dp] * 4 + [dp]
+        L.orc_multipole_accel.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_int, dp]
+        L.orc_leapfrog.argtypes = [sz, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int]
+        self.values = np.zeros(40001)
+        self.grads = np.zeros(40001)
+        L.orc_gravity_lut(_ptr(self.values), _ptr(self.grads))
+
+    def _planes(self, r):
+        r = np.asarray(r, dtype=np.float64)
+        return [np.ascontiguousarray(r[:, k]) for k in range(4)]
+
+    def direct(self, r, m, G, kernel="m4", targets=None):
+        x, y, z, h = self._planes(r)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        n = len(m)
+        t = None if targets is None else np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = n if t is None else t.size
+        a = np.zeros(3 * nt)
+        g = _ptr(self.grads) if kernel == "m4" else None
+        self.lib.orc_gravity_direct(n, _ptr(x), _ptr(y), _ptr(z), _ptr(h), _ptr(m), g, G,
+                                    None if t is None else _ptr(t, C.c_uint32), nt, _ptr(a))
+        return a.reshape(3, nt).T.copy()
+
+    def points(self, r, m, G, p, kernel="m4"):
+        x, y, z, h = self._planes(r)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        p = np.asarray(p, dtype=np.float64)
+        pp = np.ascontiguousarray(p[:, :3].T).ravel()
+        a = np.zeros(3 * p.shape[0])
+        g = _ptr(self.grads) if kernel == "m4" else None
+        self.lib.orc_gravity_points(len(m), _ptr(x), _ptr(y), _ptr(z), _ptr(h), _ptr(m), g, G, _ptr(pp), p.shape[0],
+                                    _ptr(a))
+        return a.reshape(3, -1).T.copy()
+
+    def energy(self, r, m, G, kernel="m4"):
+        x, y, z, h = self._planes(r)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        v = _ptr(self.values) if kernel == "m4" else None
+        return self.lib.orc_gravity_energy(len(m), _ptr(x), _ptr(y), _ptr(z), _ptr(h), _ptr(m), v, G)
+
+    def moments(self, r, m):
+        x, y, z, _ = self._planes(r)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        out = np.zeros(16)
+        self.lib.orc_gravity_moments(len(m), _ptr(x), _ptr(y), _ptr(z), _ptr(m), _ptr(out))
+        return out
+
+    def multipole(self, mom13, dr, order):
+        mom = np.ascontiguousarray(mom13, dtype=np.float64)
+        out = np.zeros(3)
+        self.lib.orc_multipole_accel(_ptr(mom), float(dr[0]), float(dr[1]), float(dr[2]), int(order), _ptr(out))
+        return out
+
+    def leapfrog(self, r, v, m, G, dt, steps, kernel="m4"):
+        r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).T).copy()
+        v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).T).copy()
+        n = r.shape[1]
+        a = np.zeros(3 * n)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        g = _ptr(self.grads) if kernel == "m4" else None
+        self.lib.orc_leapfrog(n, _ptr(r), _ptr(v), _ptr(a), _ptr(m), g, G, dt, steps)
+        return r.T.copy(), v.T.copy(), a.reshape(3, n).T.copy()
+
+
+def ref_gravity(r, m, G, solver="barnes_hut", kernel="m4", theta=0.5, order=2, leaf=25, points=None):
+    """The reference's IGravity (BruteForceGravity or BarnesHut) on (r, m): returns the self-gravity
+    accelerations (n x 3) and the root moments { M, q2[5], q3[7] } (Barnes-Hut), or the accelerations at
+    `points` when given."""
+    L = RefRun.lib()
+    r4 = np.zeros((len(m), 4))
+    r = np.asarray(r, dtype=np.float64)
+    r4[:, :r.shape[1]] = r
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    sol = {"brute_force": 0, "barnes_hut": 1}[solver]
+    ker = {"point": 0, "m4": 1}[kernel]
+    if points is not None:
+        p = np.ascontiguousarray(np.asarray(points, dtype=np.float64)[:, :3])
+        a = np.zeros((p.shape[0], 3))
+        L.ref_gravity_points(sol, ker, theta, order, leaf, G, len(m), _ptr(r4), _ptr(m), _ptr(p), p.shape[0], _ptr(a))
+        return a
+    a = np.zeros((len(m), 3))
+    mom = np.zeros(13)
+    L.ref_gravity(sol, ker, theta, order, leaf, G, len(m), _ptr(r4), _ptr(m), _ptr(a), _ptr(mom))
+    return a, mom

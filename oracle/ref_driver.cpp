@@ -2,6 +2,9 @@
 // ref_driver.h.  Compiled against the reference headers where they lie (oracle/Makefile, -I<ref>/core).
 #include "ref_driver.h"
 
+#include "gravity/BarnesHut.h"
+#include "gravity/BruteForceGravity.h"
+#include "gravity/Moments.h"
 #include "math/rng/Rng.h"
 #include "objects/finders/KdTree.h"
 #include "quantities/IMaterial.h"
@@ -334,3 +337,74 @@ size_t ref_pairs(void* h, uint32_t* pi, uint32_t* pj, size_t cap) {
 }
 
 } // extern "C"
+
+namespace {
+
+AutoPtr<IGravity> make_gravity(int solver, int kernel, double theta, int order, int leaf, double G) {
+    GravityLutKernel k;
+    if (kernel == 1) k = GravityKernel<CubicSpline<3>>();
+    if (solver == 0) return makeAuto<BruteForceGravity>(std::move(k), G);
+    return makeAuto<BarnesHut>(theta, MultipoleOrder(order), std::move(k), Size(leaf), 50, G);
+}
+
+Storage gravity_storage(size_t n, const double* r, const double* m) {
+    Array<Vector> pos(n);
+    Array<Float> mass(n);
+    for (size_t i = 0; i < n; ++i) {
+        pos[i] = Vector(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+        mass[i] = m[i];
+    }
+    Storage st;
+    st.insert<Vector>(QuantityId::POSITION, OrderEnum::SECOND, std::move(pos));
+    st.insert<Float>(QuantityId::MASS, OrderEnum::ZERO, std::move(mass));
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ref_gravity(int solver, int kernel, double theta, int order, int leaf, double G, size_t n, const double* r,
+                 const double* m, double* a, double* moments) {
+    Storage st = gravity_storage(n, r, m);
+    AutoPtr<IGravity> g = make_gravity(solver, kernel, theta, order, leaf, G);
+    SequentialScheduler& seq = *SequentialScheduler::getGlobalInstance();
+    g->build(seq, st);
+    ArrayView<Vector> dv = st.getD2t<Vector>(QuantityId::POSITION);
+    Statistics stats;
+    g->evalSelfGravity(seq, dv, stats);
+    for (size_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) a[3 * i + k] = dv[i][k];
+    }
+    if (moments && solver == 1) {
+        const MultipoleExpansion<3> ms = static_cast<BarnesHut&>(*g).getMoments();
+        moments[0] = ms.order<0>().value();
+        const TracelessMultipole<2>& q2 = ms.order<2>();
+        const TracelessMultipole<3>& q3 = ms.order<3>();
+        moments[1] = q2.value<0, 0>();
+        moments[2] = q2.value<0, 1>();
+        moments[3] = q2.value<0, 2>();
+        moments[4] = q2.value<1, 1>();
+        moments[5] = q2.value<1, 2>();
+        moments[6] = q3.value<0, 0, 0>();
+        moments[7] = q3.value<0, 0, 1>();
+        moments[8] = q3.value<0, 0, 2>();
+        moments[9] = q3.value<0, 1, 1>();
+        moments[10] = q3.value<0, 1, 2>();
+        moments[11] = q3.value<1, 1, 1>();
+        moments[12] = q3.value<1, 1, 2>();
+    }
+}
+
+void ref_gravity_points(int solver, int kernel, double theta, int order, int leaf, double G, size_t n,
+                        const double* r, const double* m, const double* p, size_t np, double* a) {
+    Storage st = gravity_storage(n, r, m);
+    AutoPtr<IGravity> g = make_gravity(solver, kernel, theta, order, leaf, G);
+    g->build(*SequentialScheduler::getGlobalInstance(), st);
+    for (size_t q = 0; q < np; ++q) {
+        const Vector v = g->evalAcceleration(Vector(p[3 * q], p[3 * q + 1], p[3 * q + 2], 0.0));
+        for (int k = 0; k < 3; ++k) a[3 * q + k] = v[k];
+    }
+}
+
+}  // extern "C"

@@ -38,6 +38,18 @@ void ref_read(void* h, double* planes, uint32_t* iplanes, size_t n);
  * returned as (min, max) sorted.  Returns the count (writes at most cap). */
 size_t ref_pairs(void* h, uint32_t* pi, uint32_t* pj, size_t cap);
 
+/* The reference's self-gravity on n particles (r: n Vectors x, y, z, h; m: masses).  solver 0 =
+ * BruteForceGravity, 1 = BarnesHut(theta, order, kernel, leaf); kernel 0 = point particles, 1 =
+ * GravityKernel<CubicSpline<3>>.  a (n x 3) receives evalSelfGravity from zero accelerations; when
+ * moments != NULL (Barnes-Hut) it receives getMoments() as { M, q2 (xx xy xz yy yz), q3 (xxx xxy xxz xyy
+ * xyz yyy yyz) }.  Sequential scheduler. */
+void ref_gravity(int solver, int kernel, double theta, int order, int leaf, double G, size_t n, const double* r,
+                 const double* m, double* a, double* moments);
+
+/* IGravity::evalAcceleration of the reference's solver at np points p (n x 3 out). */
+void ref_gravity_points(int solver, int kernel, double theta, int order, int leaf, double G, size_t n,
+                        const double* r, const double* m, const double* p, size_t np, double* a);
+
 #ifdef __cplusplus
 }
 #endif

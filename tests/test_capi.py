@@ -1,5 +1,5 @@
-"""C ABI checks that need no GPU: the library loads, exports every entry point include/osph_gpu.h
-declares, the ctypes structures match the C layouts, and the host-built M4 table equals the oracle's."""
+"""C ABI checks that need no GPU: the library loads, exports every entry point include/*.h declares,
+the ctypes structures match the C layouts, and the host-built M4 table equals the oracle's."""
 import ctypes as C
 import os
 import re
@@ -8,24 +8,52 @@ import tempfile
 
 import numpy as np
 
-from opensph_amd import gpu
+from opensph_amd import gpu, gravity
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "osph_gpu.h")
+HEADERS = sorted(os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))
+                 if f.endswith(".h"))
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(osph_\w+)\s*\(", text, re.M)))
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        names |= set(re.findall(r"^\s*(?:int|void|const char\*)\s+(osph_\w+)\s*\(", text, re.M))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
     lib = gpu.load_library()
     names = declared_functions()
-    assert len(names) >= 15
+    assert len(names) >= 30
     for name in names:
         assert hasattr(lib, name), name
-    assert sorted(gpu.EXPORTS) == names
+    assert sorted(gpu.EXPORTS + gravity.EXPORTS) == names
+
+
+def test_gravity_struct_layouts_match_header():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "osph_gravity.h"
+int main(void) {
+    printf("%zu %zu %zu %zu\n", sizeof(osph_gravity_config), sizeof(osph_gravity_stats),
+           offsetof(osph_gravity_config, leaf_size), offsetof(osph_gravity_config, G));
+    return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    got = [int(x) for x in out]
+    want = [C.sizeof(gravity.GravityConfig), C.sizeof(gravity.GravityStats), gravity.GravityConfig.leaf_size.offset,
+            gravity.GravityConfig.G.offset]
+    assert got == want
 
 
 def test_struct_layouts_match_header():
