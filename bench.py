@@ -118,6 +118,52 @@ def compare_precisions(gpu, st, cfg, runs):
             "energy_mixed_vs_fp64": (emx - e64) / e0}
 
 
+def nbody_state(n, seed=7):
+    """Uniform sphere of R = 100 km, equal masses 1.25e12 kg, h = 10 m (M4 softening support 20 m), at rest."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((0, 3))
+    while len(out) < n:
+        x = rng.uniform(-1.0, 1.0, size=(int(1.3 * (n - len(out))) + 1024, 3))
+        out = np.concatenate([out, x[np.einsum("ij,ij->i", x, x) < 1.0]])
+    r = np.c_[out[:n] * 1.0e5, np.full(n, 10.0)]
+    return r, np.full(n, 1.25e12)
+
+
+def run_nbody(args):
+    """Self-gravity N-body: Barnes-Hut (theta 0.5, quadrupole, leaf 25, M4-softened) + LeapFrog, dt = 1 s."""
+    from opensph_amd import gpu
+    from opensph_amd.gravity import GpuGravity
+
+    n = args.nbody_n
+    r, m = nbody_state(n)
+    g = GpuGravity(r, m, solver="barnes_hut", kernel="m4", theta=0.5, order=2, leaf_size=25)
+    g.leapfrog(1.0, args.warmup)
+    g.sync()
+    g.profile(True)
+    t0 = time.perf_counter()
+    g.leapfrog(1.0, args.steps)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    prof = g.profile_read()
+    g.profile(False)
+    # accuracy of the last evaluation against the direct sum on sampled targets
+    sample = np.random.default_rng(3).choice(n, 256, replace=False).astype(np.uint32)
+    bh = g.get("dv")[sample]
+    direct = g.direct(sample)
+    rel = np.linalg.norm(bh - direct, axis=1) / np.linalg.norm(direct, axis=1)
+    name, cus = gpu.device_info()
+    out = {"metric": "particle-steps/s, 8M N-body Barnes-Hut + LeapFrog", "value": n * args.steps / elapsed,
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "uniform sphere R 100 km, equal masses, M4 softening h 10 m, theta 0.5, "
+                                  "quadrupole, leaf 25, LeapFrog dt 1 s", "particles": n, "device": name},
+           "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in sorted(prof.items())},
+           "accuracy": {"sample": int(sample.size), "rel_err_median": float(np.median(rel)),
+                        "rel_err_max": float(rel.max())}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +176,9 @@ def main():
                     help="launch-shape setting key=value passed to osph_tune (e.g. pairs_lds=40960)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the 1 GiB device copy (HBM roof, FETCH_SIZE/WRITE_SIZE calibration)")
+    ap.add_argument("--workload", default="impact", choices=["impact", "nbody"],
+                    help="impact: the SPH headline; nbody: self-gravity Barnes-Hut + LeapFrog")
+    ap.add_argument("--nbody-n", type=int, default=8_000_000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X; "
@@ -137,6 +186,10 @@ def main():
         sys.exit(2)
 
     from opensph_amd import gpu, initial
+
+    if args.workload == "nbody":
+        run_nbody(args)
+        return
 
     if args.calibrate:
         nbytes, iters = 1 << 30, 10

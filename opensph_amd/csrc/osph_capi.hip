@@ -591,7 +591,11 @@ int osph_device_info(char* name, size_t cap, int* compute_units) {
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, dev) != hipSuccess) return OSPH_ERR_HIP;
     if (name && cap) {
-        std::snprintf(name, cap, "%s (%s)", p.name, p.gcnArchName);
+        if (p.name[0]) {
+            std::snprintf(name, cap, "%s (%s)", p.name, p.gcnArchName);
+        } else {
+            std::snprintf(name, cap, "%s", p.gcnArchName);  // some runtimes report no marketing name
+        }
     }
     if (compute_units) *compute_units = p.multiProcessorCount;
     return OSPH_OK;

@@ -52,6 +52,14 @@ struct NodeMom {
 };
 static_assert(sizeof(NodeMom) == 128, "NodeMom is 128 bytes");
 
+// What the walk reads of an internal node, in one 64-byte record.
+struct WalkNode {
+    double4 geo;  // centre of mass, r_open
+    uint32_t first, count, left, skip;
+    uint32_t pad[4];
+};
+static_assert(sizeof(WalkNode) == 64, "WalkNode is 64 bytes");
+
 struct Tree {
     uint32_t n;        // particles; internal nodes 0..n-2, particle k is node n-1+k
     uint32_t leaf;     // leaf size
@@ -65,7 +73,8 @@ struct Tree {
     double4* lo;       // [n-1]: box
     double4* hi;
     NodeMom* mom;      // [n-1]
-    uint32_t* flag;    // [n-1]: arrivals of the bottom-up pass
+    WalkNode* wn;      // [n-1]
+    uint32_t* flag;    // [n-1]: node depths while the moments are built
     const double4* pos;  // sorted x, y, z, h
     const double* gm;    // sorted G m
     double thetaInv;
@@ -179,10 +188,12 @@ __global__ void __launch_bounds__(BLOCK) k_radix_tree(Tree t, const uint64_t* __
     t.right[i] = rc;
     t.first[i] = uint32_t(lo);
     t.count[i] = uint32_t(hi - lo + 1);
+    t.wn[i].first = uint32_t(lo);
+    t.wn[i].count = uint32_t(hi - lo + 1);
+    t.wn[i].left = lc;
     t.parent[lc] = uint32_t(i);
     t.parent[rc] = uint32_t(i);
     if (i == 0) t.parent[0] = END;
-    t.flag[i] = 0u;
 }
 
 // Depth-first successor of each node: the right sibling of the nearest ancestor-or-self that is a left
@@ -201,6 +212,7 @@ __global__ void __launch_bounds__(BLOCK) k_skip(Tree t) {
         c = p;
     }
     t.skip[id] = next;
+    if (id + 1 < t.n) t.wn[id].skip = next;
 }
 
 struct Moments {
@@ -265,53 +277,94 @@ __device__ void add_shifted(const Moments& m, const double d[3], double q2[5], d
     }
 }
 
-// Bottom-up moments: one thread per particle climbs while it is the second child to arrive.
-__global__ void __launch_bounds__(BLOCK) k_moments(Tree t) {
-    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= t.n) return;
-    uint32_t node = t.parent[t.n - 1 + k];
-    for (int level = 0; node != END; ++level) {
-        if (level >= MAX_DEPTH) {
-            atomicOr(t.error, 1u);
-            return;
-        }
-        __threadfence();
-        if (atomicAdd(&t.flag[node], 1u) == 0u) return;
-        __threadfence();
-        Moments a, b;
-        load_node(t, t.left[node], a);
-        load_node(t, t.right[node], b);
-        const double M = a.M + b.M;
-        double c[3], lo[3], hi[3];
-        for (int q = 0; q < 3; ++q) {
-            lo[q] = dmin(a.lo[q], b.lo[q]);
-            hi[q] = dmax(a.hi[q], b.hi[q]);
-        }
-        NodeMom out{};
-        double r_open = 0.0;
-        if (M == 0.0) {
-            c[0] = c[1] = c[2] = 0.0;
-        } else {
-            // BarnesHut::buildInner (BarnesHut.cpp:447-468)
-            for (int q = 0; q < 3; ++q) c[q] = (a.M * a.c[q] + b.M * b.c[q]) / M;
-            double rm[3];
-            for (int q = 0; q < 3; ++q) rm[q] = dmax(c[q] - lo[q], hi[q] - c[q]);
-            r_open = 2.0 / sqrt(3.0) * t.thetaInv * sqrt(rm[0] * rm[0] + rm[1] * rm[1] + rm[2] * rm[2]);
-            out.M = M;
-            double da[3], db[3];
-            for (int q = 0; q < 3; ++q) {
-                da[q] = a.c[q] - c[q];
-                db[q] = b.c[q] - c[q];
-            }
-            add_shifted(a, da, out.q2, out.q3);
-            add_shifted(b, db, out.q2, out.q3);
-        }
-        t.geo[node] = make_double4(c[0], c[1], c[2], r_open);
-        t.lo[node] = make_double4(lo[0], lo[1], lo[2], 0.0);
-        t.hi[node] = make_double4(hi[0], hi[1], hi[2], 0.0);
-        t.mom[node] = out;
-        node = t.parent[node];
+// Node `node` from its two children: box, centre of mass and opening radius (BarnesHut::buildInner,
+// BarnesHut.cpp:433-468), moments by the parallel-axis theorem.
+__device__ void combine(const Tree& t, uint32_t node) {
+    Moments a, b;
+    load_node(t, t.left[node], a);
+    load_node(t, t.right[node], b);
+    const double M = a.M + b.M;
+    double c[3], lo[3], hi[3];
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = dmin(a.lo[q], b.lo[q]);
+        hi[q] = dmax(a.hi[q], b.hi[q]);
     }
+    NodeMom out{};
+    double r_open = 0.0;
+    if (M == 0.0) {
+        c[0] = c[1] = c[2] = 0.0;
+    } else {
+        for (int q = 0; q < 3; ++q) c[q] = (a.M * a.c[q] + b.M * b.c[q]) / M;
+        double rm[3];
+        for (int q = 0; q < 3; ++q) rm[q] = dmax(c[q] - lo[q], hi[q] - c[q]);
+        r_open = 2.0 / sqrt(3.0) * t.thetaInv * sqrt(rm[0] * rm[0] + rm[1] * rm[1] + rm[2] * rm[2]);
+        out.M = M;
+        double da[3], db[3];
+        for (int q = 0; q < 3; ++q) {
+            da[q] = a.c[q] - c[q];
+            db[q] = b.c[q] - c[q];
+        }
+        add_shifted(a, da, out.q2, out.q3);
+        add_shifted(b, db, out.q2, out.q3);
+    }
+    t.geo[node] = make_double4(c[0], c[1], c[2], r_open);
+    t.wn[node].geo = make_double4(c[0], c[1], c[2], r_open);
+    t.lo[node] = make_double4(lo[0], lo[1], lo[2], 0.0);
+    t.hi[node] = make_double4(hi[0], hi[1], hi[2], 0.0);
+    t.mom[node] = out;
+}
+
+// Depth of every internal node (the number of its ancestors) and the node count per depth.  The
+// moments are then built one depth per launch, deepest first: the children of a node are complete
+// when the launch of its depth starts, with no fences between work-groups.
+__global__ void __launch_bounds__(BLOCK) k_depth(Tree t, uint32_t* __restrict__ depth, uint32_t* __restrict__ per_level) {
+    __shared__ uint32_t hist[MAX_DEPTH + 1];
+    for (int d = threadIdx.x; d <= MAX_DEPTH; d += blockDim.x) hist[d] = 0u;
+    __syncthreads();
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i + 1 < t.n) {
+        uint32_t c = uint32_t(i), d = 0;
+        while (c != 0u && d < uint32_t(MAX_DEPTH)) {
+            c = t.parent[c];
+            ++d;
+        }
+        if (c != 0u) {
+            atomicOr(t.error, 1u);
+            d = 0;
+        }
+        depth[i] = d;
+        atomicAdd(&hist[d], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d <= MAX_DEPTH; d += blockDim.x) {
+        if (hist[d]) atomicAdd(&per_level[d], hist[d]);
+    }
+}
+
+// Scatter of the internal nodes into per-depth buckets: positions within the block from LDS counters,
+// one global reservation per depth and block.
+__global__ void __launch_bounds__(BLOCK) k_bucket(uint32_t n_internal, const uint32_t* __restrict__ depth,
+                                                  uint32_t* __restrict__ cursor, uint32_t* __restrict__ nodes) {
+    __shared__ uint32_t cnt[MAX_DEPTH + 1];
+    __shared__ uint32_t base[MAX_DEPTH + 1];
+    for (int d = threadIdx.x; d <= MAX_DEPTH; d += blockDim.x) cnt[d] = 0u;
+    __syncthreads();
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool active = i < n_internal;
+    const uint32_t d = active ? depth[i] : 0u;
+    const uint32_t local = active ? atomicAdd(&cnt[d], 1u) : 0u;
+    __syncthreads();
+    for (int k = threadIdx.x; k <= MAX_DEPTH; k += blockDim.x) {
+        base[k] = cnt[k] ? atomicAdd(&cursor[k], cnt[k]) : 0u;
+    }
+    __syncthreads();
+    if (active) nodes[base[d] + local] = uint32_t(i);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_moments_level(Tree t, const uint32_t* __restrict__ nodes, uint32_t count) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    combine(t, nodes[k]);
 }
 
 // Leaves (targets): nodes of at most leaf_size particles whose parent is larger, and particles whose
@@ -407,31 +460,31 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
             id = t.skip[id];
             continue;
         }
-        const uint32_t f = t.first[id], c = t.count[id];
-        if (f <= tf && tf + tc <= f + c) {  // an ancestor of the target
-            id = t.left[id];
+        const WalkNode w = t.wn[id];
+        if (w.first <= tf && tf + tc <= w.first + w.count) {  // an ancestor of the target
+            id = w.left;
             continue;
         }
-        const bool is_leaf = c <= t.leaf;
-        const double4 g = t.geo[id];
+        const bool is_leaf = w.count <= t.leaf;
+        const double4 g = w.geo;
         if (is_leaf && g.w == 0.0) {
             if (nx < cap_x) xl[nx] = id;
             ++nx;
-            id = t.skip[id];
+            id = w.skip;
         } else if (g.w == 0.0) {
             // more than leaf_size coincident particles: no opening ball; their expansion is singular
             // for a target at the same point, so the node is opened down to its leaves
-            id = t.left[id];
+            id = w.left;
         } else if (!ball_overlaps_box(g, lo, hi)) {
             if (nn < cap_n) nl[nn] = id;
             ++nn;
-            id = t.skip[id];
+            id = w.skip;
         } else if (is_leaf) {
             if (nx < cap_x) xl[nx] = id;
             ++nx;
-            id = t.skip[id];
+            id = w.skip;
         } else {
-            id = t.left[id];
+            id = w.left;
         }
     }
     ncount[q] = nn;
@@ -443,10 +496,36 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
 // ---- evaluation ---------------------------------------------------------------------------------------
 
 // GravityLutKernel::grad (core/sph/kernel/GravityKernel.h:60-70) times the source mass, added to a.
-// grads == nullptr is the point-particle kernel (radius 0).
+// grads == nullptr is the point-particle kernel (radius 0).  REF evaluates it as the reference does
+// (the direct sum is then bit-identical to it); otherwise the tree sums use the cheaper form below.
+template <bool REF>
 __device__ __forceinline__ void add_pair(const double* __restrict__ grads, double rx, double ry, double rz, double h,
                                          double m, double a[3]) {
 #pragma clang fp contract(off)
+    if (!REF) {
+        // the same branches, decided on |r|^2 against (4 - EPS) h^2 without dividing by h, and the
+        // Newtonian branch with one reciprocal square root
+        const double d2 = rx * rx + ry * ry + rz * rz;
+        if (grads && d2 < (4.0 - EPS_D) * (h * h)) {
+            const double hInv = 1.0 / h;
+            const double qSqr = d2 * (hInv * hInv);
+            const double fi = double(LUT_ENTRIES / 4) * qSqr;
+            const uint32_t i1 = uint32_t(fi);
+            const double ratio = fi - double(i1);
+            const double gr = grads[i1] * (1.0 - ratio) + grads[i1 + 1] * ratio;
+            const double f = m * (hInv * hInv * hInv) * gr;
+            a[0] += rx * f;
+            a[1] += ry * f;
+            a[2] += rz * f;
+            return;
+        }
+        const double il = rsqrt(d2);
+        const double f = m * (il * il * il);
+        a[0] += rx * f;
+        a[1] += ry * f;
+        a[2] += rz * f;
+        return;
+    }
     if (grads) {
         const double hInv = 1.0 / h;
         const double sx = rx * hInv, sy = ry * hInv, sz = rz * hInv;
@@ -474,8 +553,9 @@ __device__ __forceinline__ void add_pair(const double* __restrict__ grads, doubl
 // evaluateGravity (core/gravity/Moments.h:318-360) of node moments at dr = r - com, added to a.
 __device__ __forceinline__ void add_multipole(const NodeMom& q, double drx, double dry, double drz, int order,
                                               double a[3]) {
-    const double inv = 1.0 / (drx * drx + dry * dry + drz * drz);
-    const double g0 = -sqrt(inv);
+    const double il = rsqrt(drx * drx + dry * dry + drz * drz);
+    const double inv = il * il;
+    const double g0 = -il;
     const double g1 = -1.0 * inv * g0;
     const double g2 = -3.0 * inv * g1;
     const double g3 = -5.0 * inv * g2;
@@ -550,13 +630,13 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
         node_range(t, xl[c], f, cnt);
         for (uint32_t j = f; j < f + cnt; ++j) {
             const double4 o = t.pos[j];
-            add_pair(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), t.gm[j], a);
+            add_pair<false>(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), t.gm[j], a);
         }
     }
     for (uint32_t j = tf + sp; j < tf + tc; j += S) {
         if (j == k) continue;
         const double4 o = t.pos[j];
-        add_pair(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), t.gm[j], a);
+        add_pair<false>(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), t.gm[j], a);
     }
     for (uint32_t o = P; o < 64; o <<= 1) {
         for (int c = 0; c < 3; ++c) a[c] += __shfl_xor(a[c], int(o), 64);
@@ -607,7 +687,7 @@ __global__ void __launch_bounds__(BLOCK) k_points(Tree t, const double4* __restr
         // BarnesHut::evalExact (BarnesHut.cpp:376-387)
         for (uint32_t j = f; j < f + c; ++j) {
             const double4 o = t.pos[j];
-            add_pair(grads, o.x - r0.x, o.y - r0.y, o.z - r0.z, o.w, t.gm[j], a);
+            add_pair<false>(grads, o.x - r0.x, o.y - r0.y, o.z - r0.z, o.w, t.gm[j], a);
         }
         id = t.skip[id];
     }
@@ -638,7 +718,7 @@ __global__ void __launch_bounds__(BLOCK) k_direct(Planes s, size_t n, const uint
             for (size_t c = 0; c < cnt; ++c) {
                 if (base + c == i) continue;
                 const double4 o = sp[c];
-                add_pair(grads, o.x - xi, o.y - yi, o.z - zi, 0.5 * (o.w + hi), sm[c], a);
+                add_pair<true>(grads, o.x - xi, o.y - yi, o.z - zi, 0.5 * (o.w + hi), sm[c], a);
             }
         }
         __syncthreads();
@@ -852,6 +932,7 @@ struct osph_gravity {
              *flag = nullptr;
     double4 *geo = nullptr, *lo = nullptr, *hi = nullptr;
     NodeMom* mom = nullptr;
+    WalkNode* wn = nullptr;
     uint32_t *leaf_at = nullptr, *targets = nullptr, *ntargets_dev = nullptr;
     void* select_temp = nullptr;
     size_t select_temp_bytes = 0;
@@ -860,6 +941,7 @@ struct osph_gravity {
     uint32_t *nlist = nullptr, *xlist = nullptr, *ncount = nullptr, *xcount = nullptr;
     uint32_t cap_n = 256, cap_x = 64;
     size_t list_targets = 0;
+    uint32_t* levels = nullptr;           // node counts per depth, then the bucket cursors
     unsigned long long* small = nullptr;  // [0..5] bounds, [6] need (2 x u32)
     unsigned long long* small_host = nullptr;
     double *lut_grads = nullptr, *lut_values = nullptr;
@@ -950,7 +1032,7 @@ int check_traversal(osph_gravity* g) {
 
 void free_particles(osph_gravity* g) {
     void* ptrs[] = { g->dblock, g->keys, g->keys_sorted, g->iota, g->order, g->sort_temp, g->pos, g->gm, g->left,
-                     g->right, g->parent, g->skip, g->first, g->count, g->flag, g->geo, g->lo, g->hi, g->mom,
+                     g->right, g->parent, g->skip, g->first, g->count, g->flag, g->geo, g->lo, g->hi, g->mom, g->wn,
                      g->leaf_at, g->targets, g->ntargets_dev, g->select_temp };
     for (void* p : ptrs) (void)hipFree(p);
     g->dblock = nullptr;
@@ -962,6 +1044,7 @@ void free_particles(osph_gravity* g) {
     g->left = g->right = g->parent = g->skip = g->first = g->count = g->flag = nullptr;
     g->geo = g->lo = g->hi = nullptr;
     g->mom = nullptr;
+    g->wn = nullptr;
     g->leaf_at = g->targets = g->ntargets_dev = nullptr;
     g->cap = 0;
 }
@@ -987,6 +1070,7 @@ int ensure_particles(osph_gravity* g, size_t n) {
     GCHK(g, hipMalloc(&g->lo, sizeof(double4) * nodes));
     GCHK(g, hipMalloc(&g->hi, sizeof(double4) * nodes));
     GCHK(g, hipMalloc(&g->mom, sizeof(NodeMom) * nodes));
+    GCHK(g, hipMalloc(&g->wn, sizeof(WalkNode) * nodes));
     GCHK(g, hipMalloc(&g->parent, sizeof(uint32_t) * ids));
     GCHK(g, hipMalloc(&g->skip, sizeof(uint32_t) * ids));
     GCHK(g, hipMalloc(&g->leaf_at, sizeof(uint32_t) * c));
@@ -1057,6 +1141,7 @@ Tree tree_of(osph_gravity* g) {
     t.lo = g->lo;
     t.hi = g->hi;
     t.mom = g->mom;
+    t.wn = g->wn;
     t.flag = g->flag;
     t.pos = g->pos;
     t.gm = g->gm;
@@ -1066,6 +1151,38 @@ Tree tree_of(osph_gravity* g) {
 }
 
 double decode(unsigned long long e) { return decode_ord_host(e); }
+
+// Node moments, deepest level first (k_depth, k_bucket, k_moments_level).  The flag array holds the
+// depths and leaf_at the nodes bucketed by depth; both are free until the leaves are marked.
+int build_moments(osph_gravity* g, const Tree& t) {
+    const size_t n = g->n;
+    hipStream_t st = g->stream;
+    uint32_t* per_level = g->levels;
+    uint32_t* cursor = g->levels + MAX_DEPTH + 1;
+    timed(g, "depth", [&] {
+        k_fill_u32g<<<1, BLOCK, 0, st>>>(per_level, MAX_DEPTH + 1, 0u);
+        k_depth<<<grid_for(n - 1), BLOCK, 0, st>>>(t, g->flag, per_level);
+    });
+    std::vector<uint32_t> counts(MAX_DEPTH + 1);
+    GCHK(g, hipMemcpyAsync(counts.data(), per_level, counts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (int e = check_traversal(g)) return e;
+    std::vector<uint32_t> off(MAX_DEPTH + 2, 0);
+    int deepest = 0;
+    for (int d = 0; d <= MAX_DEPTH; ++d) {
+        off[d + 1] = off[d] + counts[d];
+        if (counts[d]) deepest = d;
+    }
+    GCHK(g, hipMemcpyAsync(cursor, off.data(), (MAX_DEPTH + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    timed(g, "moments", [&] {
+        k_bucket<<<grid_for(n - 1), BLOCK, 0, st>>>(uint32_t(n - 1), g->flag, cursor, g->leaf_at);
+        for (int d = deepest; d >= 0; --d) {
+            if (counts[d]) {
+                k_moments_level<<<grid_for(counts[d]), BLOCK, 0, st>>>(t, g->leaf_at + off[d], counts[d]);
+            }
+        }
+    });
+    return OK;
+}
 
 int build_tree(osph_gravity* g) {
     const size_t n = g->n;
@@ -1102,7 +1219,7 @@ int build_tree(osph_gravity* g) {
     if (n > 1) {
         timed(g, "radix_tree", [&] { k_radix_tree<<<grid_for(n - 1), BLOCK, 0, st>>>(t, g->keys_sorted); });
         timed(g, "skip", [&] { k_skip<<<grid_for(2 * n - 1), BLOCK, 0, st>>>(t); });
-        timed(g, "moments", [&] { k_moments<<<grid_for(n), BLOCK, 0, st>>>(t); });
+        if (int e = build_moments(g, t)) return e;
     } else {
         const uint32_t endv = END;
         GCHK(g, hipMemcpyAsync(g->parent, &endv, sizeof(uint32_t), hipMemcpyHostToDevice, st));
@@ -1259,6 +1376,7 @@ int osph_gravity_create(osph_gravity** out, const osph_gravity_config* cfg) {
     };
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) return bail(ERR_HIP);
     if (hipMalloc(&g->small, 16 * sizeof(unsigned long long)) != hipSuccess) return bail(ERR_HIP);
+    if (hipMalloc(&g->levels, 2 * (MAX_DEPTH + 1) * sizeof(uint32_t)) != hipSuccess) return bail(ERR_HIP);
     if (hipHostMalloc(&g->small_host, 16 * sizeof(unsigned long long)) != hipSuccess) return bail(ERR_HIP);
     std::vector<double> grads(LUT_ENTRIES + 1), values(LUT_ENTRIES + 1);
     const double qSqrToIdx = double(LUT_ENTRIES) * (1.0 / (2.0 * 2.0));
@@ -1283,8 +1401,8 @@ void osph_gravity_destroy(osph_gravity* g) {
     resolve(g);
     for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
     free_particles(g);
-    void* ptrs[] = { g->nlist, g->xlist, g->ncount, g->xcount, g->small, g->lut_grads, g->lut_values, g->scratch,
-                     g->staging };
+    void* ptrs[] = { g->nlist, g->xlist, g->ncount, g->xcount, g->small, g->levels, g->lut_grads, g->lut_values,
+                     g->scratch, g->staging };
     for (void* p : ptrs) (void)hipFree(p);
     if (g->small_host) (void)hipHostFree(g->small_host);
     if (g->stream) (void)hipStreamDestroy(g->stream);
