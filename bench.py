@@ -151,6 +151,13 @@ def run_nbody(args):
     bh = g.get("dv")[sample]
     direct = g.direct(sample)
     rel = np.linalg.norm(bh - direct, axis=1) / np.linalg.norm(direct, axis=1)
+    rms = np.sqrt(np.mean(np.sum(direct ** 2, axis=1)))
+    # interaction counts of one evaluation (outside the timed region) for the eval kernel's roofline
+    g.build()
+    g.profile(True)
+    _, st = g.eval_self()
+    eval_ms = g.profile_read().get("eval", (float("nan"), 1))[0]
+    g.profile(False)
     name, cus = gpu.device_info()
     out = {"metric": "particle-steps/s, 8M N-body Barnes-Hut + LeapFrog", "value": n * args.steps / elapsed,
            "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -160,7 +167,10 @@ def run_nbody(args):
                                   "quadrupole, leaf 25, LeapFrog dt 1 s", "particles": n, "device": name},
            "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in sorted(prof.items())},
            "accuracy": {"sample": int(sample.size), "rel_err_median": float(np.median(rel)),
-                        "rel_err_max": float(rel.max())}}
+                        "rel_err_max": float(rel.max()),
+                        "err_over_rms_max": float(np.max(np.linalg.norm(bh - direct, axis=1)) / rms)},
+           "interactions": {"node_lists_total": int(st["nodes_approx"]), "leaf_lists_total": int(st["nodes_exact"]),
+                            "eval_ms": eval_ms}}
     print(json.dumps(out), flush=True)
 
 

@@ -593,19 +593,42 @@ __device__ __forceinline__ void add_multipole(const NodeMom& q, double drx, doub
     a[2] += az;
 }
 
-// One wavefront per target leaf.  Lanes are (particle p, split s) with P = pow2 >= count particles and
-// S = 64 / P splits; split s takes every S-th entry of the lists and of the target's own pairs, and the
-// splits are summed with cross-lane adds at the end.  Accelerations are added to the state planes.
+// Synchronisation of the lanes of one wavefront around its private LDS (no work-group barrier: the
+// wavefronts of a work-group run different targets with different loop counts).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-wavefront staging area of k_eval: 64 node expansions (SoA) or 64 source particles.
+struct EvalStage {
+    double c[3][64];
+    double M[64];
+    double q2[5][64];
+    double q3[7][64];
+    double4 src[64];
+    double sm[64];
+    uint32_t sidx[64];
+};
+
+// One wavefront per target leaf.  Lanes are (particle p, split sp) with P = pow2 >= count particles and
+// S = 64 / P splits.  The node list and the source particles (every listed leaf followed by the target
+// leaf itself) are staged 64 at a time in the wavefront's LDS with coalesced loads; split sp then takes
+// every S-th staged entry, and the splits are summed with cross-lane adds at the end.  Accelerations
+// are added to the state planes.
+template <int ORDER>
 __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t* __restrict__ order,
                                                 const uint32_t* __restrict__ targets, uint32_t ntargets,
                                                 const uint32_t* __restrict__ nlist, uint32_t cap_n,
                                                 const uint32_t* __restrict__ xlist, uint32_t cap_x,
                                                 const uint32_t* __restrict__ ncount,
-                                                const uint32_t* __restrict__ xcount, const double* __restrict__ grads,
-                                                int order_m) {
+                                                const uint32_t* __restrict__ xcount, const double* __restrict__ grads) {
+    __shared__ EvalStage stage[BLOCK / 64];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    EvalStage& S_ = stage[wave];
     const uint32_t q = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64)) + wave;
-    if (q >= ntargets) return;
+    if (q >= ntargets) return;  // uniform over the wavefront
     const uint32_t T = targets[q];
     uint32_t tf, tc;
     node_range(t, T, tf, tc);
@@ -620,23 +643,84 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
     const uint32_t nn = min(ncount[q], cap_n), nx = min(xcount[q], cap_x);
     const uint32_t* nl = nlist + size_t(q) * cap_n;
     const uint32_t* xl = xlist + size_t(q) * cap_x;
-    for (uint32_t c = sp; c < nn; c += S) {
-        const uint32_t id = nl[c];
-        const double4 g = t.geo[id];
-        add_multipole(t.mom[id], me.x - g.x, me.y - g.y, me.z - g.z, order_m, a);
-    }
-    for (uint32_t c = sp; c < nx; c += S) {
-        uint32_t f, cnt;
-        node_range(t, xl[c], f, cnt);
-        for (uint32_t j = f; j < f + cnt; ++j) {
-            const double4 o = t.pos[j];
-            add_pair<false>(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), t.gm[j], a);
+
+    // node list
+    for (uint32_t base = 0; base < nn; base += 64) {
+        const uint32_t cnt = min(64u, nn - base);
+        if (lane < cnt) {
+            const uint32_t id = nl[base + lane];
+            const double4 g = t.geo[id];
+            const NodeMom& mm = t.mom[id];
+            S_.c[0][lane] = g.x;
+            S_.c[1][lane] = g.y;
+            S_.c[2][lane] = g.z;
+            S_.M[lane] = mm.M;
+            if (ORDER >= 2) {
+                for (int c = 0; c < 5; ++c) S_.q2[c][lane] = mm.q2[c];
+            }
+            if (ORDER >= 3) {
+                for (int c = 0; c < 7; ++c) S_.q3[c][lane] = mm.q3[c];
+            }
         }
+        wave_sync();
+        for (uint32_t j = sp; j < cnt; j += S) {
+            NodeMom mm;
+            mm.M = S_.M[j];
+            if (ORDER >= 2) {
+                for (int c = 0; c < 5; ++c) mm.q2[c] = S_.q2[c][j];
+            }
+            if (ORDER >= 3) {
+                for (int c = 0; c < 7; ++c) mm.q3[c] = S_.q3[c][j];
+            }
+            add_multipole(mm, me.x - S_.c[0][j], me.y - S_.c[1][j], me.z - S_.c[2][j], ORDER, a);
+        }
+        wave_sync();
     }
-    for (uint32_t j = tf + sp; j < tf + tc; j += S) {
-        if (j == k) continue;
-        const double4 o = t.pos[j];
-        add_pair<false>(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), t.gm[j], a);
+
+    // source particles: entries e < nx are the listed leaves, entry nx is the target leaf
+    const uint32_t nent = nx + 1;
+    for (uint32_t eb = 0; eb < nent; eb += 64) {
+        // entry eb + lane: its range and the exclusive prefix of the counts
+        uint32_t f = 0, c = 0;
+        const uint32_t e = eb + lane;
+        if (e < nx) {
+            node_range(t, xl[e], f, c);
+        } else if (e == nx) {
+            f = tf;
+            c = tc;
+        }
+        uint32_t incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if (lane >= uint32_t(o)) incl += v;
+        }
+        const uint32_t excl = incl - c;
+        const uint32_t total = __shfl(incl, 63, 64);
+        for (uint32_t s0 = 0; s0 < total; s0 += 64) {
+            const uint32_t slot = s0 + lane;
+            const uint32_t cnt = min(64u, total - s0);
+            // the entry holding this slot: the last lane whose exclusive prefix is <= slot (all lanes take
+            // part in the cross-lane reads; only lanes with a slot load)
+            int lo = 0;
+            for (int step = 32; step > 0; step >>= 1) {
+                const uint32_t ex = __shfl(excl, (lo + step) & 63, 64);
+                if (lo + step < 64 && ex <= slot) lo += step;
+            }
+            const uint32_t fe = __shfl(f, lo, 64), xe = __shfl(excl, lo, 64);
+            if (lane < cnt) {
+                const uint32_t src = fe + (slot - xe);
+                S_.src[lane] = t.pos[src];
+                S_.sm[lane] = t.gm[src];
+                S_.sidx[lane] = src;
+            }
+            wave_sync();
+            for (uint32_t j = sp; j < cnt; j += S) {
+                if (S_.sidx[j] == k) continue;
+                const double4 o = S_.src[j];
+                add_pair<false>(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), S_.sm[j], a);
+            }
+            wave_sync();
+        }
     }
     for (uint32_t o = P; o < 64; o <<= 1) {
         for (int c = 0; c < 3; ++c) a[c] += __shfl_xor(a[c], int(o), 64);
@@ -1272,8 +1356,17 @@ int eval_bh(osph_gravity* g, osph_gravity_stats* stats) {
     const double* grads = g->cfg.kernel == OSPH_GRAVITY_KERNEL_M4 ? g->lut_grads : nullptr;
     const unsigned blocks = unsigned((nt + (BLOCK / 64) - 1) / (BLOCK / 64));
     timed(g, "eval", [&] {
-        k_eval<<<blocks > 0 ? blocks : 1, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist,
-                                                         g->cap_x, g->ncount, g->xcount, grads, ord);
+        const unsigned nb = blocks > 0 ? blocks : 1;
+        if (ord == 3) {
+            k_eval<3><<<nb, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist, g->cap_x,
+                                            g->ncount, g->xcount, grads);
+        } else if (ord == 2) {
+            k_eval<2><<<nb, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist, g->cap_x,
+                                            g->ncount, g->xcount, grads);
+        } else {
+            k_eval<0><<<nb, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist, g->cap_x,
+                                            g->ncount, g->xcount, grads);
+        }
     });
     if (stats) {
         std::vector<uint32_t> a(nt), b(nt);
