@@ -169,8 +169,8 @@ def run_nbody(args):
            "accuracy": {"sample": int(sample.size), "rel_err_median": float(np.median(rel)),
                         "rel_err_max": float(rel.max()),
                         "err_over_rms_max": float(np.max(np.linalg.norm(bh - direct, axis=1)) / rms)},
-           "interactions": {"node_lists_total": int(st["nodes_approx"]), "leaf_lists_total": int(st["nodes_exact"]),
-                            "eval_ms": eval_ms}}
+           "interactions": {"nodes_approx": int(st["nodes_approx"]), "leaves_exact": int(st["nodes_exact"]),
+                            "expansions": int(st["expansions"]), "pairs": int(st["pairs"]), "eval_ms": eval_ms}}
     print(json.dumps(out), flush=True)
 
 

@@ -131,6 +131,11 @@ int osph_integrate(osph_ctx* ctx, osph_stats* stats);
 int osph_step_pc(osph_ctx* ctx, double dt, double* next_dt, int* criterion);
 int osph_timestep(osph_ctx* ctx, double* dt, int* criterion);
 int osph_sync(osph_ctx* ctx);
+/* Integrals of the resident state (TotalEnergy, TotalKineticEnergy, TotalInternalEnergy, TotalMomentum,
+ * TotalAngularMomentum about the origin, CenterOfMass, TotalMass; core/physics/Integrals.cpp:9-180):
+ * out = { E, E_kin, E_int, p x y z, L x y z, com x y z, M }, FP64 with a fixed reduction order. */
+enum { OSPH_INTEGRALS = 13 };
+int osph_integrals(osph_ctx* ctx, double out[OSPH_INTEGRALS]);
 /* Launch-shape tuning of the neighbour-list and pair kernels (no reference counterpart: the reference
  * has no device kernels).  Keys: "pairs_block", "lists_block" (64, 128 or 256 threads) and
  * "pairs_lds", "lists_lds" (0..65536 bytes of dynamic LDS per block, which caps resident blocks per CU),

@@ -57,6 +57,8 @@ typedef struct {
     uint64_t nodes_approx;  /* StatisticsId::GRAVITY_NODES_APPROX: nodes evaluated by their moments */
     uint64_t nodes_exact;   /* GRAVITY_NODES_EXACT: leaves evaluated pair by pair */
     uint64_t node_count;    /* GRAVITY_NODE_COUNT */
+    uint64_t expansions;    /* particle-node multipole evaluations (no reference counterpart: work count) */
+    uint64_t pairs;         /* particle-particle kernel evaluations (likewise) */
 } osph_gravity_stats;
 
 int osph_gravity_create(osph_gravity** g, const osph_gravity_config* cfg);

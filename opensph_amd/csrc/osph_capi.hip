@@ -63,6 +63,8 @@ struct osph_ctx {
 
     unsigned char* staging = nullptr;
     size_t staging_cap = 0;
+    double* scratch = nullptr;  // reductions (integrals)
+    size_t scratch_cap = 0;
 
     bool prof = false;
     std::vector<ProfEvent> events;
@@ -678,6 +680,7 @@ void osph_destroy(osph_ctx* c) {
     (void)hipFree(c->cend);
     (void)hipFree(c->list);
     (void)hipFree(c->staging);
+    (void)hipFree(c->scratch);
     (void)hipFree(c->mats_dev);
     (void)hipFree(c->small);
     (void)hipFree(c->lut);
@@ -819,6 +822,36 @@ int osph_integrate(osph_ctx* c, osph_stats* stats) {
     if (!c) return OSPH_ERR_ARG;
     if (int e = integrate(c, stats)) return e;
     return sync_check(c);
+}
+
+int osph_integrals(osph_ctx* c, double out[OSPH_INTEGRALS]) {
+    if (!c || !out) return OSPH_ERR_ARG;
+    std::memset(out, 0, OSPH_INTEGRALS * sizeof(double));
+    const size_t n = c->n;
+    if (!n) return OSPH_OK;
+    const size_t need = integrals_scratch_doubles(n);
+    if (need > c->scratch_cap) {
+        (void)hipFree(c->scratch);
+        c->scratch = nullptr;
+        c->scratch_cap = 0;
+        HIPCHK(c, hipMalloc(&c->scratch, need * sizeof(double)));
+        c->scratch_cap = need;
+    }
+    timed(c, "integrals", [&] { launch_integrals(L(c), c->s, c->scratch, n); });
+    double v[12];
+    HIPCHK(c, hipMemcpyAsync(v, c->scratch + need - 12, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    if (int e = sync_check(c)) return e;
+    // TotalEnergy = kinetic + internal; CenterOfMass = sum m r / sum m (Integrals.cpp:95-180)
+    out[0] = v[0] + v[1];
+    out[1] = v[0];
+    out[2] = v[1];
+    for (int k = 0; k < 3; ++k) {
+        out[3 + k] = v[2 + k];
+        out[6 + k] = v[5 + k];
+        out[9 + k] = v[8 + k] / v[11];
+    }
+    out[12] = v[11];
+    return OSPH_OK;
 }
 
 int osph_timestep(osph_ctx* c, double* dt, int* crit) {

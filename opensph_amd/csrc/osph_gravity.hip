@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
                                                 uint32_t* __restrict__ nlist, uint32_t cap_n,
                                                 uint32_t* __restrict__ xlist, uint32_t cap_x,
                                                 uint32_t* __restrict__ ncount, uint32_t* __restrict__ xcount,
-                                                uint32_t* need) {
+                                                uint32_t* __restrict__ work, uint32_t* need) {
     const uint32_t q = uint32_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= ntargets) return;
     const uint32_t T = targets[q];
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
         hi[1] = b.y;
         hi[2] = b.z;
     }
-    uint32_t nn = 0, nx = 0;
+    uint32_t nn = 0, nx = 0, xsum = 0;
     uint32_t* nl = nlist + size_t(q) * cap_n;
     uint32_t* xl = xlist + size_t(q) * cap_x;
     uint32_t id = 0;  // the root: internal node 0, or particle 0 when n == 1
@@ -457,6 +457,7 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
             // a single-particle leaf has r_open = 0: always summed pair by pair (BarnesHut.cpp:237-243)
             if (nx < cap_x) xl[nx] = id;
             ++nx;
+            ++xsum;
             id = t.skip[id];
             continue;
         }
@@ -470,6 +471,7 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
         if (is_leaf && g.w == 0.0) {
             if (nx < cap_x) xl[nx] = id;
             ++nx;
+            xsum += w.count;
             id = w.skip;
         } else if (g.w == 0.0) {
             // more than leaf_size coincident particles: no opening ball; their expansion is singular
@@ -482,6 +484,7 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
         } else if (is_leaf) {
             if (nx < cap_x) xl[nx] = id;
             ++nx;
+            xsum += w.count;
             id = w.skip;
         } else {
             id = w.left;
@@ -489,6 +492,8 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
     }
     ncount[q] = nn;
     xcount[q] = nx;
+    work[2 * size_t(q)] = tc * nn;                        // particle-node expansions
+    work[2 * size_t(q) + 1] = tc * xsum + tc * (tc - 1);  // particle pairs
     if (nn > cap_n) atomicMax(&need[0], nn);
     if (nx > cap_x) atomicMax(&need[1], nx);
 }
@@ -1022,7 +1027,7 @@ struct osph_gravity {
     size_t select_temp_bytes = 0;
     uint32_t ntargets = 0;
     // interaction lists
-    uint32_t *nlist = nullptr, *xlist = nullptr, *ncount = nullptr, *xcount = nullptr;
+    uint32_t *nlist = nullptr, *xlist = nullptr, *ncount = nullptr, *xcount = nullptr, *work = nullptr;
     uint32_t cap_n = 256, cap_x = 64;
     size_t list_targets = 0;
     uint32_t* levels = nullptr;           // node counts per depth, then the bucket cursors
@@ -1200,13 +1205,15 @@ int ensure_lists(osph_gravity* g, size_t targets) {
     (void)hipFree(g->xlist);
     (void)hipFree(g->ncount);
     (void)hipFree(g->xcount);
-    g->nlist = g->xlist = g->ncount = g->xcount = nullptr;
+    (void)hipFree(g->work);
+    g->nlist = g->xlist = g->ncount = g->xcount = g->work = nullptr;
     g->list_targets = 0;
     const size_t t = need > 0 ? need : 1;
     GCHK(g, hipMalloc(&g->nlist, sizeof(uint32_t) * t * g->cap_n));
     GCHK(g, hipMalloc(&g->xlist, sizeof(uint32_t) * t * g->cap_x));
     GCHK(g, hipMalloc(&g->ncount, sizeof(uint32_t) * t));
     GCHK(g, hipMalloc(&g->xcount, sizeof(uint32_t) * t));
+    GCHK(g, hipMalloc(&g->work, 2 * sizeof(uint32_t) * t));
     g->list_targets = t;
     return OK;
 }
@@ -1337,7 +1344,7 @@ int eval_bh(osph_gravity* g, osph_gravity_stats* stats) {
         timed(g, "walk", [&] {
             k_fill_u32g<<<1, 64, 0, st>>>(need, 2, 0u);
             k_walk<<<grid_for(nt), BLOCK, 0, st>>>(t, g->targets, nt, g->nlist, g->cap_n, g->xlist, g->cap_x, g->ncount,
-                                                   g->xcount, need);
+                                                   g->xcount, g->work, need);
         });
         GCHK(g, hipMemcpyAsync(g->small_host + 6, need, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         if (int e = check_traversal(g)) return e;
@@ -1369,16 +1376,19 @@ int eval_bh(osph_gravity* g, osph_gravity_stats* stats) {
         }
     });
     if (stats) {
-        std::vector<uint32_t> a(nt), b(nt);
+        std::vector<uint32_t> a(nt), b(nt), w(2 * size_t(nt));
         if (nt) {
             GCHK(g, hipMemcpyAsync(a.data(), g->ncount, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             GCHK(g, hipMemcpyAsync(b.data(), g->xcount, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            GCHK(g, hipMemcpyAsync(w.data(), g->work, w.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         }
         if (int e = sync(g)) return e;
-        stats->nodes_approx = stats->nodes_exact = 0;
+        *stats = osph_gravity_stats{};
         for (uint32_t k = 0; k < nt; ++k) {
             stats->nodes_approx += a[k];
             stats->nodes_exact += b[k];
+            stats->expansions += w[2 * size_t(k)];
+            stats->pairs += w[2 * size_t(k) + 1];
         }
         stats->node_count = 2 * uint64_t(n) - 1;
     }
@@ -1423,7 +1433,10 @@ int eval_self(osph_gravity* g, osph_gravity_stats* stats) {
     k_iota<<<grid_for(n), BLOCK, 0, g->stream>>>(tg, n);
     if (int e = eval_direct(g, tg, n, out)) return e;
     k_add_direct<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, tg, out, n);
-    if (stats) *stats = osph_gravity_stats{ 0, 0, 0 };
+    if (stats) {
+        *stats = osph_gravity_stats{};
+        stats->pairs = uint64_t(n) * (n - 1);
+    }
     return OK;
 }
 
@@ -1494,7 +1507,7 @@ void osph_gravity_destroy(osph_gravity* g) {
     resolve(g);
     for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
     free_particles(g);
-    void* ptrs[] = { g->nlist, g->xlist, g->ncount, g->xcount, g->small, g->levels, g->lut_grads, g->lut_values,
+    void* ptrs[] = { g->nlist, g->xlist, g->ncount, g->xcount, g->work, g->small, g->levels, g->lut_grads, g->lut_values,
                      g->scratch, g->staging };
     for (void* p : ptrs) (void)hipFree(p);
     if (g->small_host) (void)hipHostFree(g->small_host);

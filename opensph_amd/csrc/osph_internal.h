@@ -181,6 +181,10 @@ void launch_stage_to_planes(const Launch& l, const unsigned char* staging, size_
 void launch_planes_to_stage(const Launch& l, unsigned char* staging, size_t stride, int ncomp,
                             const int* offsets_bytes, int elem_bytes, void* const* src, size_t n);
 void launch_stream_copy(hipStream_t stream, const void* a, void* b, size_t bytes);
+// Integrals of the state into scratch (integrals_scratch_doubles(n) doubles); the 12 results are the
+// last 12 doubles.
+size_t integrals_scratch_doubles(size_t n);
+void launch_integrals(const Launch& l, DevState s, double* scratch, size_t n);
 size_t sort_pairs_temp_bytes(size_t n);
 void sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
                 const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits);

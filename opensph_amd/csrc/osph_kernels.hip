@@ -649,20 +649,16 @@ __device__ __forceinline__ float m4_grad_f(float q2) {
     return -0.75f * INV_PI * (t * t) / q;
 }
 
-// Kahan-compensated FP32 accumulator.
-struct Kahan {
-    float sum = 0.f, c = 0.f;
-    __device__ __forceinline__ void add(float v) {
-        const float y = v - c;
-        const float t = sum + y;
-        c = (t - sum) - y;
-        sum = t;
-    }
-    __device__ __forceinline__ double value() const { return double(sum) - double(c); }
+// FP64 accumulator of FP32 terms: each term is widened exactly and summed in FP64 (on gfx950 an FP64 add
+// issues at the rate of an unpacked FP32 add, so this costs two instructions per term).
+struct Acc {
+    double sum = 0.0;
+    __device__ __forceinline__ void add(float v) { sum += double(v); }
+    __device__ __forceinline__ double value() const { return sum; }
 };
 
 // Mixed-precision pair sums: the same terms and gather form as k_pairs_fp64, evaluated in FP32 on
-// lattice-relative positions with Kahan-compensated FP32 accumulators; results are written in FP64.
+// lattice-relative positions and accumulated in FP64; results are written in FP64.
 __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
                                                        const PairRecF* __restrict__ rec,
                                                        const uint32_t* __restrict__ order,
@@ -675,7 +671,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
     const int lxi = int(me.lat & 1023u), lyi = int((me.lat >> 10) & 1023u), lzi = int(me.lat >> 20);
     const float rhoInv = 1.0f / me.rho;
     const float alpha = float(prm.av_alpha), beta = float(prm.av_beta);
-    Kahan ax, ay, az, du, divv, gxx, gyy, gzz, gxy, gxz, gyz;
+    Acc ax, ay, az, du, divv, gxx, gyy, gzz, gxy, gxz, gyz;
     const uint32_t cnt = count[k];
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
@@ -1076,7 +1072,67 @@ __global__ void __launch_bounds__(BLOCK) k_stream_copy(const double2* __restrict
     }
 }
 
+// Integrals of the state (core/physics/Integrals.cpp:24-180): per block, in a fixed reduction order, then
+// one block over the partials.  v[0] kinetic energy, v[1] internal energy, v[2..4] momentum, v[5..7]
+// angular momentum about the origin, v[8..10] sum of m r, v[11] total mass.
+constexpr int NINT = 12;
+
+__global__ void __launch_bounds__(BLOCK) k_integrals(DevState s, size_t n, double* __restrict__ partial) {
+    double v[NINT] = { 0.0 };
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const double m = s.d[M][i];
+        const double x = s.d[X][i], y = s.d[Y][i], z = s.d[Z][i];
+        const double vx = s.d[VX][i], vy = s.d[VY][i], vz = s.d[VZ][i];
+        v[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        v[1] += m * s.d[U][i];
+        v[2] += m * vx;
+        v[3] += m * vy;
+        v[4] += m * vz;
+        v[5] += m * (y * vz - z * vy);
+        v[6] += m * (z * vx - x * vz);
+        v[7] += m * (x * vy - y * vx);
+        v[8] += m * x;
+        v[9] += m * y;
+        v[10] += m * z;
+        v[11] += m;
+    }
+    __shared__ double red[BLOCK / 64][NINT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < NINT; ++k) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) red[wave][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NINT) {
+        double x = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) x += red[w][threadIdx.x];
+        partial[size_t(blockIdx.x) * NINT + threadIdx.x] = x;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_integrals_final(const double* __restrict__ partial, int nblocks,
+                                                        double* __restrict__ out) {
+    if (threadIdx.x >= NINT) return;
+    double x = 0.0;
+    for (int b = 0; b < nblocks; ++b) x += partial[size_t(b) * NINT + threadIdx.x];
+    out[threadIdx.x] = x;
+}
+
 }  // namespace
+
+size_t integrals_scratch_doubles(size_t n) {
+    unsigned g = grid_for(n);
+    g = g > 1024 ? 1024 : g;
+    return size_t(g) * NINT + NINT;
+}
+
+void launch_integrals(const Launch& l, DevState s, double* scratch, size_t n) {
+    unsigned g = grid_for(n);
+    g = g > 1024 ? 1024 : g;
+    k_integrals<<<g, BLOCK, 0, l.stream>>>(s, n, scratch);
+    k_integrals_final<<<1, 64, 0, l.stream>>>(scratch, int(g), scratch + size_t(g) * NINT);
+}
 
 void launch_stream_copy(hipStream_t stream, const void* a, void* b, size_t bytes) {
     const size_t n2 = bytes / sizeof(double2);

@@ -28,7 +28,7 @@ PRECISION = {"fp64": 0, "mixed": 1}
 EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info", "osph_resize",
            "osph_set_quantity", "osph_get_quantity", "osph_zero_derivatives", "osph_build_neighbors",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
-           "osph_stream_copy", "osph_tune",
+           "osph_stream_copy", "osph_tune", "osph_integrals",
            "osph_profile_enable", "osph_profile_read"]
 
 
@@ -83,6 +83,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_kernel_lut.argtypes = [dp, dp]
     L.osph_stream_copy.argtypes = [sz, i, dp]
     L.osph_tune.argtypes = [vp, C.c_char_p, C.c_longlong]
+    L.osph_integrals.argtypes = [vp, dp]
     L.osph_profile_enable.argtypes = [vp, i]
     L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
     _lib = L
@@ -247,6 +248,13 @@ class GpuSolver:
         crit = C.c_int()
         self._check(self.L.osph_timestep(self.ctx, C.byref(dt), C.byref(crit)), "osph_timestep")
         return dt.value, crit.value
+
+    def integrals(self) -> Dict[str, object]:
+        """Integrals.cpp diagnostics of the resident state (osph_integrals)."""
+        out = np.zeros(13)
+        self._check(self.L.osph_integrals(self.ctx, out.ctypes.data_as(C.POINTER(C.c_double))), "osph_integrals")
+        return {"energy": out[0], "kinetic_energy": out[1], "internal_energy": out[2], "momentum": out[3:6],
+                "angular_momentum": out[6:9], "center_of_mass": out[9:12], "mass": out[12]}
 
     def pairs(self) -> Tuple[np.ndarray, np.ndarray]:
         self._check(self.L.osph_build_neighbors(self.ctx), "osph_build_neighbors")

@@ -33,7 +33,8 @@ class GravityConfig(C.Structure):
 
 
 class GravityStats(C.Structure):
-    _fields_ = [("nodes_approx", C.c_uint64), ("nodes_exact", C.c_uint64), ("node_count", C.c_uint64)]
+    _fields_ = [("nodes_approx", C.c_uint64), ("nodes_exact", C.c_uint64), ("node_count", C.c_uint64),
+                ("expansions", C.c_uint64), ("pairs", C.c_uint64)]
 
 
 _bound = False
@@ -120,8 +121,7 @@ class GpuGravity:
         st = GravityStats()
         self._check(self.L.osph_gravity_eval_self(self.ctx, out.ctypes.data, 32, C.byref(st)),
                     "osph_gravity_eval_self")
-        return out[:, :3].copy(), {"nodes_approx": st.nodes_approx, "nodes_exact": st.nodes_exact,
-                                   "node_count": st.node_count}
+        return out[:, :3].copy(), {name: getattr(st, name) for name, _ in GravityStats._fields_}
 
     def eval_points(self, p: np.ndarray) -> np.ndarray:
         p4 = _vec4(p)

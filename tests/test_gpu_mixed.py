@@ -1,4 +1,4 @@
-"""The mixed-precision path (FP32 lattice-relative pair sums with Kahan accumulators, FP64 state and
+"""The mixed-precision path (FP32 lattice-relative pair sums accumulated in FP64, FP64 state and
 integration): derivative error against the FP64 path on the same state, exact neighbour counts, and
 energy conservation over 100 predictor-corrector steps."""
 import numpy as np
