@@ -174,6 +174,61 @@ def run_nbody(args):
     print(json.dumps(out), flush=True)
 
 
+def ensemble_members(count, seed=11):
+    """Impact ensemble: N ~ exp(U(ln 5e3, ln 2e5)), R_p/R_t ~ U(0.05, 0.2) at equal particle spacing,
+    angle ~ U(0, 90) deg, speed ~ U(3, 7) km/s, R_t = 50 km, basalt; one seed per member."""
+    from opensph_amd import initial
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = float(np.exp(rng.uniform(np.log(5e3), np.log(2e5))))
+        ratio = rng.uniform(0.05, 0.2)
+        n_p = max(100, int(n * ratio ** 3 / (1 + ratio ** 3)))
+        n_t = max(100, int(n) - n_p)
+        st, cfg = initial.impact(n_target=n_t, n_projectile=n_p, r_target=50e3, r_projectile=50e3 * ratio,
+                                 speed=rng.uniform(3e3, 7e3), angle_deg=rng.uniform(0.0, 90.0),
+                                 seed=int(rng.integers(1 << 30)), flaws=True)
+        out.append((st, cfg))
+    return out
+
+
+def run_ensemble(args):
+    """Impact ensemble: all members stepped together (osph_step_pc_batch) against one after another."""
+    from opensph_amd import gpu
+
+    t0 = time.perf_counter()
+    ms = ensemble_members(args.ensemble_n)
+    total = sum(st.n for st, _ in ms)
+    log(f"ensemble of {len(ms)} members, {total} particles, set up in {time.perf_counter() - t0:.1f}s")
+    solvers = [gpu.GpuSolver(st, cfg, precision="mixed") for st, cfg in ms]
+    for _ in range(args.warmup):
+        gpu.step_batch(solvers)
+    for s in solvers:
+        s.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gpu.step_batch(solvers)
+    for s in solvers:
+        s.sync()
+    batched = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for s in solvers:
+            s.step()
+    for s in solvers:
+        s.sync()
+    sequential = time.perf_counter() - t0
+    name, _ = gpu.device_info()
+    out = {"metric": "particle-steps/s, impact ensemble (mixed precision, per-member time steps)",
+           "value": total * args.steps / batched, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * batched / args.steps, "higher_is_better": True,
+           "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": "impact ensemble", "members": len(ms), "particles": total, "device": name},
+           "sequential_value": total * args.steps / sequential}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,9 +241,11 @@ def main():
                     help="launch-shape setting key=value passed to osph_tune (e.g. pairs_lds=40960)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the 1 GiB device copy (HBM roof, FETCH_SIZE/WRITE_SIZE calibration)")
-    ap.add_argument("--workload", default="impact", choices=["impact", "nbody"],
-                    help="impact: the SPH headline; nbody: self-gravity Barnes-Hut + LeapFrog")
+    ap.add_argument("--workload", default="impact", choices=["impact", "nbody", "ensemble"],
+                    help="impact: the SPH headline; nbody: self-gravity Barnes-Hut + LeapFrog; "
+                         "ensemble: batched impact ensemble")
     ap.add_argument("--nbody-n", type=int, default=8_000_000)
+    ap.add_argument("--ensemble-n", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X; "
@@ -199,6 +256,9 @@ def main():
 
     if args.workload == "nbody":
         run_nbody(args)
+        return
+    if args.workload == "ensemble":
+        run_ensemble(args)
         return
 
     if args.calibrate:

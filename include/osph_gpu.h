@@ -130,6 +130,12 @@ typedef struct {
 int osph_integrate(osph_ctx* ctx, osph_stats* stats);
 int osph_step_pc(osph_ctx* ctx, double dt, double* next_dt, int* criterion);
 int osph_timestep(osph_ctx* ctx, double* dt, int* criterion);
+/* One osph_step_pc of each of `count` independent contexts (an ensemble of runs, each with its own
+ * time step dt[k]), interleaved phase by phase so that the contexts' streams run concurrently and the
+ * host waits overlap.  Each context's result is bit-identical to osph_step_pc on it alone.  On failure
+ * the first failing context's code is returned and its osph_last_error() holds the message.  No
+ * reference counterpart: the reference runs ensembles as separate jobs. */
+int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* next_dt, int* criterion);
 int osph_sync(osph_ctx* ctx);
 /* Integrals of the resident state (TotalEnergy, TotalKineticEnergy, TotalInternalEnergy, TotalMomentum,
  * TotalAngularMomentum about the origin, CenterOfMass, TotalMass; core/physics/Integrals.cpp:9-180):

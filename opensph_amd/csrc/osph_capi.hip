@@ -350,7 +350,8 @@ Screen make_screen(const osph_ctx* c, const double lo[3], const double hi[3], do
 // modes 1 and 2) with the per-class grid bounds, then the grid, the sort by cell, the cell ranges and
 // the packed neighbour records (ISymmetricFinder::build).  One host synchronisation reads the class
 // bounds; the first search after an upload adds one more for the class reference h.
-int search_grid(osph_ctx* c, int mode, double dt) {
+// First half of search_grid: everything up to the class-bound readback, which is left in flight.
+int search_begin(osph_ctx* c, int mode, double dt) {
     const size_t n = c->n;
     Launch l = L(c);
     timed(c, "init_small", [&] { launch_init_small(l, c->small); });
@@ -369,6 +370,14 @@ int search_grid(osph_ctx* c, int mode, double dt) {
           [&] { launch_advance(l, c->s, c->mats_dev, c->prm, mode, dt, href, cacc, n); });
     HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CLASSES, cacc, CLASS_STRIDE * MAX_CLASSES * sizeof(unsigned long long),
                              hipMemcpyDeviceToHost, c->stream));
+    return OSPH_OK;
+}
+
+// Second half: waits for the class bounds, then the grid, the sort by cell, the ranges and the records.
+int search_finish(osph_ctx* c) {
+    const size_t n = c->n;
+    Launch l = L(c);
+    const double href = c->href;
     if (int e = sync_check(c)) return e;
 
     double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
@@ -469,6 +478,11 @@ int search_grid(osph_ctx* c, int mode, double dt) {
         timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->pos, c->posf, sc, c->group, n); });
     }
     return OSPH_OK;
+}
+
+int search_grid(osph_ctx* c, int mode, double dt) {
+    if (int e = search_begin(c, mode, dt)) return e;
+    return search_finish(c);
 }
 
 uint32_t* overflow_flag(osph_ctx* c) { return reinterpret_cast<uint32_t*>(c->small + SMALL_OVERFLOW); }
@@ -866,31 +880,45 @@ int osph_timestep(osph_ctx* c, double* dt, int* crit) {
     return OSPH_OK;
 }
 
-// One PredictorCorrector step with the state resident: the fused predict pass and neighbour search,
-// the lists, the pair sums and the fused finalize + correct + criteria.  The list overflow flag is read
-// with the criteria at the end of the step; after an overflow the finalize pass has left the predicted
-// state untouched, so the lists are grown and the step's second half is run again.
-int osph_step_pc(osph_ctx* c, double dt, double* next_dt, int* crit) {
-    if (!c) return OSPH_ERR_ARG;
+// One PredictorCorrector step with the state resident, in three phases so that a batch of contexts
+// can interleave them:
+//   begin   the fused predict pass and the class bounds (readback in flight);
+//   middle  the grid, the lists, the pair sums and the fused finalize + correct + criteria (readback of
+//           the criterion minima and the list overflow flag in flight);
+//   end     the time step, or -- after a list overflow, when the finalize pass has left the predicted
+//           state untouched -- larger lists and the second half of the step again.
+int step_begin(osph_ctx* c, double dt) {
     if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(OSPH_ERR_ARG, "time step must be positive");
-    const size_t n = c->n;
-    Launch l = L(c);
     c->neighbors_valid = false;
-    if (n == 0) {
+    if (c->n == 0) return OSPH_OK;
+    return search_begin(c, 2, dt);
+}
+
+void launch_second_half(osph_ctx* c, double dt) {
+    launch_pair_sums(c);
+    timed(c, "finalize_correct", [&] {
+        launch_finalize_correct(L(c), c->s, c->mats_dev, c->prm, dt, c->small + SMALL_CRITERIA, overflow_flag(c), c->n);
+    });
+    (void)hipMemcpyAsync(c->small_host + SMALL_CRITERIA, c->small + SMALL_CRITERIA,
+                         (SMALL_OVERFLOW + 1 - SMALL_CRITERIA) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         c->stream);
+}
+
+int step_middle(osph_ctx* c, double dt) {
+    if (c->n == 0) return OSPH_OK;
+    if (int e = search_finish(c)) return e;
+    if (int e = launch_lists(c)) return e;
+    launch_second_half(c, dt);
+    return OSPH_OK;
+}
+
+int step_end(osph_ctx* c, double dt, double* next_dt, int* crit) {
+    if (c->n == 0) {
         unsigned long long none[4] = { ~0ull, ~0ull, ~0ull, ~0ull };
         select_timestep(c, none, next_dt, crit);
         return OSPH_OK;
     }
-    if (int e = search_grid(c, 2, dt)) return e;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        if (int e = launch_lists(c)) return e;
-        launch_pair_sums(c);
-        timed(c, "finalize_correct", [&] {
-            launch_finalize_correct(l, c->s, c->mats_dev, c->prm, dt, c->small + SMALL_CRITERIA, overflow_flag(c), n);
-        });
-        HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CRITERIA, c->small + SMALL_CRITERIA,
-                                 (SMALL_OVERFLOW + 1 - SMALL_CRITERIA) * sizeof(unsigned long long),
-                                 hipMemcpyDeviceToHost, c->stream));
         if (int e = sync_check(c)) return e;
         if (!overflowed(c)) {
             c->neighbors_valid = true;
@@ -898,8 +926,37 @@ int osph_step_pc(osph_ctx* c, double dt, double* next_dt, int* crit) {
             return OSPH_OK;
         }
         if (int e = grow_lists(c)) return e;
+        if (int e = launch_lists(c)) return e;
+        launch_second_half(c, dt);
     }
     return c->fail(OSPH_ERR_STATE, "neighbour list did not fit after growing");
+}
+
+int osph_step_pc(osph_ctx* c, double dt, double* next_dt, int* crit) {
+    if (!c) return OSPH_ERR_ARG;
+    if (int e = step_begin(c, dt)) return e;
+    if (int e = step_middle(c, dt)) return e;
+    return step_end(c, dt, next_dt, crit);
+}
+
+int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* next_dt, int* crit) {
+    if (count < 0 || (count > 0 && (!ctxs || !dt))) return OSPH_ERR_ARG;
+    for (int k = 0; k < count; ++k) {
+        if (!ctxs[k]) return OSPH_ERR_ARG;
+        for (int j = 0; j < k; ++j) {
+            if (ctxs[j] == ctxs[k]) return ctxs[k]->fail(OSPH_ERR_ARG, "a context appears twice in the batch");
+        }
+    }
+    for (int k = 0; k < count; ++k) {
+        if (int e = step_begin(ctxs[k], dt[k])) return e;
+    }
+    for (int k = 0; k < count; ++k) {
+        if (int e = step_middle(ctxs[k], dt[k])) return e;
+    }
+    for (int k = 0; k < count; ++k) {
+        if (int e = step_end(ctxs[k], dt[k], next_dt ? next_dt + k : nullptr, crit ? crit + k : nullptr)) return e;
+    }
+    return OSPH_OK;
 }
 
 int osph_tune(osph_ctx* c, const char* key, long long value) {

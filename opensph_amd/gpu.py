@@ -28,7 +28,7 @@ PRECISION = {"fp64": 0, "mixed": 1}
 EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info", "osph_resize",
            "osph_set_quantity", "osph_get_quantity", "osph_zero_derivatives", "osph_build_neighbors",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
-           "osph_stream_copy", "osph_tune", "osph_integrals",
+           "osph_stream_copy", "osph_tune", "osph_integrals", "osph_step_pc_batch",
            "osph_profile_enable", "osph_profile_read"]
 
 
@@ -84,6 +84,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_stream_copy.argtypes = [sz, i, dp]
     L.osph_tune.argtypes = [vp, C.c_char_p, C.c_longlong]
     L.osph_integrals.argtypes = [vp, dp]
+    L.osph_step_pc_batch.argtypes = [C.POINTER(vp), i, dp, dp, C.POINTER(i)]
     L.osph_profile_enable.argtypes = [vp, i]
     L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
     _lib = L
@@ -313,3 +314,25 @@ def device_info() -> Tuple[str, int]:
     if rc != 0:
         raise GpuError("no HIP device")
     return buf.value.decode(), cu.value
+
+
+def step_batch(solvers) -> Tuple[np.ndarray, np.ndarray]:
+    """One predictor-corrector step of every solver in the list (osph_step_pc_batch), each with its own
+    current time step; returns the next time steps and criterion ids and advances each solver's dt."""
+    if not solvers:
+        return np.zeros(0), np.zeros(0, dtype=np.int32)
+    L = solvers[0].L
+    n = len(solvers)
+    ctxs = (C.c_void_p * n)(*[s.ctx.value for s in solvers])
+    dt = np.ascontiguousarray([s.dt for s in solvers], dtype=np.float64)
+    nxt = np.zeros(n)
+    crit = np.zeros(n, dtype=np.int32)
+    rc = L.osph_step_pc_batch(ctxs, n, dt.ctypes.data_as(C.POINTER(C.c_double)),
+                              nxt.ctypes.data_as(C.POINTER(C.c_double)), crit.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != 0:
+        msgs = "; ".join(L.osph_last_error(s.ctx).decode() for s in solvers if L.osph_last_error(s.ctx))
+        raise GpuError(f"osph_step_pc_batch failed ({rc}): {msgs}")
+    for s, d in zip(solvers, nxt):
+        s.dt = float(d)
+    return nxt, crit
+
