@@ -35,6 +35,13 @@ struct osph_ctx {
     DevState s{};
     double* dblock = nullptr;
     uint32_t* ublock = nullptr;
+    // the state is kept in cell order, renewed every tune.reorder_period searches into the second block
+    double* dblock2 = nullptr;
+    uint32_t* ublock2 = nullptr;
+    uint32_t* perm = nullptr;     // slot -> storage index
+    uint32_t* perm2 = nullptr;
+    uint32_t* slot_of = nullptr;  // storage index -> slot
+    int since_reorder = 0;
 
     // neighbour structures (all sized for cap particles)
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *iota = nullptr, *order = nullptr;
@@ -135,6 +142,14 @@ Launch L(osph_ctx* c) { return Launch{ c->stream, c->tune }; }
 void free_state(osph_ctx* c) {
     (void)hipFree(c->dblock);
     (void)hipFree(c->ublock);
+    (void)hipFree(c->dblock2);
+    (void)hipFree(c->ublock2);
+    (void)hipFree(c->perm);
+    (void)hipFree(c->perm2);
+    (void)hipFree(c->slot_of);
+    c->dblock2 = nullptr;
+    c->ublock2 = nullptr;
+    c->perm = c->perm2 = c->slot_of = nullptr;
     (void)hipFree(c->keys);
     (void)hipFree(c->keys_sorted);
     (void)hipFree(c->iota);
@@ -460,6 +475,22 @@ int search_finish(osph_ctx* c) {
         launch_clear_u32(l, c->cend, cells, 0u);
         launch_cell_ranges(l, c->keys_sorted, g, c->cstart, c->cend, n);
     });
+    if (c->tune.reorder_period > 0 && ++c->since_reorder >= c->tune.reorder_period) {
+        // move the state into this search's cell order: the record gather and the pair kernels' writes
+        // then run over (nearly) consecutive slots until the next reorder
+        timed(c, "reorder", [&] {
+            DevState dst = c->s;
+            for (int k = 0; k < NDPLANES; ++k) dst.d[k] = c->dblock2 + size_t(k) * c->cap;
+            for (int k = 0; k < NIPLANES; ++k) dst.u[k] = c->ublock2 + size_t(k) * c->cap;
+            launch_permute_state(l, c->s, dst, c->order, c->perm, c->perm2, c->slot_of, n);
+            launch_iota(l, c->order, n);
+            c->s = dst;
+        });
+        std::swap(c->dblock, c->dblock2);
+        std::swap(c->ublock, c->ublock2);
+        std::swap(c->perm, c->perm2);
+        c->since_reorder = 0;
+    }
     if (c->cfg.precision == OSPH_PRECISION_MIXED) {
         // lattice of power-of-two edge >= 2 h_min (10 bits per axis) for the FP32 relative positions
         Lattice lat{};
@@ -719,6 +750,11 @@ int osph_resize(osph_ctx* c, size_t n) {
         const size_t cap = n;
         HIPCHK(c, hipMalloc(&c->dblock, sizeof(double) * NDPLANES * cap));
         HIPCHK(c, hipMalloc(&c->ublock, sizeof(uint32_t) * NIPLANES * cap));
+        HIPCHK(c, hipMalloc(&c->dblock2, sizeof(double) * NDPLANES * cap));
+        HIPCHK(c, hipMalloc(&c->ublock2, sizeof(uint32_t) * NIPLANES * cap));
+        HIPCHK(c, hipMalloc(&c->perm, sizeof(uint32_t) * cap));
+        HIPCHK(c, hipMalloc(&c->perm2, sizeof(uint32_t) * cap));
+        HIPCHK(c, hipMalloc(&c->slot_of, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->keys, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->keys_sorted, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->iota, sizeof(uint32_t) * cap));
@@ -740,10 +776,13 @@ int osph_resize(osph_ctx* c, size_t n) {
     }
     c->n = n;
     if (c->cap) {
-        // every plane starts at zero (doubles: all-zero bits)
+        // every plane starts at zero (doubles: all-zero bits); slots start in storage order
         launch_clear_u32(L(c), reinterpret_cast<uint32_t*>(c->dblock), size_t(2) * NDPLANES * c->cap, 0u);
         launch_clear_u32(L(c), c->ublock, size_t(NIPLANES) * c->cap, 0u);
+        launch_iota(L(c), c->perm, c->cap);
+        launch_iota(L(c), c->slot_of, c->cap);
     }
+    c->since_reorder = c->tune.reorder_period;  // the first search moves the state into cell order
     return sync_check(c);
 }
 
@@ -769,7 +808,7 @@ int osph_set_quantity(osph_ctx* c, int q, int order, const void* host, size_t st
     HIPCHK(c, hipMemcpyAsync(c->staging, host, n * stride, hipMemcpyHostToDevice, c->stream));
     void* dst[8];
     plane_ptrs(c, d, dst);
-    launch_stage_to_planes(L(c), c->staging, stride, d.ncomp, d.off, d.elem, dst, n);
+    launch_stage_to_planes(L(c), c->staging, stride, d.ncomp, d.off, d.elem, dst, c->slot_of, n);
     c->neighbors_valid = false;
     c->href = 0.0;
     return sync_check(c);
@@ -788,7 +827,7 @@ int osph_get_quantity(osph_ctx* c, int q, int order, void* host, size_t stride) 
     if ((n * stride) % 4 == 0) launch_clear_u32(L(c), reinterpret_cast<uint32_t*>(c->staging), n * stride / 4, 0u);
     void* src[8];
     plane_ptrs(c, d, src);
-    launch_planes_to_stage(L(c), c->staging, stride, d.ncomp, d.off, d.elem, src, n);
+    launch_planes_to_stage(L(c), c->staging, stride, d.ncomp, d.off, d.elem, src, c->slot_of, n);
     HIPCHK(c, hipMemcpyAsync(host, c->staging, n * stride, hipMemcpyDeviceToHost, c->stream));
     return sync_check(c);
 }
@@ -809,10 +848,12 @@ int osph_get_pairs(osph_ctx* c, uint32_t* pi, uint32_t* pj, size_t cap, size_t* 
     if (!c || !count) return OSPH_ERR_ARG;
     if (!c->neighbors_valid) return c->fail(OSPH_ERR_STATE, "neighbours not built");
     const size_t n = c->n;
-    std::vector<uint32_t> cnt(n), order(n), list(n * size_t(c->kmax));
+    std::vector<uint32_t> cnt(n), order(n), perm(n), list(n * size_t(c->kmax));
     if (n) {
         HIPCHK(c, hipMemcpy(cnt.data(), c->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(order.data(), c->order, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(perm.data(), c->perm, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) order[k] = perm[order[k]];  // sorted position -> storage index
         HIPCHK(c, hipMemcpy(list.data(), c->list, list.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
     std::vector<uint64_t> keys;
@@ -976,6 +1017,8 @@ int osph_tune(osph_ctx* c, const char* key, long long value) {
         c->tune.cell_divisor = int(value);
     } else if (k == "lists_screen" && (value == -1 || value == 0 || value == 1)) {
         c->tune.lists_screen = int(value);
+    } else if (k == "reorder_period" && value >= 0 && value <= 1000000) {
+        c->tune.reorder_period = int(value);
     } else {
         return c->fail(OSPH_ERR_ARG, "unknown tuning key or value out of range: " + k);
     }

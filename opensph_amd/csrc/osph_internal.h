@@ -127,6 +127,7 @@ struct Tuning {
     int lists_lds = 0;
     int cell_divisor = 1;  // cells of 2 h_max(class) / divisor
     int lists_screen = -1; // FP32 screen in the list kernel: 1 on, 0 off, -1 on for mixed precision only
+    int reorder_period = 10;  // move the state into cell order every this many searches (0: never)
 };
 
 struct Launch {
@@ -176,10 +177,17 @@ void launch_finalize_correct(const Launch& l, DevState s, const osph_material* m
                              unsigned long long* mins, const uint32_t* overflow, size_t n);
 void launch_criteria(const Launch& l, DevState s, const osph_material* mats, Params prm,
                      unsigned long long* mins, size_t n);
+// Host element i <-> device slot slot_of[i].
 void launch_stage_to_planes(const Launch& l, const unsigned char* staging, size_t stride, int ncomp,
-                            const int* offsets_bytes, int elem_bytes, void* const* dst, size_t n);
+                            const int* offsets_bytes, int elem_bytes, void* const* dst, const uint32_t* slot_of,
+                            size_t n);
 void launch_planes_to_stage(const Launch& l, unsigned char* staging, size_t stride, int ncomp,
-                            const int* offsets_bytes, int elem_bytes, void* const* src, size_t n);
+                            const int* offsets_bytes, int elem_bytes, void* const* src, const uint32_t* slot_of,
+                            size_t n);
+// dst slot k <- src slot order[k] for every plane; perm_out[k] = perm_in[order[k]]; slot_of[perm_out[k]] = k.
+void launch_permute_state(const Launch& l, DevState src, DevState dst, const uint32_t* order, const uint32_t* perm_in,
+                          uint32_t* perm_out, uint32_t* slot_of, size_t n);
+void launch_iota(const Launch& l, uint32_t* p, size_t n);
 void launch_stream_copy(hipStream_t stream, const void* a, void* b, size_t bytes);
 // Integrals of the state into scratch (integrals_scratch_doubles(n) doubles); the 12 results are the
 // last 12 doubles.

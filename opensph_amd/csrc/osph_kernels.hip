@@ -1034,6 +1034,7 @@ struct StageDesc {
     int elem;
     int off[8];
     void* p[8];
+    const uint32_t* slot;  // storage index -> device slot
 };
 
 __global__ void __launch_bounds__(BLOCK) k_stage_to_planes(const unsigned char* __restrict__ st, size_t stride,
@@ -1041,11 +1042,12 @@ __global__ void __launch_bounds__(BLOCK) k_stage_to_planes(const unsigned char* 
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const unsigned char* e = st + i * stride;
+    const size_t k = d.slot[i];
     for (int c = 0; c < d.ncomp; ++c) {
         if (d.elem == 8) {
-            static_cast<double*>(d.p[c])[i] = *reinterpret_cast<const double*>(e + d.off[c]);
+            static_cast<double*>(d.p[c])[k] = *reinterpret_cast<const double*>(e + d.off[c]);
         } else {
-            static_cast<uint32_t*>(d.p[c])[i] = *reinterpret_cast<const uint32_t*>(e + d.off[c]);
+            static_cast<uint32_t*>(d.p[c])[k] = *reinterpret_cast<const uint32_t*>(e + d.off[c]);
         }
     }
 }
@@ -1055,13 +1057,37 @@ __global__ void __launch_bounds__(BLOCK) k_planes_to_stage(unsigned char* __rest
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned char* e = st + i * stride;
+    const size_t k = d.slot[i];
     for (int c = 0; c < d.ncomp; ++c) {
         if (d.elem == 8) {
-            *reinterpret_cast<double*>(e + d.off[c]) = static_cast<const double*>(d.p[c])[i];
+            *reinterpret_cast<double*>(e + d.off[c]) = static_cast<const double*>(d.p[c])[k];
         } else {
-            *reinterpret_cast<uint32_t*>(e + d.off[c]) = static_cast<const uint32_t*>(d.p[c])[i];
+            *reinterpret_cast<uint32_t*>(e + d.off[c]) = static_cast<const uint32_t*>(d.p[c])[k];
         }
     }
+}
+
+// Moves the state into cell order: slot k of dst receives slot order[k] of src, and the slot <-> storage
+// maps follow (perm_out[k] = storage index of slot k, slot_of[storage index] = k).
+__global__ void __launch_bounds__(BLOCK) k_permute_state(DevState src, DevState dst, const uint32_t* __restrict__ order,
+                                                         const uint32_t* __restrict__ perm_in,
+                                                         uint32_t* __restrict__ perm_out,
+                                                         uint32_t* __restrict__ slot_of, size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = order[k];
+#pragma unroll 4
+    for (int p = 0; p < NDPLANES; ++p) dst.d[p][k] = src.d[p][i];
+#pragma unroll
+    for (int p = 0; p < NIPLANES; ++p) dst.u[p][k] = src.u[p][i];
+    const uint32_t g = perm_in[i];
+    perm_out[k] = g;
+    slot_of[g] = uint32_t(k);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_iota_u32(uint32_t* p, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = uint32_t(i);
 }
 
 // STREAM-style copy: b[i] = a[i], 16 bytes per lane per iteration.
@@ -1263,17 +1289,32 @@ static StageDesc make_desc(int ncomp, const int* off, int elem, void* const* p) 
 }
 
 void launch_stage_to_planes(const Launch& l, const unsigned char* staging, size_t stride, int ncomp,
-                            const int* offsets_bytes, int elem_bytes, void* const* dst, size_t n) {
+                            const int* offsets_bytes, int elem_bytes, void* const* dst, const uint32_t* slot_of,
+                            size_t n) {
     if (!n) return;
-    k_stage_to_planes<<<grid_for(n), BLOCK, 0, l.stream>>>(staging, stride,
-                                                            make_desc(ncomp, offsets_bytes, elem_bytes, dst), n);
+    StageDesc d = make_desc(ncomp, offsets_bytes, elem_bytes, dst);
+    d.slot = slot_of;
+    k_stage_to_planes<<<grid_for(n), BLOCK, 0, l.stream>>>(staging, stride, d, n);
 }
 
 void launch_planes_to_stage(const Launch& l, unsigned char* staging, size_t stride, int ncomp,
-                            const int* offsets_bytes, int elem_bytes, void* const* src, size_t n) {
+                            const int* offsets_bytes, int elem_bytes, void* const* src, const uint32_t* slot_of,
+                            size_t n) {
     if (!n) return;
-    k_planes_to_stage<<<grid_for(n), BLOCK, 0, l.stream>>>(staging, stride,
-                                                            make_desc(ncomp, offsets_bytes, elem_bytes, src), n);
+    StageDesc d = make_desc(ncomp, offsets_bytes, elem_bytes, src);
+    d.slot = slot_of;
+    k_planes_to_stage<<<grid_for(n), BLOCK, 0, l.stream>>>(staging, stride, d, n);
+}
+
+void launch_permute_state(const Launch& l, DevState src, DevState dst, const uint32_t* order, const uint32_t* perm_in,
+                          uint32_t* perm_out, uint32_t* slot_of, size_t n) {
+    if (!n) return;
+    k_permute_state<<<grid_for(n), BLOCK, 0, l.stream>>>(src, dst, order, perm_in, perm_out, slot_of, n);
+}
+
+void launch_iota(const Launch& l, uint32_t* p, size_t n) {
+    if (!n) return;
+    k_iota_u32<<<grid_for(n), BLOCK, 0, l.stream>>>(p, n);
 }
 
 size_t sort_pairs_temp_bytes(size_t n) {
