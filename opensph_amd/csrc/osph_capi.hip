@@ -47,7 +47,7 @@ struct osph_ctx {
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *iota = nullptr, *order = nullptr;
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
-    uint32_t *cstart = nullptr, *cend = nullptr;
+    uint32_t* cell_first = nullptr;  // [cells + 1]: first sorted position of each cell
     size_t cell_cap = 0;
     PairRec* rec = nullptr;
     PairRecF* recf = nullptr;
@@ -194,14 +194,12 @@ int ensure_list(osph_ctx* c, size_t entries) {
 }
 
 int ensure_cells(osph_ctx* c, size_t cells) {
-    if (cells <= c->cell_cap) return OSPH_OK;
-    (void)hipFree(c->cstart);
-    (void)hipFree(c->cend);
-    c->cstart = c->cend = nullptr;
+    if (cells + 1 <= c->cell_cap) return OSPH_OK;
+    (void)hipFree(c->cell_first);
+    c->cell_first = nullptr;
     c->cell_cap = 0;
-    HIPCHK(c, hipMalloc(&c->cstart, cells * sizeof(uint32_t)));
-    HIPCHK(c, hipMalloc(&c->cend, cells * sizeof(uint32_t)));
-    c->cell_cap = cells;
+    HIPCHK(c, hipMalloc(&c->cell_first, (cells + 1) * sizeof(uint32_t)));
+    c->cell_cap = cells + 1;
     return OSPH_OK;
 }
 
@@ -470,11 +468,7 @@ int search_finish(osph_ctx* c) {
     timed(c, "sort", [&] {
         sort_pairs(l, c->sort_temp, c->sort_temp_bytes, c->keys, c->keys_sorted, c->iota, c->order, n, g.key_bits);
     });
-    timed(c, "cell_ranges", [&] {
-        launch_clear_u32(l, c->cstart, cells, 0u);
-        launch_clear_u32(l, c->cend, cells, 0u);
-        launch_cell_ranges(l, c->keys_sorted, g, c->cstart, c->cend, n);
-    });
+    timed(c, "cell_ranges", [&] { launch_cell_first(l, c->keys_sorted, g, c->cell_first, cells, n); });
     if (c->tune.reorder_period > 0 && ++c->since_reorder >= c->tune.reorder_period) {
         // move the state into this search's cell order: the record gather and the pair kernels' writes
         // then run over (nearly) consecutive slots until the next reorder
@@ -521,8 +515,8 @@ uint32_t* overflow_flag(osph_ctx* c) { return reinterpret_cast<uint32_t*>(c->sma
 int launch_lists(osph_ctx* c) {
     if (int e = ensure_list(c, c->n * size_t(c->kmax))) return e;
     timed(c, "neighbor_lists", [&] {
-        launch_neighbor_lists(L(c), c->pos, c->posf, c->screen, c->grid, c->cstart, c->cend, c->list, c->count,
-                              c->kmax, overflow_flag(c), c->n);
+        launch_neighbor_lists(L(c), c->pos, c->posf, c->screen, c->grid, c->cell_first, c->list, c->count, c->kmax,
+                              overflow_flag(c), c->n);
     });
     return OSPH_OK;
 }
@@ -721,8 +715,7 @@ void osph_destroy(osph_ctx* c) {
     resolve_events(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     free_state(c);
-    (void)hipFree(c->cstart);
-    (void)hipFree(c->cend);
+    (void)hipFree(c->cell_first);
     (void)hipFree(c->list);
     (void)hipFree(c->staging);
     (void)hipFree(c->scratch);

@@ -116,6 +116,22 @@ __host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) 
     return morton_spread(x) | (morton_spread(y) << 1) | (morton_spread(z) << 2);
 }
 
+__host__ __device__ inline uint32_t spread2(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// Cell key of a neighbour grid with 2^bits cells per axis: the rows of cells along x are kept whole and
+// ordered by the Morton code of (y, z), so that every x-run of cells is one contiguous range of sorted
+// particles.
+__host__ __device__ inline uint32_t row_key(uint32_t x, uint32_t y, uint32_t z, int bits) {
+    return ((spread2(y) | (spread2(z) << 1)) << bits) | x;
+}
+
 // ---- launchers implemented in osph_kernels.hip -------------------------------------------------------
 // Launch shapes of the pair-stage kernels, chosen by the tuner (osph_tune): threads per block and
 // dynamic LDS per block (which caps the number of resident blocks per CU, and with it the size of the
@@ -156,13 +172,14 @@ void launch_advance(const Launch& l, DevState s, const osph_material* mats, Para
                     double href, unsigned long long* acc, size_t n);
 void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n);
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value);
-void launch_cell_ranges(const Launch& l, const uint32_t* keys, Grid g, uint32_t* cstart, uint32_t* cend, size_t n);
+// first[c] = the first sorted position whose cell index is >= c, for c in [0, cells] (first[cells] = n).
+void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* first, size_t cells, size_t n);
 void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, double4* pos,
                  float4* posf, Screen sc, uint32_t* group, size_t n);
 // Neighbour lists are row-major: list[k * kmax + c], kmax a multiple of 4.
 void launch_neighbor_lists(const Launch& l, const double4* pos, const float4* posf, Screen sc, Grid g,
-                           const uint32_t* cstart, const uint32_t* cend, uint32_t* list, uint32_t* count, uint32_t kmax,
-                           uint32_t* overflow, size_t n);
+                           const uint32_t* first, uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow,
+                           size_t n);
 void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
                        uint32_t kmax, size_t n);

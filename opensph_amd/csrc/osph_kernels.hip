@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_keys(DevState s, Grid g, uint32_
     const int cls = h_class(s.d[H][i], g.hmin, g.nclass);
     int c[3];
     cell_of(g.c[cls], s.d[X][i], s.d[Y][i], s.d[Z][i], c);
-    keys[i] = (uint32_t(cls) << g.shift) | morton3(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
+    keys[i] = (uint32_t(cls) << g.shift) | row_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]), g.c[cls].bits);
     idx[i] = uint32_t(i);
 }
 
@@ -313,14 +313,25 @@ __global__ void __launch_bounds__(BLOCK) k_fill_u32(uint32_t* p, size_t count, u
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cell_ranges(const uint32_t* __restrict__ keys, Grid g, uint32_t* cstart,
-                                                       uint32_t* cend, size_t n) {
-    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t key = keys[k];
-    const uint32_t cell = g.c[key >> g.shift].base + (key & ((1u << g.shift) - 1u));
-    if (k == 0 || keys[k - 1] != key) cstart[cell] = uint32_t(k);
-    if (k == n - 1 || keys[k + 1] != key) cend[cell] = uint32_t(k + 1);
+// Cell index of sorted position k (monotonic in k: the class bases and row keys increase with the key).
+__device__ __forceinline__ uint32_t cell_index(const Grid& g, uint32_t key) {
+    return g.c[key >> g.shift].base + (key & ((1u << g.shift) - 1u));
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cell_first(const uint32_t* __restrict__ keys, Grid g,
+                                                      uint32_t* __restrict__ first, size_t cells, size_t n) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c > cells) return;
+    size_t lo = 0, hi = n;  // the first k in [lo, hi] with cell_index(k) >= c
+    while (lo < hi) {
+        const size_t mid = (lo + hi) >> 1;
+        if (cell_index(g, keys[mid]) < c) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    first[c] = uint32_t(lo);
 }
 
 // Pack the neighbour-side data of particle order[k] into rec[k].
@@ -395,16 +406,15 @@ __device__ __forceinline__ bool exact_pair(double xi, double yi, double zi, doub
 template <bool SCREEN>
 __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restrict__ pos,
                                                           const float4* __restrict__ posf, Screen sc, Grid g,
-                                                          const uint32_t* __restrict__ cstart,
-                                                          const uint32_t* __restrict__ cend, uint32_t* __restrict__ list,
-                                                          uint32_t* __restrict__ count, uint32_t kmax,
-                                                          uint32_t* overflow, size_t n) {
+                                                          const uint32_t* __restrict__ first,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                                                          uint32_t kmax, uint32_t* overflow, size_t n) {
     const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double4 me = pos[k];
     const float4 mf = SCREEN ? posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     const double p[3] = { me.x, me.y, me.z };
-    uint4* row = reinterpret_cast<uint4*>(list + size_t(k) * kmax);
+    uint4* row4 = reinterpret_cast<uint4*>(list + size_t(k) * kmax);
     uint32_t buf[4] = { 0, 0, 0, 0 };
     uint32_t cnt = 0;
     for (int cls = 0; cls < g.nclass; ++cls) {
@@ -424,7 +434,9 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
             lo[a] = c - r < 0 ? 0 : c - r;
             up[a] = c + r >= G.dim[a] ? G.dim[a] - 1 : c + r;
         }
-        // cells whose box lies farther than R (with a relative margin) hold no neighbour
+        // rows of cells whose yz-rectangle lies farther than R (with a relative margin) hold no neighbour;
+        // in the others, a neighbour lies within the chord half-width w = sqrt(R^2 - d_yz^2) of the
+        // particle along x, and the cells of that x-run are one contiguous range of sorted particles
         const double R2 = R * R * (1.0 + 1e-6);
         for (int cz = lo[2]; cz <= up[2]; ++cz) {
             const double tz = dmax(dmax(G.lo[2] + cz * G.cell - p[2], p[2] - (G.lo[2] + (cz + 1) * G.cell)), 0.0);
@@ -432,35 +444,36 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
                 const double ty = dmax(dmax(G.lo[1] + cy * G.cell - p[1], p[1] - (G.lo[1] + (cy + 1) * G.cell)), 0.0);
                 const double dzy = tz * tz + ty * ty;
                 if (dzy > R2) continue;
-                for (int cx = lo[0]; cx <= up[0]; ++cx) {
-                    const double tx =
-                        dmax(dmax(G.lo[0] + cx * G.cell - p[0], p[0] - (G.lo[0] + (cx + 1) * G.cell)), 0.0);
-                    if (dzy + tx * tx > R2) continue;
-                    const uint32_t cell = G.base + morton3(uint32_t(cx), uint32_t(cy), uint32_t(cz));
-                    const uint32_t b = cstart[cell], e = cend[cell];
-                    for (uint32_t k2 = b; k2 < e; ++k2) {
-                        bool hit;
-                        if (SCREEN) {
-                            const float4 o = posf[k2];
-                            const float dx = mf.x - o.x, dy = mf.y - o.y, dz = mf.z - o.z;
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            const float hs = mf.w + o.w;
-                            const float rr = hs * hs;
-                            hit = d2 < rr * sc.lo;
-                            if (!hit && !(d2 > rr * sc.hi)) {
-                                const double4 od = pos[k2];
-                                hit = exact_pair(me.x, me.y, me.z, me.w, od.x, od.y, od.z, od.w);
-                            }
-                        } else {
-                            const double4 o = pos[k2];
-                            hit = exact_pair(me.x, me.y, me.z, me.w, o.x, o.y, o.z, o.w);
+                const double w = sqrt(R2 - dzy);
+                int x0 = int(floor((p[0] - w - G.lo[0]) * G.inv_cell));
+                int x1 = int(floor((p[0] + w - G.lo[0]) * G.inv_cell));
+                x0 = x0 < lo[0] ? lo[0] : x0;
+                x1 = x1 > up[0] ? up[0] : x1;
+                if (x0 > x1) continue;
+                const uint32_t row = G.base + row_key(0u, uint32_t(cy), uint32_t(cz), G.bits);
+                const uint32_t b = first[row + uint32_t(x0)], e = first[row + uint32_t(x1) + 1u];
+                for (uint32_t k2 = b; k2 < e; ++k2) {
+                    bool hit;
+                    if (SCREEN) {
+                        const float4 o = posf[k2];
+                        const float dx = mf.x - o.x, dy = mf.y - o.y, dz = mf.z - o.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const float hs = mf.w + o.w;
+                        const float rr = hs * hs;
+                        hit = d2 < rr * sc.lo;
+                        if (!hit && !(d2 > rr * sc.hi)) {
+                            const double4 od = pos[k2];
+                            hit = exact_pair(me.x, me.y, me.z, me.w, od.x, od.y, od.z, od.w);
                         }
-                        if (hit && k2 != k) {
-                            buf[cnt & 3u] = k2;
-                            ++cnt;
-                            if ((cnt & 3u) == 0 && cnt <= kmax) {
-                                row[(cnt >> 2) - 1] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
-                            }
+                    } else {
+                        const double4 o = pos[k2];
+                        hit = exact_pair(me.x, me.y, me.z, me.w, o.x, o.y, o.z, o.w);
+                    }
+                    if (hit && k2 != k) {
+                        buf[cnt & 3u] = k2;
+                        ++cnt;
+                        if ((cnt & 3u) == 0 && cnt <= kmax) {
+                            row4[(cnt >> 2) - 1] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
                         }
                     }
                 }
@@ -468,7 +481,7 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
         }
     }
     if ((cnt & 3u) != 0 && cnt < kmax) {
-        row[cnt >> 2] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+        row4[cnt >> 2] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
     }
     count[k] = cnt;
     if (cnt > kmax) atomicOr(overflow, 1u);
@@ -1193,9 +1206,8 @@ void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value
     k_fill_u32<<<g, BLOCK, 0, l.stream>>>(p, count, value);
 }
 
-void launch_cell_ranges(const Launch& l, const uint32_t* keys, Grid g, uint32_t* cstart, uint32_t* cend, size_t n) {
-    if (!n) return;
-    k_cell_ranges<<<grid_for(n), BLOCK, 0, l.stream>>>(keys, g, cstart, cend, n);
+void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* first, size_t cells, size_t n) {
+    k_cell_first<<<grid_for(cells + 1), BLOCK, 0, l.stream>>>(keys, g, first, cells, n);
 }
 
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
@@ -1219,16 +1231,16 @@ void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order,
 }
 
 void launch_neighbor_lists(const Launch& l, const double4* pos, const float4* posf, Screen sc, Grid g,
-                           const uint32_t* cstart, const uint32_t* cend, uint32_t* list, uint32_t* count, uint32_t kmax,
-                           uint32_t* overflow, size_t n) {
+                           const uint32_t* first, uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow,
+                           size_t n) {
     if (!n) return;
     const int b = l.tune.lists_block;
     if (sc.on) {
         k_neighbor_lists<true><<<grid_for(n, b), b, size_t(l.tune.lists_lds), l.stream>>>(
-            pos, posf, sc, g, cstart, cend, list, count, kmax, overflow, n);
+            pos, posf, sc, g, first, list, count, kmax, overflow, n);
     } else {
         k_neighbor_lists<false><<<grid_for(n, b), b, size_t(l.tune.lists_lds), l.stream>>>(
-            pos, posf, sc, g, cstart, cend, list, count, kmax, overflow, n);
+            pos, posf, sc, g, first, list, count, kmax, overflow, n);
     }
 }
 
