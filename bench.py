@@ -29,6 +29,11 @@ FP32_PEAK_TFLOPS = 157.3
 # symmetric form (LUT gradient 18, pressure 24, velocity divergence 16, artificial viscosity 60,
 # velocity gradient 38, stress divergence 42), without the pair test (done by the list kernel).
 FLOP_PER_PAIR = 198.0
+# Barnes-Hut evaluation (FP64): a particle pair with the Newtonian branch (difference 3, |r|^2 5, radius
+# test 4, 1/sqrt 1, cube 2, mass 1, accumulate 6) and a quadrupole expansion at one particle (difference 3,
+# |r|^2 5, 1/sqrt and powers 8, q2 d 15, d q2 d 6, terms 15, monopole 9); counted as in the pair figure.
+FLOP_PER_GRAVITY_PAIR = 22.0
+FLOP_PER_QUADRUPOLE = 61.0
 
 
 def log(*a):
@@ -170,7 +175,12 @@ def run_nbody(args):
                         "rel_err_max": float(rel.max()),
                         "err_over_rms_max": float(np.max(np.linalg.norm(bh - direct, axis=1)) / rms)},
            "interactions": {"nodes_approx": int(st["nodes_approx"]), "leaves_exact": int(st["nodes_exact"]),
-                            "expansions": int(st["expansions"]), "pairs": int(st["pairs"]), "eval_ms": eval_ms}}
+                            "expansions": int(st["expansions"]), "pairs": int(st["pairs"]), "eval_ms": eval_ms},
+           "roofline": {"bound": "mfma", "kernel": "eval",
+                        "achieved": (FLOP_PER_GRAVITY_PAIR * st["pairs"] + FLOP_PER_QUADRUPOLE * st["expansions"])
+                        / (eval_ms * 1e-3) / 1e12,
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "traffic": None}}
+    out["roofline"]["frac"] = out["roofline"]["achieved"] / FP64_PEAK_TFLOPS
     print(json.dumps(out), flush=True)
 
 

@@ -47,7 +47,9 @@ struct osph_ctx {
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *iota = nullptr, *order = nullptr;
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
-    uint32_t* cell_first = nullptr;  // [cells + 1]: first sorted position of each cell
+    uint32_t* cell_first = nullptr;  // [cells + 1]: first sorted position of each cell (valid rows only)
+    uint32_t* row_range = nullptr;   // [2 rows]: sorted range of each row of cells
+    size_t row_cap = 0;
     size_t cell_cap = 0;
     PairRec* rec = nullptr;
     PairRecF* recf = nullptr;
@@ -193,9 +195,20 @@ int ensure_list(osph_ctx* c, size_t entries) {
     return OSPH_OK;
 }
 
+int ensure_rows(osph_ctx* c, size_t rows) {
+    if (rows <= c->row_cap) return OSPH_OK;
+    (void)hipFree(c->row_range);
+    c->row_range = nullptr;
+    c->row_cap = 0;
+    HIPCHK(c, hipMalloc(&c->row_range, 2 * rows * sizeof(uint32_t)));
+    c->row_cap = rows;
+    return OSPH_OK;
+}
+
 int ensure_cells(osph_ctx* c, size_t cells) {
     if (cells + 1 <= c->cell_cap) return OSPH_OK;
     (void)hipFree(c->cell_first);
+    (void)hipFree(c->row_range);
     c->cell_first = nullptr;
     c->cell_cap = 0;
     HIPCHK(c, hipMalloc(&c->cell_first, (cells + 1) * sizeof(uint32_t)));
@@ -450,8 +463,9 @@ int search_finish(osph_ctx* c) {
             G.dim[k] = int(std::floor((G.hi[k] - G.lo[k]) / G.cell)) + 1;
             maxdim = std::max(maxdim, G.dim[k]);
         }
+        // 2^bits > dim: every row keeps a slot past its last cell for the end of an x-run
         int bits = 1;
-        while ((1 << bits) < maxdim) ++bits;
+        while ((1 << bits) <= maxdim) ++bits;
         G.bits = bits;
         base += uint32_t(1) << (3 * bits);
     }
@@ -468,7 +482,14 @@ int search_finish(osph_ctx* c) {
     timed(c, "sort", [&] {
         sort_pairs(l, c->sort_temp, c->sort_temp_bytes, c->keys, c->keys_sorted, c->iota, c->order, n, g.key_bits);
     });
-    timed(c, "cell_ranges", [&] { launch_cell_first(l, c->keys_sorted, g, c->cell_first, cells, n); });
+    size_t rows = 0;
+    for (int cl = 0; cl < g.nclass; ++cl) {
+        if (!g.c[cl].empty) rows += size_t(g.c[cl].dim[1]) * size_t(g.c[cl].dim[2]);
+    }
+    if (int e = ensure_rows(c, rows)) return e;
+    timed(c, "cell_ranges", [&] {
+        launch_cell_first(l, c->keys_sorted, g, c->row_range, c->cell_first, rows, n);
+    });
     if (c->tune.reorder_period > 0 && ++c->since_reorder >= c->tune.reorder_period) {
         // move the state into this search's cell order: the record gather and the pair kernels' writes
         // then run over (nearly) consecutive slots until the next reorder
@@ -716,6 +737,7 @@ void osph_destroy(osph_ctx* c) {
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     free_state(c);
     (void)hipFree(c->cell_first);
+    (void)hipFree(c->row_range);
     (void)hipFree(c->list);
     (void)hipFree(c->staging);
     (void)hipFree(c->scratch);
@@ -1006,7 +1028,7 @@ int osph_tune(osph_ctx* c, const char* key, long long value) {
         c->tune.pairs_lds = int(value);
     } else if (k == "lists_lds" && lds_ok) {
         c->tune.lists_lds = int(value);
-    } else if (k == "cell_divisor" && (value == 1 || value == 2 || value == 3)) {
+    } else if (k == "cell_divisor" && value >= 1 && value <= 4) {
         c->tune.cell_divisor = int(value);
     } else if (k == "lists_screen" && (value == -1 || value == 0 || value == 1)) {
         c->tune.lists_screen = int(value);

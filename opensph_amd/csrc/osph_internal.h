@@ -141,7 +141,7 @@ struct Tuning {
     int pairs_lds = 0;
     int lists_block = 256;
     int lists_lds = 0;
-    int cell_divisor = 1;  // cells of 2 h_max(class) / divisor
+    int cell_divisor = 3;  // cells of 2 h_max(class) / divisor
     int lists_screen = -1; // FP32 screen in the list kernel: 1 on, 0 off, -1 on for mixed precision only
     int reorder_period = 10;  // move the state into cell order every this many searches (0: never)
 };
@@ -172,8 +172,10 @@ void launch_advance(const Launch& l, DevState s, const osph_material* mats, Para
                     double href, unsigned long long* acc, size_t n);
 void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n);
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value);
-// first[c] = the first sorted position whose cell index is >= c, for c in [0, cells] (first[cells] = n).
-void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* first, size_t cells, size_t n);
+// first[cell] = the first sorted position whose cell index is >= cell, for every cell (x, y, z) of a
+// row (y < dim_y, z < dim_z) with x in [0, dim_x]; row_range (2 per row) is scratch.
+void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* row_range, uint32_t* first,
+                       size_t rows, size_t n);
 void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, double4* pos,
                  float4* posf, Screen sc, uint32_t* group, size_t n);
 // Neighbour lists are row-major: list[k * kmax + c], kmax a multiple of 4.

@@ -318,20 +318,65 @@ __device__ __forceinline__ uint32_t cell_index(const Grid& g, uint32_t key) {
     return g.c[key >> g.shift].base + (key & ((1u << g.shift) - 1u));
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cell_first(const uint32_t* __restrict__ keys, Grid g,
-                                                      uint32_t* __restrict__ first, size_t cells, size_t n) {
-    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c > cells) return;
-    size_t lo = 0, hi = n;  // the first k in [lo, hi] with cell_index(k) >= c
+// First sorted position in [lo, hi) whose cell index is >= c (hi when none is).
+__device__ __forceinline__ uint32_t first_at_least(const uint32_t* __restrict__ keys, const Grid& g, uint32_t c,
+                                                   uint32_t lo, uint32_t hi) {
     while (lo < hi) {
-        const size_t mid = (lo + hi) >> 1;
+        const uint32_t mid = lo + ((hi - lo) >> 1);
         if (cell_index(g, keys[mid]) < c) {
             lo = mid + 1;
         } else {
             hi = mid;
         }
     }
-    first[c] = uint32_t(lo);
+    return lo;
+}
+
+// Row q of the grid (rows of all non-empty classes numbered in turn): its class, y and z.
+__device__ __forceinline__ bool row_of(const Grid& g, size_t q, int& cls, uint32_t& cy, uint32_t& cz) {
+    for (int c = 0; c < g.nclass; ++c) {
+        const GridClass& G = g.c[c];
+        if (G.empty) continue;
+        const size_t nr = size_t(G.dim[1]) * size_t(G.dim[2]);
+        if (q < nr) {
+            cls = c;
+            cy = uint32_t(q % size_t(G.dim[1]));
+            cz = uint32_t(q / size_t(G.dim[1]));
+            return true;
+        }
+        q -= nr;
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_row_range(const uint32_t* __restrict__ keys, Grid g,
+                                                     uint32_t* __restrict__ range, size_t rows, uint32_t n) {
+    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= rows) return;
+    int cls;
+    uint32_t cy, cz;
+    if (!row_of(g, q, cls, cy, cz)) return;
+    const GridClass& G = g.c[cls];
+    const uint32_t row = G.base + row_key(0u, cy, cz, G.bits);
+    range[2 * q] = first_at_least(keys, g, row, 0u, n);
+    range[2 * q + 1] = first_at_least(keys, g, row + (1u << G.bits), 0u, n);
+}
+
+// One thread per cell x in [0, dim_x] of a row: the binary search runs inside the row's own range.
+__global__ void __launch_bounds__(BLOCK) k_cell_first(const uint32_t* __restrict__ keys, Grid g,
+                                                      const uint32_t* __restrict__ range, uint32_t* __restrict__ first,
+                                                      size_t rows, int xslots) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t q = t / size_t(xslots);
+    const uint32_t x = uint32_t(t % size_t(xslots));
+    if (q >= rows) return;
+    int cls;
+    uint32_t cy, cz;
+    if (!row_of(g, q, cls, cy, cz)) return;
+    const GridClass& G = g.c[cls];
+    if (x > uint32_t(G.dim[0])) return;
+    const uint32_t cell = G.base + row_key(x, cy, cz, G.bits);
+    first[cell] = first_at_least(keys, g, cell, range[2 * q], range[2 * q + 1]);
 }
 
 // Pack the neighbour-side data of particle order[k] into rec[k].
@@ -1206,8 +1251,15 @@ void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value
     k_fill_u32<<<g, BLOCK, 0, l.stream>>>(p, count, value);
 }
 
-void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* first, size_t cells, size_t n) {
-    k_cell_first<<<grid_for(cells + 1), BLOCK, 0, l.stream>>>(keys, g, first, cells, n);
+void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* row_range, uint32_t* first,
+                       size_t rows, size_t n) {
+    if (!rows) return;
+    int xslots = 1;
+    for (int c = 0; c < g.nclass; ++c) {
+        if (!g.c[c].empty) xslots = xslots > g.c[c].dim[0] + 1 ? xslots : g.c[c].dim[0] + 1;
+    }
+    k_row_range<<<grid_for(rows), BLOCK, 0, l.stream>>>(keys, g, row_range, rows, uint32_t(n));
+    k_cell_first<<<grid_for(rows * size_t(xslots)), BLOCK, 0, l.stream>>>(keys, g, row_range, first, rows, xslots);
 }
 
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,

@@ -47,7 +47,7 @@ def test_pairs_bit_exact(oracle, name):
     assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
 
 
-@pytest.mark.parametrize("divisor", [1, 2, 3])
+@pytest.mark.parametrize("divisor", [1, 2, 3, 4])
 def test_pairs_mixed_smoothing_lengths(oracle, divisor):
     """Four smoothing-length classes (h spans a factor 16): every class grid and cross-class search, with
     neighbour cells of 2 h_max / divisor."""
