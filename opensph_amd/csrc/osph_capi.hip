@@ -392,10 +392,12 @@ int search_begin(osph_ctx* c, int mode, double dt) {
     }
     const double href = c->href;
     unsigned long long* cacc = c->small + SMALL_CLASSES;
+    unsigned long long* vacc = c->cfg.precision == OSPH_PRECISION_MIXED ? c->small + SMALL_VEL : nullptr;
     timed(c, mode == 2 ? "advance" : (mode == 1 ? "material_init" : "class_bounds"),
-          [&] { launch_advance(l, c->s, c->mats_dev, c->prm, mode, dt, href, cacc, n); });
-    HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CLASSES, cacc, CLASS_STRIDE * MAX_CLASSES * sizeof(unsigned long long),
-                             hipMemcpyDeviceToHost, c->stream));
+          [&] { launch_advance(l, c->s, c->mats_dev, c->prm, mode, dt, href, cacc, vacc, n); });
+    HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CLASSES, cacc,
+                             (SMALL_VEL + 6 - SMALL_CLASSES) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
     return OSPH_OK;
 }
 
@@ -516,6 +518,22 @@ int search_finish(osph_ctx* c) {
         for (int k = 0; k < 3; ++k) lat.lo[k] = lo[k];
         lat.S = S;
         lat.Sf = float(S);
+        // velocity lattice: power-of-two edge with the velocity range within 1000 cells
+        double vrange = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            lat.vlo[k] = decode_ord(c->small_host[SMALL_VEL + k]);
+            const double vhi = decode_ord(c->small_host[SMALL_VEL + 3 + k]);
+            if (!std::isfinite(lat.vlo[k]) || !std::isfinite(vhi)) {
+                return c->fail(OSPH_ERR_STATE, "velocities must be finite");
+            }
+            vrange = std::max(vrange, vhi - lat.vlo[k]);
+        }
+        double Sv = 1.0;
+        if (vrange > 0.0) Sv = std::exp2(std::ceil(std::log2(vrange / 1000.0)));
+        while (vrange / Sv > 1000.0) Sv *= 2.0;
+        Sv = std::max(Sv, std::exp2(-100.0));  // keeps the FP32 copy of the edge normal
+        lat.Sv = Sv;
+        lat.Svf = float(Sv);
         c->lattice = lat;
         timed(c, "pack", [&] {
             launch_pack_mixed(l, c->s, c->prm, lat, c->order, c->recf, c->pos, c->posf, sc, c->group, n);
@@ -1028,6 +1046,8 @@ int osph_tune(osph_ctx* c, const char* key, long long value) {
         c->tune.pairs_lds = int(value);
     } else if (k == "lists_lds" && lds_ok) {
         c->tune.lists_lds = int(value);
+    } else if (k == "pairs_lanes" && (value == 1 || value == 2 || value == 4)) {
+        c->tune.pairs_lanes = int(value);
     } else if (k == "cell_divisor" && value >= 1 && value <= 4) {
         c->tune.cell_divisor = int(value);
     } else if (k == "lists_screen" && (value == -1 || value == 0 || value == 1)) {

@@ -47,13 +47,13 @@ static_assert(sizeof(PairRec) == 128, "PairRec must be 128 bytes");
 // r_i - r_j = (L_i - L_j) S + (d_i - d_j) keeps FP32 relative precision at any distance from the origin.
 // Velocities stay FP64: bodies move at km/s with m/s internal differences, which FP32 would round away.
 struct alignas(16) PairRecF {
-    double vx, vy, vz;
-    float dx, dy, dz, h;
-    float m, rho, cs, pr;
-    float sxx, syy, sxy, sxz, syz;
-    uint32_t lat, group, pad0, pad1, pad2;
+    float dx, dy, dz, h;     // position relative to its lattice cell, smoothing length
+    float ux, uy, uz, m;     // velocity relative to its velocity-lattice cell, mass
+    float rho, cs, pr, sxx;  // pr = p / rho^2, s.. = S / rho^2
+    float syy, sxy, sxz, syz;
+    uint32_t lat, vlat, group, pad;  // lattice cells (10 bits per axis) of the position and the velocity
 };
-static_assert(sizeof(PairRecF) == 96, "PairRecF must be 96 bytes");
+static_assert(sizeof(PairRecF) == 80, "PairRecF must be 80 bytes");
 
 // FP32 screen of the neighbour test: positions relative to the origin o, and the band [lo, hi] = [1 - b,
 // 1 + b] around the squared radius outside which the FP32 comparison equals the exact FP64 one.
@@ -63,10 +63,15 @@ struct Screen {
     int on;
 };
 
+// Power-of-two lattices of the mixed-precision records: positions on cells of edge S from lo, velocities
+// on cells of edge Sv from vlo (10 bits per axis each); the FP32 parts are offsets inside the cell.
 struct Lattice {
     double lo[3];
     double S;     // power of two
     float Sf;
+    double vlo[3];
+    double Sv;    // power of two
+    float Svf;
 };
 
 // Particles are binned into smoothing-length classes, class = min(nclass - 1, floor(log2(h / h_min))).
@@ -139,6 +144,7 @@ __host__ __device__ inline uint32_t row_key(uint32_t x, uint32_t y, uint32_t z, 
 struct Tuning {
     int pairs_block = 256;
     int pairs_lds = 0;
+    int pairs_lanes = 2;  // lanes per particle in the pair kernels (1, 2 or 4)
     int lists_block = 256;
     int lists_lds = 0;
     int cell_divisor = 3;  // cells of 2 h_max(class) / divisor
@@ -153,23 +159,27 @@ struct Launch {
 
 // Accumulators of one step (64 words of device memory): SMALL_BOUNDS + 0..7 global bounds,
 // SMALL_CRITERIA + 0..3 the criterion minima, SMALL_OVERFLOW the list overflow flag (low 32 bits),
-// SMALL_CLASSES + CLASS_STRIDE * c + 0..8 the bounds of class c.
+// SMALL_CLASSES + CLASS_STRIDE * c + 0..8 the bounds of class c, SMALL_VEL + 0..5 the velocity bounds
+// (min x, y, z, max x, y, z; mixed precision only).
 constexpr int SMALL_BOUNDS = 0;
 constexpr int SMALL_CRITERIA = 8;
 constexpr int SMALL_OVERFLOW = 12;
 constexpr int SMALL_CLASSES = 16;
 constexpr int CLASS_STRIDE = 10;
+constexpr int SMALL_VEL = 56;
 constexpr int SMALL_WORDS = 64;
-static_assert(SMALL_CLASSES + CLASS_STRIDE * MAX_CLASSES <= SMALL_WORDS, "accumulator layout");
+static_assert(SMALL_CLASSES + CLASS_STRIDE * MAX_CLASSES <= SMALL_VEL, "accumulator layout");
+static_assert(SMALL_VEL + 6 <= SMALL_WORDS, "accumulator layout");
 
 // Clears every accumulator (min slots to all ones, max and count slots to zero) in one launch.
 void launch_init_small(const Launch& l, unsigned long long* small);
 // acc[0..2] = min x, y, z; acc[3..5] = max x, y, z; acc[6] = max h; acc[7] = min h (order-encoded).
 void launch_bounds_reduce(const Launch& l, DevState s, unsigned long long* acc, size_t n);
 // mode 2: predictions + material initialize; 1: material initialize; 0: neither.  Then the per-class
-// grid bounds (class of h relative to href) into acc[CLASS_STRIDE * c + 0..8].
+// grid bounds (class of h relative to href) into acc[CLASS_STRIDE * c + 0..8] and, when vel is not null,
+// the velocity bounds into vel[0..5].
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
-                    double href, unsigned long long* acc, size_t n);
+                    double href, unsigned long long* acc, unsigned long long* vel, size_t n);
 void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n);
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value);
 // first[cell] = the first sorted position whose cell index is >= cell, for every cell (x, y, z) of a

@@ -174,6 +174,8 @@ __global__ void __launch_bounds__(SMALL_WORDS) k_init_small(unsigned long long* 
         is_min = k < 3 || k == 7;
     } else if (t >= SMALL_CRITERIA && t < SMALL_CRITERIA + 4) {
         is_min = true;
+    } else if (t >= SMALL_VEL && t < SMALL_VEL + 6) {
+        is_min = t < SMALL_VEL + 3;
     } else {
         is_min = t < SMALL_BOUNDS + 3 || t == SMALL_BOUNDS + 7;
     }
@@ -448,6 +450,8 @@ __device__ __forceinline__ bool exact_pair(double xi, double yi, double zi, doub
 // instead of 32).  The FP32 squared distance and radius carry a relative error below sc.band (bounded on
 // the host from the domain extent and h_min), so d2 < R2 (1 - band) and d2 > R2 (1 + band) decide the
 // exact test; only candidates inside the band are re-tested in FP64, and the pair set is unchanged.
+constexpr uint32_t LIST_BATCH = 4;
+
 template <bool SCREEN>
 __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restrict__ pos,
                                                           const float4* __restrict__ posf, Screen sc, Grid g,
@@ -462,6 +466,11 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
     uint4* row4 = reinterpret_cast<uint4*>(list + size_t(k) * kmax);
     uint32_t buf[4] = { 0, 0, 0, 0 };
     uint32_t cnt = 0;
+    auto append = [&](uint32_t k2) {
+        buf[cnt & 3u] = k2;
+        ++cnt;
+        if ((cnt & 3u) == 0 && cnt <= kmax) row4[(cnt >> 2) - 1] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+    };
     for (int cls = 0; cls < g.nclass; ++cls) {
         const GridClass& G = g.c[cls];
         if (G.empty) continue;
@@ -497,28 +506,38 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
                 if (x0 > x1) continue;
                 const uint32_t row = G.base + row_key(0u, uint32_t(cy), uint32_t(cz), G.bits);
                 const uint32_t b = first[row + uint32_t(x0)], e = first[row + uint32_t(x1) + 1u];
-                for (uint32_t k2 = b; k2 < e; ++k2) {
-                    bool hit;
+                // candidates in batches of LIST_BATCH: all loads of a batch are issued before the first
+                // test (clamped to the last candidate, so every load stays inside [b, e))
+                for (uint32_t k0 = b; k0 < e; k0 += LIST_BATCH) {
                     if (SCREEN) {
-                        const float4 o = posf[k2];
-                        const float dx = mf.x - o.x, dy = mf.y - o.y, dz = mf.z - o.z;
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        const float hs = mf.w + o.w;
-                        const float rr = hs * hs;
-                        hit = d2 < rr * sc.lo;
-                        if (!hit && !(d2 > rr * sc.hi)) {
-                            const double4 od = pos[k2];
-                            hit = exact_pair(me.x, me.y, me.z, me.w, od.x, od.y, od.z, od.w);
+                        float4 o[LIST_BATCH];
+#pragma unroll
+                        for (uint32_t u = 0; u < LIST_BATCH; ++u) o[u] = posf[k0 + u < e ? k0 + u : e - 1];
+#pragma unroll
+                        for (uint32_t u = 0; u < LIST_BATCH; ++u) {
+                            const uint32_t k2 = k0 + u;
+                            if (k2 >= e) break;
+                            const float dx = mf.x - o[u].x, dy = mf.y - o[u].y, dz = mf.z - o[u].z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            const float hs = mf.w + o[u].w;
+                            const float rr = hs * hs;
+                            bool hit = d2 < rr * sc.lo;
+                            if (!hit && !(d2 > rr * sc.hi)) {
+                                const double4 od = pos[k2];
+                                hit = exact_pair(me.x, me.y, me.z, me.w, od.x, od.y, od.z, od.w);
+                            }
+                            if (hit && k2 != k) append(k2);
                         }
                     } else {
-                        const double4 o = pos[k2];
-                        hit = exact_pair(me.x, me.y, me.z, me.w, o.x, o.y, o.z, o.w);
-                    }
-                    if (hit && k2 != k) {
-                        buf[cnt & 3u] = k2;
-                        ++cnt;
-                        if ((cnt & 3u) == 0 && cnt <= kmax) {
-                            row4[(cnt >> 2) - 1] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+                        double4 o[LIST_BATCH];
+#pragma unroll
+                        for (uint32_t u = 0; u < LIST_BATCH; ++u) o[u] = pos[k0 + u < e ? k0 + u : e - 1];
+#pragma unroll
+                        for (uint32_t u = 0; u < LIST_BATCH; ++u) {
+                            const uint32_t k2 = k0 + u;
+                            if (k2 >= e) break;
+                            const bool hit = exact_pair(me.x, me.y, me.z, me.w, o[u].x, o[u].y, o[u].z, o[u].w);
+                            if (hit && k2 != k) append(k2);
                         }
                     }
                 }
@@ -541,6 +560,70 @@ __device__ __forceinline__ double lut_grad(const double* __restrict__ grads, dou
     return grads[i1] * (1.0 - ratio) + grads[i1 + 1] * ratio;
 }
 
+// Sum of v over the LPP adjacent lanes that share one particle (LPP = 1, 2 or 4), by DPP quad
+// permutations (lane ^ 1, then lane ^ 2).  Every lane of the group gets the same, order-fixed result:
+// (v0 + v1) + (v2 + v3) on lanes 0 and 1, (v2 + v3) + (v0 + v1) on lanes 2 and 3, equal since FP
+// addition commutes.  All lanes of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int LPP>
+__device__ __forceinline__ double group_sum(double v) {
+    static_assert(LPP == 1 || LPP == 2 || LPP == 4, "lanes per particle must be 1, 2 or 4");
+    if constexpr (LPP >= 2) v += dpp_move<0xb1>(v);  // quad_perm [1, 0, 3, 2]
+    if constexpr (LPP >= 4) v += dpp_move<0x4e>(v);  // quad_perm [2, 3, 0, 1]
+    return v;
+}
+
+// Particle handled by this lane and the lane's rank among the LPP lanes of that particle.  Lanes past
+// the end keep running (with an empty list) so that the group sums see every lane active.
+template <int LPP>
+struct PairLane {
+    size_t k;
+    uint32_t sub;
+    bool valid;
+    __device__ __forceinline__ PairLane(size_t n) {
+        const size_t t = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+        k = t / LPP;
+        sub = uint32_t(t % LPP);
+        valid = k < n;
+        if (!valid) k = n - 1;
+    }
+};
+
+// The neighbour list of one particle, split among its LPP lanes by quads (4 entries, 16 B): lane `sub`
+// takes quads sub, sub + LPP, ..., in order, the next quad loaded one quad ahead.  f(record, entry) runs
+// for each listed entry of the lane.  (Loading the next record ahead as well costs the pair kernels one
+// wave per SIMD of occupancy and measured slower.)
+template <int LPP, typename Rec, typename F>
+__device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, const uint32_t* __restrict__ list,
+                                                  size_t k, uint32_t kmax, uint32_t listed, uint32_t sub, F&& f) {
+    const uint32_t nq = (listed + 3u) >> 2;
+    if (sub >= nq) return;
+    const uint4* row = reinterpret_cast<const uint4*>(list + k * kmax);
+    const uint32_t myq = (nq - sub + LPP - 1) / LPP;  // quads of this lane
+    const uint32_t lastq = sub + (myq - 1) * LPP;
+    const uint32_t tail = listed - 4u * lastq;
+    const uint32_t total = 4u * (myq - 1) + (tail < 4u ? tail : 4u);  // entries of this lane
+    uint4 quad = row[sub];
+    uint4 nquad = myq > 1 ? row[sub + LPP] : quad;
+    for (uint32_t t = 0; t < total; ++t) {
+        const uint32_t sel = t & 3u;
+        if (sel == 0 && t > 0) {
+            quad = nquad;
+            const uint32_t j = (t >> 2) + 1;
+            nquad = j < myq ? row[sub + j * LPP] : quad;
+        }
+        const uint32_t e = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
+        const Rec o = rec[e];
+        f(o, e);
+    }
+}
+
 // FP64 pair sums in gather form.  For particle i (sorted position k) and each neighbour j, the kernel
 // gradient of SymmetrizeSmoothingLengths<LutKernel<3>> (Kernel.h:32-36,640-643) at hbar, then the
 // contributions every derivative of the term set makes to particle i:
@@ -551,29 +634,26 @@ __device__ __forceinline__ double lut_grad(const double* __restrict__ grads, dou
 //   StandardAV                               av/Standard.h:62-83
 //   NeighborCountTerm                        HelperTerms.h:38-46
 // The accumulated buffers replace the storage buffers (Accumulated::store, Accumulated.cpp:87-100).
+// LPP lanes share one particle: each sums its share of the list, then the partial sums are added across
+// the group (fewer particles in flight per XCD, so their neighbour records stay in its L2).
+template <int LPP>
 __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, const PairRec* __restrict__ rec,
                                                       const uint32_t* __restrict__ group,
                                                       const uint32_t* __restrict__ order,
                                                       const uint32_t* __restrict__ list,
                                                       const uint32_t* __restrict__ count,
                                                       const double* __restrict__ lut, uint32_t kmax, size_t n) {
-    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    const PairLane<LPP> ln(n);
+    const size_t k = ln.k;
     const PairRec me = rec[k];
     const uint32_t gi = group[k];
     const double rhoInv = 1.0 / me.rho;
     double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, divv = 0.0;
     double gxx = 0.0, gyy = 0.0, gzz = 0.0, gxy = 0.0, gxz = 0.0, gyz = 0.0;
-    const uint32_t cnt = count[k];
+    const uint32_t cnt = ln.valid ? count[k] : 0u;
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
-    const uint4* row = reinterpret_cast<const uint4*>(list + size_t(k) * kmax);
-    uint4 quad = make_uint4(0, 0, 0, 0);
-    for (uint32_t c = 0; c < listed; ++c) {
-        if ((c & 3u) == 0) quad = row[c >> 2];
-        const uint32_t sel = c & 3u;
-        const uint32_t k2 = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
-        const PairRec o = rec[k2];
+    for_each_neighbor<LPP>(rec, list, k, kmax, listed, ln.sub, [&](const PairRec& o, uint32_t k2) {
         const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
         const double hbar = 0.5 * (me.h + o.h);
         const double hInv = 1.0 / hbar;
@@ -619,7 +699,21 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
             az -= mj * (av * gz);
             du += mj * (0.5 * av * -(wx * gx + wy * gy + wz * gz));
         }
+    });
+    ax = group_sum<LPP>(ax);
+    ay = group_sum<LPP>(ay);
+    az = group_sum<LPP>(az);
+    du = group_sum<LPP>(du);
+    divv = group_sum<LPP>(divv);
+    if (solid) {
+        gxx = group_sum<LPP>(gxx);
+        gyy = group_sum<LPP>(gyy);
+        gzz = group_sum<LPP>(gzz);
+        gxy = group_sum<LPP>(gxy);
+        gxz = group_sum<LPP>(gxz);
+        gyz = group_sum<LPP>(gyz);
     }
+    if (!ln.valid || ln.sub != 0) return;
     const uint32_t i = order[k];
     s.d[AX][i] = ax;
     s.d[AY][i] = ay;
@@ -640,7 +734,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
 // ---- mixed precision ---------------------------------------------------------------------------------
 
 // Packs the mixed-precision neighbour record of particle order[k] (and the exact FP64 position used by
-// the neighbour-list build).
+// the neighbour-list build): position and velocity as 10-bit lattice cells plus FP32 offsets inside them.
 __global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, Lattice L,
                                                       const uint32_t* __restrict__ order, PairRecF* __restrict__ rec,
                                                       double4* __restrict__ pos, float4* __restrict__ posf,
@@ -660,9 +754,18 @@ __global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, La
         c[a] = uint32_t(f);
         d[a] = float(x[a] - (L.lo[a] + f * L.S));
     }
-    r.vx = s.d[VX][i];
-    r.vy = s.d[VY][i];
-    r.vz = s.d[VZ][i];
+    const double v[3] = { s.d[VX][i], s.d[VY][i], s.d[VZ][i] };
+    uint32_t cv[3];
+    float u[3];
+    for (int a = 0; a < 3; ++a) {
+        double f = floor((v[a] - L.vlo[a]) / L.Sv);
+        f = f < 0.0 ? 0.0 : (f > 1023.0 ? 1023.0 : f);
+        cv[a] = uint32_t(f);
+        u[a] = float((v[a] - L.vlo[a]) - f * L.Sv);
+    }
+    r.ux = u[0];
+    r.uy = u[1];
+    r.uz = u[2];
     r.dx = d[0];
     r.dy = d[1];
     r.dz = d[2];
@@ -683,12 +786,13 @@ __global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, La
         r.sxx = r.syy = r.sxy = r.sxz = r.syz = 0.f;
     }
     r.lat = c[0] | (c[1] << 10) | (c[2] << 20);
+    r.vlat = cv[0] | (cv[1] << 10) | (cv[2] << 20);
     uint32_t gid = 0;
     if (prm.sum_only_undamaged && prm.has_reduce) {
         gid = (s.d[RED][i] == 0.0) ? NO_GROUP : s.u[FLAG][i];
     }
     r.group = gid;
-    r.pad0 = r.pad1 = r.pad2 = 0u;
+    r.pad = 0u;
     rec[k] = r;
     pos[k] = make_double4(x[0], x[1], x[2], h);
     if (sc.on) posf[k] = screen_pos(sc, x[0], x[1], x[2], h);
@@ -712,39 +816,31 @@ __device__ __forceinline__ float m4_grad_f(float q2) {
 struct Acc {
     double sum = 0.0;
     __device__ __forceinline__ void add(float v) { sum += double(v); }
-    __device__ __forceinline__ double value() const { return sum; }
 };
 
 // Mixed-precision pair sums: the same terms and gather form as k_pairs_fp64, evaluated in FP32 on
-// lattice-relative positions and accumulated in FP64; results are written in FP64.
+// lattice-relative positions and velocities and accumulated in FP64; results are written in FP64.  A
+// difference (L_j - L_i) S + (d_j - d_i) is exact in its lattice part and rounds the FP32 offsets once, so
+// it keeps FP32 precision relative to the cell edge S, whatever the magnitudes of the absolute values.
+template <int LPP>
 __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
                                                        const PairRecF* __restrict__ rec,
                                                        const uint32_t* __restrict__ order,
                                                        const uint32_t* __restrict__ list,
                                                        const uint32_t* __restrict__ count, uint32_t kmax, size_t n) {
-    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    const PairLane<LPP> ln(n);
+    const size_t k = ln.k;
     const PairRecF me = rec[k];
     const uint32_t gi = me.group;
     const int lxi = int(me.lat & 1023u), lyi = int((me.lat >> 10) & 1023u), lzi = int(me.lat >> 20);
+    const int vxi = int(me.vlat & 1023u), vyi = int((me.vlat >> 10) & 1023u), vzi = int(me.vlat >> 20);
     const float rhoInv = 1.0f / me.rho;
     const float alpha = float(prm.av_alpha), beta = float(prm.av_beta);
     Acc ax, ay, az, du, divv, gxx, gyy, gzz, gxy, gxz, gyz;
-    const uint32_t cnt = count[k];
+    const uint32_t cnt = ln.valid ? count[k] : 0u;
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
-    const uint4* row = reinterpret_cast<const uint4*>(list + size_t(k) * kmax);
-    // the next quad of list entries is loaded one quad ahead (hides the global-memory latency)
-    uint4 quad = listed > 0 ? row[0] : make_uint4(0, 0, 0, 0);
-    uint4 next = listed > 4 ? row[1] : make_uint4(0, 0, 0, 0);
-    for (uint32_t c = 0; c < listed; ++c) {
-        const uint32_t sel = c & 3u;
-        if (sel == 0 && c > 0) {
-            quad = next;
-            next = (c + 4 < listed) ? row[(c >> 2) + 1] : make_uint4(0, 0, 0, 0);
-        }
-        const uint32_t e = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
-        const PairRecF o = rec[e];
+    for_each_neighbor<LPP>(rec, list, k, kmax, listed, ln.sub, [&](const PairRecF& o, uint32_t) {
         const int lxj = int(o.lat & 1023u), lyj = int((o.lat >> 10) & 1023u), lzj = int(o.lat >> 20);
         const float rx = float(lxi - lxj) * L.Sf + (me.dx - o.dx);
         const float ry = float(lyi - lyj) * L.Sf + (me.dy - o.dy);
@@ -756,7 +852,10 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
         const float gk = m4_grad_f(d2 * hInv2);
         const float h5g = (hInv2 * hInv2) * hInv * gk;
         const float gx = rx * h5g, gy = ry * h5g, gz = rz * h5g;
-        const float wx = float(o.vx - me.vx), wy = float(o.vy - me.vy), wz = float(o.vz - me.vz);
+        const int vxj = int(o.vlat & 1023u), vyj = int((o.vlat >> 10) & 1023u), vzj = int(o.vlat >> 20);
+        const float wx = float(vxj - vxi) * L.Svf + (o.ux - me.ux);  // v_j - v_i
+        const float wy = float(vyj - vyi) * L.Svf + (o.uy - me.uy);
+        const float wz = float(vzj - vzi) * L.Svf + (o.uz - me.uz);
         const float mj = o.m;
         const float mjr = mj * rhoInv;
         const float wg = wx * gx + wy * gy + wz * gz;
@@ -790,20 +889,32 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
         ax.add(mj * fx);
         ay.add(mj * fy);
         az.add(mj * fz);
-    }
-    const uint32_t i = order[k];
-    s.d[AX][i] = ax.value();
-    s.d[AY][i] = ay.value();
-    s.d[AZ][i] = az.value();
-    s.d[DU][i] = du.value();
-    s.d[DIVV][i] = divv.value();
+    });
+    const double sax = group_sum<LPP>(ax.sum), say = group_sum<LPP>(ay.sum), saz = group_sum<LPP>(az.sum);
+    const double sdu = group_sum<LPP>(du.sum), sdivv = group_sum<LPP>(divv.sum);
+    double g[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
     if (solid) {
-        s.d[GXX][i] = gxx.value();
-        s.d[GYY][i] = gyy.value();
-        s.d[GZZ][i] = gzz.value();
-        s.d[GXY][i] = gxy.value();
-        s.d[GXZ][i] = gxz.value();
-        s.d[GYZ][i] = gyz.value();
+        g[0] = group_sum<LPP>(gxx.sum);
+        g[1] = group_sum<LPP>(gyy.sum);
+        g[2] = group_sum<LPP>(gzz.sum);
+        g[3] = group_sum<LPP>(gxy.sum);
+        g[4] = group_sum<LPP>(gxz.sum);
+        g[5] = group_sum<LPP>(gyz.sum);
+    }
+    if (!ln.valid || ln.sub != 0) return;
+    const uint32_t i = order[k];
+    s.d[AX][i] = sax;
+    s.d[AY][i] = say;
+    s.d[AZ][i] = saz;
+    s.d[DU][i] = sdu;
+    s.d[DIVV][i] = sdivv;
+    if (solid) {
+        s.d[GXX][i] = g[0];
+        s.d[GYY][i] = g[1];
+        s.d[GZZ][i] = g[2];
+        s.d[GXY][i] = g[3];
+        s.d[GXZ][i] = g[4];
+        s.d[GYZ][i] = g[5];
     }
     s.u[NCNT][i] = cnt;
 }
@@ -1057,23 +1168,60 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_correct(DevState s, const os
 }
 
 // One pass before the neighbour search, MODE 2: predictions (predict_at) then the material initialize;
-// MODE 1: the material initialize only; MODE 0: neither.  Every mode accumulates the per-class grid
-// bounds of the resulting positions, with the class of h relative to href.
+// MODE 1: the material initialize only.
 template <int MODE>
 __global__ void __launch_bounds__(BLOCK) k_advance(DevState s, const osph_material* __restrict__ mats, Params prm,
-                                                   double dt, double href, unsigned long long* acc, size_t n) {
+                                                   double dt, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const osph_material m = mats[s.u[MAT][i]];
+    if (MODE > 1) predict_at(s, m, prm, dt, i);
+    material_init_at(s, m, prm, i);
+}
+
+// Per-class grid bounds of the positions, with the class of h relative to href, and with VEL the bounds
+// of the velocities (vel[0..2] min, vel[3..5] max).  A separate pass: the per-class accumulators held
+// across the predictor would cost the streaming pass half its occupancy.
+template <bool VEL>
+__global__ void __launch_bounds__(BLOCK) k_class_bounds(DevState s, double href, unsigned long long* acc,
+                                                        unsigned long long* vel, size_t n) {
     ClassAcc ca;
     ca.init();
+    double vlo[3] = { INFINITY, INFINITY, INFINITY }, vhi[3] = { -INFINITY, -INFINITY, -INFINITY };
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-        if (MODE > 0) {
-            const osph_material m = mats[s.u[MAT][i]];
-            if (MODE > 1) predict_at(s, m, prm, dt, i);
-            material_init_at(s, m, prm, i);
-        }
         const double h = s.d[H][i];
         ca.add(h_class(h, href, MAX_CLASSES), s.d[X][i], s.d[Y][i], s.d[Z][i], h);
+        if (VEL) {
+            const double v[3] = { s.d[VX][i], s.d[VY][i], s.d[VZ][i] };
+            for (int a = 0; a < 3; ++a) {
+                vlo[a] = dmin(vlo[a], v[a]);
+                vhi[a] = dmax(vhi[a], v[a]);
+            }
+        }
     }
     ca.reduce(acc);
+    if (VEL) {
+        __shared__ unsigned long long red[BLOCK / 64][6];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int k = 0; k < 6; ++k) {
+            const unsigned long long r = k < 3 ? wave_min(ord_bits(vlo[k])) : wave_max(ord_bits(vhi[k - 3]));
+            if (lane == 0) red[wave][k] = r;
+        }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const int k = threadIdx.x;
+            unsigned long long r = red[0][k];
+            for (int w = 1; w < int(blockDim.x / 64); ++w) {
+                const unsigned long long x = red[w][k];
+                r = k < 3 ? (x < r ? x : r) : (x > r ? x : r);
+            }
+            if (k < 3) {
+                atomicMin(&vel[k], r);
+            } else {
+                atomicMax(&vel[k], r);
+            }
+        }
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_criteria(DevState s, const osph_material* __restrict__ mats, Params prm,
@@ -1263,16 +1411,19 @@ void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* 
 }
 
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
-                    double href, unsigned long long* acc, size_t n) {
+                    double href, unsigned long long* acc, unsigned long long* vel, size_t n) {
     if (!n) return;
-    unsigned g = grid_for(n);
-    g = g > 2048 ? 2048 : g;
     if (mode == 2) {
-        k_advance<2><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, acc, n);
+        k_advance<2><<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, n);
     } else if (mode == 1) {
-        k_advance<1><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, acc, n);
+        k_advance<1><<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, n);
+    }
+    unsigned g = grid_for(n);
+    g = g > 1024 ? 1024 : g;
+    if (vel) {
+        k_class_bounds<true><<<g, BLOCK, 0, l.stream>>>(s, href, acc, vel, n);
     } else {
-        k_advance<0><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, acc, n);
+        k_class_bounds<false><<<g, BLOCK, 0, l.stream>>>(s, href, acc, vel, n);
     }
 }
 
@@ -1300,9 +1451,16 @@ void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* r
                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
                        uint32_t kmax, size_t n) {
     if (!n) return;
-    const int b = l.tune.pairs_block;
-    k_pairs_fp64<<<grid_for(n, b), b, size_t(l.tune.pairs_lds), l.stream>>>(s, prm, rec, group, order, list, count,
-                                                                              lut_grads, kmax, n);
+    const int b = l.tune.pairs_block, lpp = l.tune.pairs_lanes;
+    const unsigned grid = grid_for(n * size_t(lpp), b);
+    const size_t lds = size_t(l.tune.pairs_lds);
+    if (lpp == 4) {
+        k_pairs_fp64<4><<<grid, b, lds, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n);
+    } else if (lpp == 2) {
+        k_pairs_fp64<2><<<grid, b, lds, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n);
+    } else {
+        k_pairs_fp64<1><<<grid, b, lds, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n);
+    }
 }
 
 void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
@@ -1314,9 +1472,16 @@ void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, con
 void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
                         const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n) {
     if (!n) return;
-    const int b = l.tune.pairs_block;
-    k_pairs_mixed<<<grid_for(n, b), b, size_t(l.tune.pairs_lds), l.stream>>>(s, prm, lat, rec, order, list, count,
-                                                                              kmax, n);
+    const int b = l.tune.pairs_block, lpp = l.tune.pairs_lanes;
+    const unsigned grid = grid_for(n * size_t(lpp), b);
+    const size_t lds = size_t(l.tune.pairs_lds);
+    if (lpp == 4) {
+        k_pairs_mixed<4><<<grid, b, lds, l.stream>>>(s, prm, lat, rec, order, list, count, kmax, n);
+    } else if (lpp == 2) {
+        k_pairs_mixed<2><<<grid, b, lds, l.stream>>>(s, prm, lat, rec, order, list, count, kmax, n);
+    } else {
+        k_pairs_mixed<1><<<grid, b, lds, l.stream>>>(s, prm, lat, rec, order, list, count, kmax, n);
+    }
 }
 
 void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n) {
