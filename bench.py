@@ -34,6 +34,10 @@ FLOP_PER_PAIR = 198.0
 # |r|^2 5, 1/sqrt and powers 8, q2 d 15, d q2 d 6, terms 15, monopole 9); counted as in the pair figure.
 FLOP_PER_GRAVITY_PAIR = 22.0
 FLOP_PER_QUADRUPOLE = 61.0
+# HBM bytes per launch of the pair kernel at the default workload and tuning, from two rocprofv3 --pmc
+# passes over this bench (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000: the factors the 1 GiB device
+# copy of --calibrate reads).  None where not measured.
+MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (620181 + 97781) * 1024.0, "pairs_fp64": None}
 
 
 def log(*a):
@@ -301,8 +305,9 @@ def main():
     achieved = FLOP_PER_PAIR * pairs / (avg_ms * 1e-3) / 1e12
     peak = FP32_PEAK_TFLOPS if main_prec == "mixed" else FP64_PEAK_TFLOPS
     dominant = max(prof.items(), key=lambda kv: kv[1][0])[0] if prof else kernel
+    traffic = MEASURED_TRAFFIC_BYTES.get(kernel) if st.n == 1062568 else None
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "kernel": kernel,
+                "frac": achieved / peak, "traffic": traffic, "kernel": kernel,
                 "kernel_ms": avg_ms, "dominant_kernel": dominant,
                 "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in sorted(prof.items())}}
     accuracy = None
