@@ -25,6 +25,15 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
     return x * q + (x < r ? x : r) + local;
 }
 
+// Orders the LDS accesses of one wavefront around this point (no cross-wave synchronisation): a
+// wavefront's LDS instructions execute in issue order, so after it every lane sees what every lane of
+// the same wavefront wrote before it.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }  // MathBasic.h:15-17
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }  // MathBasic.h:20-22
 

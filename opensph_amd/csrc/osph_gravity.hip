@@ -598,14 +598,6 @@ __device__ __forceinline__ void add_multipole(const NodeMom& q, double drx, doub
     a[2] += az;
 }
 
-// Synchronisation of the lanes of one wavefront around its private LDS (no work-group barrier: the
-// wavefronts of a work-group run different targets with different loop counts).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Per-wavefront staging area of k_eval: 64 node expansions (SoA) or 64 source particles.
 struct EvalStage {
     double c[3][64];

@@ -488,57 +488,94 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
             lo[a] = c - r < 0 ? 0 : c - r;
             up[a] = c + r >= G.dim[a] ? G.dim[a] - 1 : c + r;
         }
-        // rows of cells whose yz-rectangle lies farther than R (with a relative margin) hold no neighbour;
-        // in the others, a neighbour lies within the chord half-width w = sqrt(R^2 - d_yz^2) of the
-        // particle along x, and the cells of that x-run are one contiguous range of sorted particles
-        const double R2 = R * R * (1.0 + 1e-6);
-        for (int cz = lo[2]; cz <= up[2]; ++cz) {
-            const double tz = dmax(dmax(G.lo[2] + cz * G.cell - p[2], p[2] - (G.lo[2] + (cz + 1) * G.cell)), 0.0);
-            for (int cy = lo[1]; cy <= up[1]; ++cy) {
-                const double ty = dmax(dmax(G.lo[1] + cy * G.cell - p[1], p[1] - (G.lo[1] + (cy + 1) * G.cell)), 0.0);
-                const double dzy = tz * tz + ty * ty;
-                if (dzy > R2) continue;
-                const double w = sqrt(R2 - dzy);
-                int x0 = int(floor((p[0] - w - G.lo[0]) * G.inv_cell));
-                int x1 = int(floor((p[0] + w - G.lo[0]) * G.inv_cell));
+        if (lo[0] > up[0] || lo[1] > up[1] || lo[2] > up[2]) continue;
+        // Rows of cells (cy, cz) whose yz-square lies farther than R hold no neighbour; in the others a
+        // neighbour lies within the chord half-width w = sqrt(R^2 - d_yz^2) of the particle along x, and
+        // the cells of that x-run are one contiguous range of sorted particles.  The row geometry is
+        // evaluated in FP32 in cell units with R widened by 1e-3 cells: the FP32 errors (below 1e-4
+        // cells for the at most a few hundred cells per axis) then only ever add candidates, and the
+        // exact test decides.  The ranges of the next row are loaded before the current row's
+        // candidates are tested, so the two loads overlap.
+        const float pcx = float((p[0] - G.lo[0]) * G.inv_cell);
+        const float pcy = float((p[1] - G.lo[1]) * G.inv_cell);
+        const float pcz = float((p[2] - G.lo[2]) * G.inv_cell);
+        const float Rc = float(R * G.inv_cell) * (1.0f + 1e-6f) + 1e-3f;
+        const float R2c = Rc * Rc;
+        int cy = lo[1] - 1, cz = lo[2];
+        float tz2 = 0.0f;
+        uint32_t zkey = 0;
+        auto plane = [&]() {
+            const float fz = float(cz);
+            const float tz = fmaxf(fmaxf(fz - pcz, pcz - (fz + 1.0f)), 0.0f);
+            tz2 = tz * tz;
+            zkey = spread2(uint32_t(cz)) << 1;
+        };
+        plane();
+        // the next row within R after (cy, cz): its first and past-the-end sorted positions
+        auto next_row = [&](uint32_t& b, uint32_t& e) -> bool {
+            for (;;) {
+                if (++cy > up[1]) {
+                    cy = lo[1];
+                    if (++cz > up[2]) return false;
+                    plane();
+                }
+                if (tz2 > R2c) {
+                    cy = up[1];
+                    continue;
+                }
+                const float fy = float(cy);
+                const float ty = fmaxf(fmaxf(fy - pcy, pcy - (fy + 1.0f)), 0.0f);
+                const float dzy = tz2 + ty * ty;
+                if (dzy > R2c) continue;
+                const float w = sqrtf(R2c - dzy);
+                int x0 = int(floorf(pcx - w));
+                int x1 = int(floorf(pcx + w));
                 x0 = x0 < lo[0] ? lo[0] : x0;
                 x1 = x1 > up[0] ? up[0] : x1;
                 if (x0 > x1) continue;
-                const uint32_t row = G.base + row_key(0u, uint32_t(cy), uint32_t(cz), G.bits);
-                const uint32_t b = first[row + uint32_t(x0)], e = first[row + uint32_t(x1) + 1u];
-                // candidates in batches of LIST_BATCH: all loads of a batch are issued before the first
-                // test (clamped to the last candidate, so every load stays inside [b, e))
-                for (uint32_t k0 = b; k0 < e; k0 += LIST_BATCH) {
-                    if (SCREEN) {
-                        float4 o[LIST_BATCH];
+                const uint32_t row = G.base + ((spread2(uint32_t(cy)) | zkey) << G.bits);
+                b = first[row + uint32_t(x0)];
+                e = first[row + uint32_t(x1) + 1u];
+                return true;
+            }
+        };
+        uint32_t nb = 0, ne = 0;
+        bool more = next_row(nb, ne);
+        while (more) {
+            const uint32_t b = nb, e = ne;
+            more = next_row(nb, ne);
+            // candidates in batches of LIST_BATCH: all loads of a batch are issued before the first
+            // test (clamped to the last candidate, so every load stays inside [b, e))
+            for (uint32_t k0 = b; k0 < e; k0 += LIST_BATCH) {
+                if (SCREEN) {
+                    float4 o[LIST_BATCH];
 #pragma unroll
-                        for (uint32_t u = 0; u < LIST_BATCH; ++u) o[u] = posf[k0 + u < e ? k0 + u : e - 1];
+                    for (uint32_t u = 0; u < LIST_BATCH; ++u) o[u] = posf[k0 + u < e ? k0 + u : e - 1];
 #pragma unroll
-                        for (uint32_t u = 0; u < LIST_BATCH; ++u) {
-                            const uint32_t k2 = k0 + u;
-                            if (k2 >= e) break;
-                            const float dx = mf.x - o[u].x, dy = mf.y - o[u].y, dz = mf.z - o[u].z;
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            const float hs = mf.w + o[u].w;
-                            const float rr = hs * hs;
-                            bool hit = d2 < rr * sc.lo;
-                            if (!hit && !(d2 > rr * sc.hi)) {
-                                const double4 od = pos[k2];
-                                hit = exact_pair(me.x, me.y, me.z, me.w, od.x, od.y, od.z, od.w);
-                            }
-                            if (hit && k2 != k) append(k2);
+                    for (uint32_t u = 0; u < LIST_BATCH; ++u) {
+                        const uint32_t k2 = k0 + u;
+                        if (k2 >= e) break;
+                        const float dx = mf.x - o[u].x, dy = mf.y - o[u].y, dz = mf.z - o[u].z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const float hs = mf.w + o[u].w;
+                        const float rr = hs * hs;
+                        bool hit = d2 < rr * sc.lo;
+                        if (!hit && !(d2 > rr * sc.hi)) {
+                            const double4 od = pos[k2];
+                            hit = exact_pair(me.x, me.y, me.z, me.w, od.x, od.y, od.z, od.w);
                         }
-                    } else {
-                        double4 o[LIST_BATCH];
+                        if (hit && k2 != k) append(k2);
+                    }
+                } else {
+                    double4 o[LIST_BATCH];
 #pragma unroll
-                        for (uint32_t u = 0; u < LIST_BATCH; ++u) o[u] = pos[k0 + u < e ? k0 + u : e - 1];
+                    for (uint32_t u = 0; u < LIST_BATCH; ++u) o[u] = pos[k0 + u < e ? k0 + u : e - 1];
 #pragma unroll
-                        for (uint32_t u = 0; u < LIST_BATCH; ++u) {
-                            const uint32_t k2 = k0 + u;
-                            if (k2 >= e) break;
-                            const bool hit = exact_pair(me.x, me.y, me.z, me.w, o[u].x, o[u].y, o[u].z, o[u].w);
-                            if (hit && k2 != k) append(k2);
-                        }
+                    for (uint32_t u = 0; u < LIST_BATCH; ++u) {
+                        const uint32_t k2 = k0 + u;
+                        if (k2 >= e) break;
+                        const bool hit = exact_pair(me.x, me.y, me.z, me.w, o[u].x, o[u].y, o[u].z, o[u].w);
+                        if (hit && k2 != k) append(k2);
                     }
                 }
             }
@@ -595,32 +632,96 @@ struct PairLane {
     }
 };
 
+// Per-wavefront LDS staging of neighbour records: 64 records of CH 16-byte chunks each, at an odd stride
+// of chunks (CH | 1) so that the lanes' 16-byte reads of their own record spread over all banks.
+template <typename Rec>
+struct StageOf {
+    static constexpr uint32_t CH = sizeof(Rec) / 16;
+    static constexpr uint32_t STRIDE = CH | 1u;
+    static constexpr uint32_t WORDS = 64 * STRIDE;  // uint4 per wavefront
+    static_assert(sizeof(Rec) % 16 == 0, "records are whole 16-byte chunks");
+};
+
 // The neighbour list of one particle, split among its LPP lanes by quads (4 entries, 16 B): lane `sub`
 // takes quads sub, sub + LPP, ..., in order, the next quad loaded one quad ahead.  f(record, entry) runs
-// for each listed entry of the lane.  (Loading the next record ahead as well costs the pair kernels one
-// wave per SIMD of occupancy and measured slower.)
-template <int LPP, typename Rec, typename F>
+// for each listed entry of the lane.
+//
+// STAGED: the records are fetched cooperatively.  In each step the wavefront needs one record per lane;
+// pass i of the CH load passes has lane L fetch chunk c of the record of lane r, (r, c) = divmod(64 i +
+// L, CH), into the wavefront's LDS stage, and each lane then reads its own record back.  Consecutive
+// lanes fetch consecutive chunks of one record, so a load instruction touches ~64 / CH records' cache
+// lines instead of one line per lane: CH times fewer line accesses in the texture path, for a longer
+// dependency chain (shuffle, load, LDS write, LDS read).  It pays for the 128-B FP64 record (1.16 ->
+// 1.06 ms at the impact) and not for the 80-B mixed one (0.45 -> 0.51 ms), which gathers directly.
+// Every lane of the wavefront must call this (lanes without entries load their own record as a
+// stand-in, and run no f).
+template <int LPP, bool STAGED, typename Rec, typename F>
 __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, const uint32_t* __restrict__ list,
-                                                  size_t k, uint32_t kmax, uint32_t listed, uint32_t sub, F&& f) {
+                                                  size_t k, uint32_t kmax, uint32_t listed, uint32_t sub,
+                                                  uint4* __restrict__ stage, F&& f) {
+    using S = StageOf<Rec>;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nq = (listed + 3u) >> 2;
-    if (sub >= nq) return;
+    const uint32_t myq = sub < nq ? (nq - sub + LPP - 1) / LPP : 0u;  // quads of this lane
+    uint32_t total = 0;  // entries of this lane
+    if (myq > 0) {
+        const uint32_t tail = listed - 4u * (sub + (myq - 1) * LPP);
+        total = 4u * (myq - 1) + (tail < 4u ? tail : 4u);
+    }
     const uint4* row = reinterpret_cast<const uint4*>(list + k * kmax);
-    const uint32_t myq = (nq - sub + LPP - 1) / LPP;  // quads of this lane
-    const uint32_t lastq = sub + (myq - 1) * LPP;
-    const uint32_t tail = listed - 4u * lastq;
-    const uint32_t total = 4u * (myq - 1) + (tail < 4u ? tail : 4u);  // entries of this lane
-    uint4 quad = row[sub];
-    uint4 nquad = myq > 1 ? row[sub + LPP] : quad;
-    for (uint32_t t = 0; t < total; ++t) {
-        const uint32_t sel = t & 3u;
-        if (sel == 0 && t > 0) {
-            quad = nquad;
-            const uint32_t j = (t >> 2) + 1;
-            nquad = j < myq ? row[sub + j * LPP] : quad;
+    if constexpr (!STAGED) {
+        if (myq == 0) return;
+        uint4 quad = row[sub];
+        uint4 nquad = myq > 1 ? row[sub + LPP] : quad;
+        for (uint32_t t = 0; t < total; ++t) {
+            const uint32_t sel = t & 3u;
+            if (sel == 0 && t > 0) {
+                quad = nquad;
+                const uint32_t j = (t >> 2) + 1;
+                nquad = j < myq ? row[sub + j * LPP] : quad;
+            }
+            const uint32_t e = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
+            const Rec o = rec[e];
+            f(o, e);
         }
-        const uint32_t e = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
-        const Rec o = rec[e];
-        f(o, e);
+        return;
+    }
+    // (in the staged loop the step index is uniform across the wavefront, so the entry is taken from
+    // rotating registers rather than selected by index, which the compiler would place in scratch)
+    const uint4 first = myq > 0 ? row[sub] : make_uint4(0, 0, 0, 0);
+    uint32_t q0 = first.x, q1 = first.y, q2 = first.z, q3 = first.w;  // entries of the current quad, next first
+    uint4 nquad = myq > 1 ? row[sub + LPP] : first;
+    const uint4* src = reinterpret_cast<const uint4*>(rec);
+    for (uint32_t t = 0; __any(t < total); ++t) {
+        const bool act = t < total;
+        if (act && (t & 3u) == 0 && t > 0) {
+            q0 = nquad.x;
+            q1 = nquad.y;
+            q2 = nquad.z;
+            q3 = nquad.w;
+            const uint32_t j = (t >> 2) + 1;
+            if (j < myq) nquad = row[sub + j * LPP];
+        }
+        const uint32_t e = act ? q0 : uint32_t(k);
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        wave_sync();  // the previous step's reads of the stage come first
+#pragma unroll
+        for (uint32_t i = 0; i < S::CH; ++i) {
+            const uint32_t q = i * 64u + lane;
+            const uint32_t r = q / S::CH, c = q - r * S::CH;
+            const uint32_t er = uint32_t(__shfl(int(e), int(r), 64));
+            stage[r * S::STRIDE + c] = src[size_t(er) * S::CH + c];
+        }
+        wave_sync();
+        union {
+            uint4 v[S::CH];
+            Rec r;
+        } o;
+#pragma unroll
+        for (uint32_t c = 0; c < S::CH; ++c) o.v[c] = stage[lane * S::STRIDE + c];
+        if (act) f(o.r, e);
     }
 }
 
@@ -653,7 +754,9 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
     const uint32_t cnt = ln.valid ? count[k] : 0u;
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
-    for_each_neighbor<LPP>(rec, list, k, kmax, listed, ln.sub, [&](const PairRec& o, uint32_t k2) {
+    __shared__ uint4 stage[(BLOCK / 64) * StageOf<PairRec>::WORDS];
+    uint4* my_stage = stage + (threadIdx.x >> 6) * StageOf<PairRec>::WORDS;
+    for_each_neighbor<LPP, true>(rec, list, k, kmax, listed, ln.sub, my_stage, [&](const PairRec& o, uint32_t k2) {
         const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
         const double hbar = 0.5 * (me.h + o.h);
         const double hInv = 1.0 / hbar;
@@ -801,14 +904,15 @@ __global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, La
 
 // M4 gradient factor grad W / q of CubicSpline<3> (core/sph/kernel/Kernel.h:172-187) in closed form,
 // FP32: for q < 1 the bracket divided by q simplifies to (2.25 q - 3) / pi (finite at q = 0, where the
-// reference returns -3 / pi); for 1 <= q < 2 it is -0.75 (2 - q)^2 / (pi q).
+// reference returns -3 / pi); for 1 <= q < 2 it is -0.75 (2 - q)^2 / (pi q).  The square root and the
+// reciprocal are the hardware instructions (about 1 ulp).
 __device__ __forceinline__ float m4_grad_f(float q2) {
     constexpr float INV_PI = 0.318309886183790671538f;
     if (!(q2 < 4.0f)) return 0.0f;
-    const float q = sqrtf(q2);
+    const float q = __builtin_amdgcn_sqrtf(q2);
     if (q < 1.0f) return INV_PI * (2.25f * q - 3.0f);
     const float t = 2.0f - q;
-    return -0.75f * INV_PI * (t * t) / q;
+    return (-0.75f * INV_PI) * (t * t) * __builtin_amdgcn_rcpf(q);
 }
 
 // FP64 accumulator of FP32 terms: each term is widened exactly and summed in FP64 (on gfx950 an FP64 add
@@ -822,6 +926,7 @@ struct Acc {
 // lattice-relative positions and velocities and accumulated in FP64; results are written in FP64.  A
 // difference (L_j - L_i) S + (d_j - d_i) is exact in its lattice part and rounds the FP32 offsets once, so
 // it keeps FP32 precision relative to the cell edge S, whatever the magnitudes of the absolute values.
+// Divisions are hardware reciprocals (about 1 ulp) times the numerator.
 template <int LPP>
 __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
                                                        const PairRecF* __restrict__ rec,
@@ -840,13 +945,13 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
     const uint32_t cnt = ln.valid ? count[k] : 0u;
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
-    for_each_neighbor<LPP>(rec, list, k, kmax, listed, ln.sub, [&](const PairRecF& o, uint32_t) {
+    for_each_neighbor<LPP, false>(rec, list, k, kmax, listed, ln.sub, nullptr, [&](const PairRecF& o, uint32_t) {
         const int lxj = int(o.lat & 1023u), lyj = int((o.lat >> 10) & 1023u), lzj = int(o.lat >> 20);
         const float rx = float(lxi - lxj) * L.Sf + (me.dx - o.dx);
         const float ry = float(lyi - lyj) * L.Sf + (me.dy - o.dy);
         const float rz = float(lzi - lzj) * L.Sf + (me.dz - o.dz);
         const float hbar = 0.5f * (me.h + o.h);
-        const float hInv = 1.0f / hbar;
+        const float hInv = __builtin_amdgcn_rcpf(hbar);
         const float hInv2 = hInv * hInv;
         const float d2 = rx * rx + ry * ry + rz * rz;
         const float gk = m4_grad_f(d2 * hInv2);
@@ -879,8 +984,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
         if (dvdr < 0.0f) {
             const float rhobar = 0.5f * (me.rho + o.rho);
             const float csbar = 0.5f * (me.cs + o.cs);
-            const float mu = hbar * dvdr / (d2 + 1.e-2f * (hbar * hbar));
-            const float av = (-alpha * csbar * mu + beta * (mu * mu)) / rhobar;
+            const float mu = hbar * dvdr * __builtin_amdgcn_rcpf(d2 + 1.e-2f * (hbar * hbar));
+            const float av = (-alpha * csbar * mu + beta * (mu * mu)) * __builtin_amdgcn_rcpf(rhobar);
             fx -= av * gx;
             fy -= av * gy;
             fz -= av * gz;
