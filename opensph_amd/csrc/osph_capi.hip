@@ -185,6 +185,10 @@ int ensure_staging(osph_ctx* c, size_t bytes) {
     return OSPH_OK;
 }
 
+size_t list_entries(size_t n, uint32_t kmax) {
+    return (n + LIST_TILE - 1) / LIST_TILE * LIST_TILE * size_t(kmax);
+}
+
 int ensure_list(osph_ctx* c, size_t entries) {
     if (entries <= c->list_cap) return OSPH_OK;
     (void)hipFree(c->list);
@@ -552,7 +556,7 @@ int search_grid(osph_ctx* c, int mode, double dt) {
 uint32_t* overflow_flag(osph_ctx* c) { return reinterpret_cast<uint32_t*>(c->small + SMALL_OVERFLOW); }
 
 int launch_lists(osph_ctx* c) {
-    if (int e = ensure_list(c, c->n * size_t(c->kmax))) return e;
+    if (int e = ensure_list(c, list_entries(c->n, c->kmax))) return e;
     timed(c, "neighbor_lists", [&] {
         launch_neighbor_lists(L(c), c->pos, c->posf, c->screen, c->grid, c->cell_first, c->list, c->count, c->kmax,
                               overflow_flag(c), c->n);
@@ -881,7 +885,7 @@ int osph_get_pairs(osph_ctx* c, uint32_t* pi, uint32_t* pj, size_t cap, size_t* 
     if (!c || !count) return OSPH_ERR_ARG;
     if (!c->neighbors_valid) return c->fail(OSPH_ERR_STATE, "neighbours not built");
     const size_t n = c->n;
-    std::vector<uint32_t> cnt(n), order(n), perm(n), list(n * size_t(c->kmax));
+    std::vector<uint32_t> cnt(n), order(n), perm(n), list(list_entries(n, c->kmax));
     if (n) {
         HIPCHK(c, hipMemcpy(cnt.data(), c->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(order.data(), c->order, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -893,7 +897,7 @@ int osph_get_pairs(osph_ctx* c, uint32_t* pi, uint32_t* pj, size_t cap, size_t* 
     for (size_t k = 0; k < n; ++k) {
         const uint64_t i = order[k];
         for (uint32_t m = 0; m < cnt[k] && m < c->kmax; ++m) {
-            const uint64_t j = order[list[size_t(k) * c->kmax + m]];
+            const uint64_t j = order[list[list_quad(k, m >> 2, c->kmax >> 2) * 4 + (m & 3u)]];
             if (i < j) keys.push_back((i << 32) | j);
         }
     }

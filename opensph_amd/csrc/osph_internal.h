@@ -137,6 +137,16 @@ __host__ __device__ inline uint32_t row_key(uint32_t x, uint32_t y, uint32_t z, 
     return ((spread2(y) | (spread2(z) << 1)) << bits) | x;
 }
 
+// Neighbour-list layout: quad j (entries 4j .. 4j + 3) of sorted particle k, with Q = kmax / 4 quads
+// per particle, is uint4 element ((k / LIST_TILE) * Q + j) * LIST_TILE + k % LIST_TILE.  Tiles of
+// LIST_TILE consecutive particles hold all their quads, quad-major: the list kernel's lanes fill whole
+// lines when they store their j-th quads, and the pair kernels still read each particle's quads from
+// one small region.  The list holds round_up(n, LIST_TILE) * kmax entries.
+constexpr uint32_t LIST_TILE = 8;
+__host__ __device__ inline size_t list_quad(size_t k, uint32_t j, uint32_t quads) {
+    return ((k / LIST_TILE) * quads + j) * LIST_TILE + (k % LIST_TILE);
+}
+
 // ---- launchers implemented in osph_kernels.hip -------------------------------------------------------
 // Launch shapes of the pair-stage kernels, chosen by the tuner (osph_tune): threads per block and
 // dynamic LDS per block (which caps the number of resident blocks per CU, and with it the size of the

@@ -443,8 +443,8 @@ __device__ __forceinline__ bool exact_pair(double xi, double yi, double zi, doub
 
 // One thread per particle (sorted position k).  For every smoothing-length class whose bounding box
 // lies within h_i + h_max(class), the cells of that class within that radius are scanned with the exact
-// pair test.  Neighbour positions go to the particle's row list[k * kmax + c] (kmax a multiple of 4),
-// buffered four at a time in registers and stored as one 16-byte write.
+// pair test.  Neighbour positions are buffered four at a time in registers and stored as one 16-byte
+// quad at list_quad(k, j, kmax / 4).
 //
 // With SCREEN, candidates are first tested in FP32 on origin-relative positions (16 bytes per candidate
 // instead of 32).  The FP32 squared distance and radius carry a relative error below sc.band (bounded on
@@ -463,13 +463,15 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
     const double4 me = pos[k];
     const float4 mf = SCREEN ? posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     const double p[3] = { me.x, me.y, me.z };
-    uint4* row4 = reinterpret_cast<uint4*>(list + size_t(k) * kmax);
+    uint4* list4 = reinterpret_cast<uint4*>(list);
     uint32_t buf[4] = { 0, 0, 0, 0 };
     uint32_t cnt = 0;
     auto append = [&](uint32_t k2) {
         buf[cnt & 3u] = k2;
         ++cnt;
-        if ((cnt & 3u) == 0 && cnt <= kmax) row4[(cnt >> 2) - 1] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+        if ((cnt & 3u) == 0 && cnt <= kmax) {
+            list4[list_quad(k, (cnt >> 2) - 1, kmax >> 2)] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+        }
     };
     for (int cls = 0; cls < g.nclass; ++cls) {
         const GridClass& G = g.c[cls];
@@ -582,7 +584,7 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
         }
     }
     if ((cnt & 3u) != 0 && cnt < kmax) {
-        row4[cnt >> 2] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+        list4[list_quad(k, cnt >> 2, kmax >> 2)] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
     }
     count[k] = cnt;
     if (cnt > kmax) atomicOr(overflow, 1u);
@@ -657,7 +659,7 @@ struct StageOf {
 // stand-in, and run no f).
 template <int LPP, bool STAGED, typename Rec, typename F>
 __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, const uint32_t* __restrict__ list,
-                                                  size_t k, uint32_t kmax, uint32_t listed, uint32_t sub,
+                                                  size_t k, uint32_t quads, uint32_t listed, uint32_t sub,
                                                   uint4* __restrict__ stage, F&& f) {
     using S = StageOf<Rec>;
     const uint32_t lane = threadIdx.x & 63u;
@@ -668,17 +670,18 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
         const uint32_t tail = listed - 4u * (sub + (myq - 1) * LPP);
         total = 4u * (myq - 1) + (tail < 4u ? tail : 4u);
     }
-    const uint4* row = reinterpret_cast<const uint4*>(list + k * kmax);
+    const uint4* list4 = reinterpret_cast<const uint4*>(list);
+    auto row = [&](uint32_t j) { return list4[list_quad(k, j, quads)]; };
     if constexpr (!STAGED) {
         if (myq == 0) return;
-        uint4 quad = row[sub];
-        uint4 nquad = myq > 1 ? row[sub + LPP] : quad;
+        uint4 quad = row(sub);
+        uint4 nquad = myq > 1 ? row(sub + LPP) : quad;
         for (uint32_t t = 0; t < total; ++t) {
             const uint32_t sel = t & 3u;
             if (sel == 0 && t > 0) {
                 quad = nquad;
                 const uint32_t j = (t >> 2) + 1;
-                nquad = j < myq ? row[sub + j * LPP] : quad;
+                nquad = j < myq ? row(sub + j * LPP) : quad;
             }
             const uint32_t e = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
             const Rec o = rec[e];
@@ -688,9 +691,9 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
     }
     // (in the staged loop the step index is uniform across the wavefront, so the entry is taken from
     // rotating registers rather than selected by index, which the compiler would place in scratch)
-    const uint4 first = myq > 0 ? row[sub] : make_uint4(0, 0, 0, 0);
+    const uint4 first = myq > 0 ? row(sub) : make_uint4(0, 0, 0, 0);
     uint32_t q0 = first.x, q1 = first.y, q2 = first.z, q3 = first.w;  // entries of the current quad, next first
-    uint4 nquad = myq > 1 ? row[sub + LPP] : first;
+    uint4 nquad = myq > 1 ? row(sub + LPP) : first;
     const uint4* src = reinterpret_cast<const uint4*>(rec);
     for (uint32_t t = 0; __any(t < total); ++t) {
         const bool act = t < total;
@@ -700,7 +703,7 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
             q2 = nquad.z;
             q3 = nquad.w;
             const uint32_t j = (t >> 2) + 1;
-            if (j < myq) nquad = row[sub + j * LPP];
+            if (j < myq) nquad = row(sub + j * LPP);
         }
         const uint32_t e = act ? q0 : uint32_t(k);
         q0 = q1;
@@ -756,7 +759,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
     const bool solid = prm.solid_stress != 0;
     __shared__ uint4 stage[(BLOCK / 64) * StageOf<PairRec>::WORDS];
     uint4* my_stage = stage + (threadIdx.x >> 6) * StageOf<PairRec>::WORDS;
-    for_each_neighbor<LPP, true>(rec, list, k, kmax, listed, ln.sub, my_stage, [&](const PairRec& o, uint32_t k2) {
+    for_each_neighbor<LPP, true>(rec, list, k, kmax >> 2, listed, ln.sub, my_stage, [&](const PairRec& o, uint32_t k2) {
         const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
         const double hbar = 0.5 * (me.h + o.h);
         const double hInv = 1.0 / hbar;
@@ -945,7 +948,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
     const uint32_t cnt = ln.valid ? count[k] : 0u;
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
-    for_each_neighbor<LPP, false>(rec, list, k, kmax, listed, ln.sub, nullptr, [&](const PairRecF& o, uint32_t) {
+    for_each_neighbor<LPP, false>(rec, list, k, kmax >> 2, listed, ln.sub, nullptr, [&](const PairRecF& o, uint32_t) {
         const int lxj = int(o.lat & 1023u), lyj = int((o.lat >> 10) & 1023u), lzj = int(o.lat >> 20);
         const float rx = float(lxi - lxj) * L.Sf + (me.dx - o.dx);
         const float ry = float(lyi - lyj) * L.Sf + (me.dy - o.dy);
