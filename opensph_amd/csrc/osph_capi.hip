@@ -74,6 +74,8 @@ struct osph_ctx {
     size_t staging_cap = 0;
     double* scratch = nullptr;  // reductions (integrals)
     size_t scratch_cap = 0;
+    unsigned long long* bounds_part = nullptr;  // per-block partial grid bounds
+    size_t bounds_part_cap = 0;
 
     bool prof = false;
     std::vector<ProfEvent> events;
@@ -397,8 +399,15 @@ int search_begin(osph_ctx* c, int mode, double dt) {
     const double href = c->href;
     unsigned long long* cacc = c->small + SMALL_CLASSES;
     unsigned long long* vacc = c->cfg.precision == OSPH_PRECISION_MIXED ? c->small + SMALL_VEL : nullptr;
+    if (bounds_partials(n) > c->bounds_part_cap) {
+        (void)hipFree(c->bounds_part);
+        c->bounds_part = nullptr;
+        c->bounds_part_cap = 0;
+        HIPCHK(c, hipMalloc(&c->bounds_part, bounds_partials(n) * sizeof(unsigned long long)));
+        c->bounds_part_cap = bounds_partials(n);
+    }
     timed(c, mode == 2 ? "advance" : (mode == 1 ? "material_init" : "class_bounds"),
-          [&] { launch_advance(l, c->s, c->mats_dev, c->prm, mode, dt, href, cacc, vacc, n); });
+          [&] { launch_advance(l, c->s, c->mats_dev, c->prm, mode, dt, href, cacc, vacc, c->bounds_part, n); });
     HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CLASSES, cacc,
                              (SMALL_VEL + 6 - SMALL_CLASSES) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                              c->stream));
@@ -763,6 +772,7 @@ void osph_destroy(osph_ctx* c) {
     (void)hipFree(c->list);
     (void)hipFree(c->staging);
     (void)hipFree(c->scratch);
+    (void)hipFree(c->bounds_part);
     (void)hipFree(c->mats_dev);
     (void)hipFree(c->small);
     (void)hipFree(c->lut);

@@ -189,7 +189,10 @@ void launch_bounds_reduce(const Launch& l, DevState s, unsigned long long* acc, 
 // grid bounds (class of h relative to href) into acc[CLASS_STRIDE * c + 0..8] and, when vel is not null,
 // the velocity bounds into vel[0..5].
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
-                    double href, unsigned long long* acc, unsigned long long* vel, size_t n);
+                    double href, unsigned long long* acc, unsigned long long* vel, unsigned long long* part,
+                    size_t n);
+// Entries of the per-block partial bounds that launch_advance needs for n particles.
+size_t bounds_partials(size_t n);
 void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n);
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value);
 // first[cell] = the first sorted position whose cell index is >= cell, for every cell (x, y, z) of a

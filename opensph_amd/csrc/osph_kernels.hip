@@ -205,87 +205,6 @@ __device__ __forceinline__ int h_class(double h, double hmin, int nclass) {
     return c < 0 ? 0 : (c >= nclass ? nclass - 1 : c);
 }
 
-// Per-class bounds of the neighbour grid, accumulated per thread and reduced per block into
-// acc[CLASS_STRIDE * c + ...]: [0..2] min x, y, z; [3..5] max x, y, z; [6] max h; [7] min h (all
-// order-encoded); [8] count.  Classes of no lane of a wave skip the wave reduction.
-struct ClassAcc {
-    unsigned long long v[MAX_CLASSES][8];
-    unsigned long long cnt[MAX_CLASSES];
-
-    __device__ void init() {
-#pragma unroll
-        for (int c = 0; c < MAX_CLASSES; ++c) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) v[c][k] = (k < 3 || k == 7) ? ~0ull : 0ull;
-            cnt[c] = 0;
-        }
-    }
-
-    __device__ void add(int cls, double x, double y, double z, double h) {
-        const unsigned long long b[4] = { ord_bits(x), ord_bits(y), ord_bits(z), ord_bits(h) };
-#pragma unroll
-        for (int c = 0; c < MAX_CLASSES; ++c) {
-            if (c != cls) continue;  // keeps the register arrays statically indexed
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                v[c][k] = b[k] < v[c][k] ? b[k] : v[c][k];
-                v[c][3 + k] = b[k] > v[c][3 + k] ? b[k] : v[c][3 + k];
-            }
-            v[c][6] = b[3] > v[c][6] ? b[3] : v[c][6];
-            v[c][7] = b[3] < v[c][7] ? b[3] : v[c][7];
-            cnt[c] += 1;
-        }
-    }
-
-    // Whole block: every thread must call it.
-    __device__ void reduce(unsigned long long* acc) {
-        constexpr int NV = 9;
-        __shared__ unsigned long long red[BLOCK / 64][MAX_CLASSES][NV];
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-        for (int c = 0; c < MAX_CLASSES; ++c) {
-            if (__any(cnt[c] != 0)) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const unsigned long long r = (k < 3 || k == 7) ? wave_min(v[c][k]) : wave_max(v[c][k]);
-                    if (lane == 0) red[wave][c][k] = r;
-                }
-                unsigned long long t = cnt[c];
-                for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-                if (lane == 0) red[wave][c][8] = t;
-            } else if (lane == 0) {
-                red[wave][c][8] = 0;
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < MAX_CLASSES * NV) {
-            const int c = threadIdx.x / NV, k = threadIdx.x % NV;
-            unsigned long long r = k == 8 ? 0ull : ((k < 3 || k == 7) ? ~0ull : 0ull);
-            bool any = false;
-            for (int w = 0; w < BLOCK / 64; ++w) {
-                if (red[w][c][8] == 0) continue;
-                any = true;
-                const unsigned long long x = red[w][c][k];
-                if (k == 8) {
-                    r += x;
-                } else {
-                    r = (k < 3 || k == 7) ? (x < r ? x : r) : (x > r ? x : r);
-                }
-            }
-            if (any) {
-                unsigned long long* dst = &acc[CLASS_STRIDE * c + k];
-                if (k == 8) {
-                    atomicAdd(dst, r);
-                } else if (k < 3 || k == 7) {
-                    atomicMin(dst, r);
-                } else {
-                    atomicMax(dst, r);
-                }
-            }
-        }
-    }
-};
-
 __device__ __forceinline__ int cell_coord(const GridClass& g, double x, int k) {
     return int(floor((x - g.lo[k]) * g.inv_cell));
 }
@@ -1275,59 +1194,132 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_correct(DevState s, const os
     reduce_criteria(v, mins);
 }
 
-// One pass before the neighbour search, MODE 2: predictions (predict_at) then the material initialize;
-// MODE 1: the material initialize only.
-template <int MODE>
-__global__ void __launch_bounds__(BLOCK) k_advance(DevState s, const osph_material* __restrict__ mats, Params prm,
-                                                   double dt, size_t n) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const osph_material m = mats[s.u[MAT][i]];
-    if (MODE > 1) predict_at(s, m, prm, dt, i);
-    material_init_at(s, m, prm, i);
+// Per-block partial grid bounds.  Values v = CLASS_VALUES * c + k for class c (k: 0..2 min x, y, z;
+// 3..5 max x, y, z; 6 max h; 7 min h; 8 count, min and max as order-preserving bit patterns), then
+// VEL_VALUES velocity bounds (min x, y, z, max x, y, z).  part[v * nblocks + block]; k_bounds_final
+// reduces them.  Two levels instead of per-block atomics: same-address atomics from thousands of blocks
+// serialise (56 us for 1024 blocks at the impact).
+constexpr int CLASS_VALUES = 9;
+constexpr int VEL_VALUES = 6;
+constexpr int BOUND_VALUES = CLASS_VALUES * MAX_CLASSES + VEL_VALUES;
+
+__device__ __forceinline__ bool bound_is_min(int v) {
+    if (v >= CLASS_VALUES * MAX_CLASSES) return v - CLASS_VALUES * MAX_CLASSES < 3;
+    const int k = v % CLASS_VALUES;
+    return k < 3 || k == 7;
 }
 
-// Per-class grid bounds of the positions, with the class of h relative to href, and with VEL the bounds
-// of the velocities (vel[0..2] min, vel[3..5] max).  A separate pass: the per-class accumulators held
-// across the predictor would cost the streaming pass half its occupancy.
+__device__ __forceinline__ unsigned long long bound_identity(int v) {
+    if (v < CLASS_VALUES * MAX_CLASSES && v % CLASS_VALUES == 8) return 0ull;
+    return bound_is_min(v) ? ~0ull : 0ull;
+}
+
+__device__ __forceinline__ unsigned long long bound_combine(int v, unsigned long long a, unsigned long long b) {
+    if (v < CLASS_VALUES * MAX_CLASSES && v % CLASS_VALUES == 8) return a + b;
+    return bound_is_min(v) ? (b < a ? b : a) : (b > a ? b : a);
+}
+
+// Block-level partial bounds of one particle per thread (cls < 0: no particle).  Every thread of the
+// block must call it.
 template <bool VEL>
-__global__ void __launch_bounds__(BLOCK) k_class_bounds(DevState s, double href, unsigned long long* acc,
-                                                        unsigned long long* vel, size_t n) {
-    ClassAcc ca;
-    ca.init();
-    double vlo[3] = { INFINITY, INFINITY, INFINITY }, vhi[3] = { -INFINITY, -INFINITY, -INFINITY };
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-        const double h = s.d[H][i];
-        ca.add(h_class(h, href, MAX_CLASSES), s.d[X][i], s.d[Y][i], s.d[Z][i], h);
-        if (VEL) {
-            const double v[3] = { s.d[VX][i], s.d[VY][i], s.d[VZ][i] };
-            for (int a = 0; a < 3; ++a) {
-                vlo[a] = dmin(vlo[a], v[a]);
-                vhi[a] = dmax(vhi[a], v[a]);
+__device__ void block_bounds(int cls, double x, double y, double z, double h, const double* v,
+                             unsigned long long* __restrict__ part, uint32_t nblocks) {
+    __shared__ unsigned long long red[BLOCK / 64][BOUND_VALUES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long b[4] = { ord_bits(x), ord_bits(y), ord_bits(z), ord_bits(h) };
+    for (int c = 0; c < MAX_CLASSES; ++c) {
+        const bool in = cls == c;
+        unsigned long long* r = red[wave] + CLASS_VALUES * c;
+        if (__any(in)) {
+            for (int k = 0; k < 3; ++k) {
+                const unsigned long long lo = wave_min(in ? b[k] : ~0ull);
+                const unsigned long long hi = wave_max(in ? b[k] : 0ull);
+                if (lane == 0) {
+                    r[k] = lo;
+                    r[3 + k] = hi;
+                }
+            }
+            const unsigned long long hmax = wave_max(in ? b[3] : 0ull), hmin = wave_min(in ? b[3] : ~0ull);
+            const unsigned long long cnt = __popcll(__ballot(in));
+            if (lane == 0) {
+                r[6] = hmax;
+                r[7] = hmin;
+                r[8] = cnt;
+            }
+        } else if (lane == 0) {
+            for (int k = 0; k < CLASS_VALUES; ++k) r[k] = bound_identity(CLASS_VALUES * c + k);
+        }
+    }
+    if (VEL) {
+        for (int k = 0; k < 3; ++k) {
+            const bool ok = cls >= 0;
+            const unsigned long long bv = ord_bits(v[k]);
+            const unsigned long long lo = wave_min(ok ? bv : ~0ull), hi = wave_max(ok ? bv : 0ull);
+            if (lane == 0) {
+                red[wave][CLASS_VALUES * MAX_CLASSES + k] = lo;
+                red[wave][CLASS_VALUES * MAX_CLASSES + 3 + k] = hi;
             }
         }
     }
-    ca.reduce(acc);
-    if (VEL) {
-        __shared__ unsigned long long red[BLOCK / 64][6];
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        for (int k = 0; k < 6; ++k) {
-            const unsigned long long r = k < 3 ? wave_min(ord_bits(vlo[k])) : wave_max(ord_bits(vhi[k - 3]));
-            if (lane == 0) red[wave][k] = r;
+    __syncthreads();
+    const int nv = VEL ? BOUND_VALUES : CLASS_VALUES * MAX_CLASSES;
+    if (int(threadIdx.x) < nv) {
+        const int vi = threadIdx.x;
+        unsigned long long acc = red[0][vi];
+        for (int w = 1; w < int(blockDim.x / 64); ++w) acc = bound_combine(vi, acc, red[w][vi]);
+        part[size_t(vi) * nblocks + blockIdx.x] = acc;
+    }
+}
+
+// One pass before the neighbour search, MODE 2: predictions (predict_at) then the material initialize;
+// MODE 1: the material initialize only; MODE 0: neither.  Every mode ends with the per-block partial
+// grid bounds of the resulting positions (class of h relative to href) and, with VEL, velocities.
+template <int MODE, bool VEL>
+__global__ void __launch_bounds__(BLOCK) k_advance(DevState s, const osph_material* __restrict__ mats, Params prm,
+                                                   double dt, double href, unsigned long long* __restrict__ part,
+                                                   size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int cls = -1;
+    double x = 0.0, y = 0.0, z = 0.0, h = 0.0, v[3] = { 0.0, 0.0, 0.0 };
+    if (i < n) {
+        if (MODE > 0) {
+            const osph_material m = mats[s.u[MAT][i]];
+            if (MODE > 1) predict_at(s, m, prm, dt, i);
+            material_init_at(s, m, prm, i);
         }
-        __syncthreads();
-        if (threadIdx.x < 6) {
-            const int k = threadIdx.x;
-            unsigned long long r = red[0][k];
-            for (int w = 1; w < int(blockDim.x / 64); ++w) {
-                const unsigned long long x = red[w][k];
-                r = k < 3 ? (x < r ? x : r) : (x > r ? x : r);
-            }
-            if (k < 3) {
-                atomicMin(&vel[k], r);
-            } else {
-                atomicMax(&vel[k], r);
-            }
+        x = s.d[X][i];
+        y = s.d[Y][i];
+        z = s.d[Z][i];
+        h = s.d[H][i];
+        cls = h_class(h, href, MAX_CLASSES);
+        if (VEL) {
+            v[0] = s.d[VX][i];
+            v[1] = s.d[VY][i];
+            v[2] = s.d[VZ][i];
+        }
+    }
+    block_bounds<VEL>(cls, x, y, z, h, v, part, gridDim.x);
+}
+
+// Second level of the bounds: block v reduces value v over the nblocks partials into acc (the class
+// values, at CLASS_STRIDE * c + k) or vel.
+__global__ void __launch_bounds__(BLOCK) k_bounds_final(const unsigned long long* __restrict__ part, uint32_t nblocks,
+                                                        unsigned long long* __restrict__ acc,
+                                                        unsigned long long* __restrict__ vel) {
+    __shared__ unsigned long long red[BLOCK / 64];
+    const int vi = blockIdx.x;
+    unsigned long long r = bound_identity(vi);
+    for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) r = bound_combine(vi, r, part[size_t(vi) * nblocks + b]);
+    for (int o = 32; o > 0; o >>= 1) r = bound_combine(vi, r, __shfl_xor(r, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < int(blockDim.x / 64); ++w) r = bound_combine(vi, r, red[w]);
+        if (vi < CLASS_VALUES * MAX_CLASSES) {
+            acc[CLASS_STRIDE * (vi / CLASS_VALUES) + vi % CLASS_VALUES] = r;
+        } else {
+            vel[vi - CLASS_VALUES * MAX_CLASSES] = r;
         }
     }
 }
@@ -1518,21 +1510,25 @@ void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* 
     k_cell_first<<<grid_for(rows * size_t(xslots)), BLOCK, 0, l.stream>>>(keys, g, row_range, first, rows, xslots);
 }
 
+size_t bounds_partials(size_t n) { return size_t(grid_for(n)) * BOUND_VALUES; }
+
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
-                    double href, unsigned long long* acc, unsigned long long* vel, size_t n) {
+                    double href, unsigned long long* acc, unsigned long long* vel, unsigned long long* part,
+                    size_t n) {
     if (!n) return;
+    const unsigned g = grid_for(n);
+    const bool v = vel != nullptr;
     if (mode == 2) {
-        k_advance<2><<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, n);
+        if (v) k_advance<2, true><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
+        else k_advance<2, false><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
     } else if (mode == 1) {
-        k_advance<1><<<grid_for(n), BLOCK, 0, l.stream>>>(s, mats, prm, dt, n);
-    }
-    unsigned g = grid_for(n);
-    g = g > 1024 ? 1024 : g;
-    if (vel) {
-        k_class_bounds<true><<<g, BLOCK, 0, l.stream>>>(s, href, acc, vel, n);
+        if (v) k_advance<1, true><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
+        else k_advance<1, false><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
     } else {
-        k_class_bounds<false><<<g, BLOCK, 0, l.stream>>>(s, href, acc, vel, n);
+        if (v) k_advance<0, true><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
+        else k_advance<0, false><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
     }
+    k_bounds_final<<<v ? BOUND_VALUES : CLASS_VALUES * MAX_CLASSES, BLOCK, 0, l.stream>>>(part, g, acc, vel);
 }
 
 void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, double4* pos,
