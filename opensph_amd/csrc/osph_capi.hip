@@ -45,6 +45,7 @@ struct osph_ctx {
 
     // neighbour structures (all sized for cap particles)
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *iota = nullptr, *order = nullptr;
+    const char* failed_launch = nullptr;  // name of the first timed launch that left a HIP error
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     uint32_t* cell_first = nullptr;  // [cells + 1]: first sorted position of each cell (valid rows only)
@@ -117,11 +118,13 @@ template <typename F>
 void timed(osph_ctx* c, const char* name, F&& f) {
     if (!c->prof) {
         f();
+        if (!c->failed_launch && hipPeekAtLastError() != hipSuccess) c->failed_launch = name;
         return;
     }
     ProfEvent ev{ name, take_event(c), take_event(c) };
     if (ev.start) (void)hipEventRecord(ev.start, c->stream);
     f();
+    if (!c->failed_launch && hipPeekAtLastError() != hipSuccess) c->failed_launch = name;
     if (ev.stop) (void)hipEventRecord(ev.stop, c->stream);
     c->events.push_back(ev);
 }
@@ -214,7 +217,6 @@ int ensure_rows(osph_ctx* c, size_t rows) {
 int ensure_cells(osph_ctx* c, size_t cells) {
     if (cells + 1 <= c->cell_cap) return OSPH_OK;
     (void)hipFree(c->cell_first);
-    (void)hipFree(c->row_range);
     c->cell_first = nullptr;
     c->cell_cap = 0;
     HIPCHK(c, hipMalloc(&c->cell_first, (cells + 1) * sizeof(uint32_t)));
@@ -350,7 +352,12 @@ void plane_ptrs(osph_ctx* c, const QDesc& d, void** out) {
 }
 
 int sync_check(osph_ctx* c) {
-    HIPCHK(c, hipGetLastError());
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        const std::string where = c->failed_launch ? std::string(" (first failing launch: ") + c->failed_launch + ")" : "";
+        c->failed_launch = nullptr;
+        return c->fail(OSPH_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(le) + where);
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return OSPH_OK;
 }
@@ -482,21 +489,34 @@ int search_finish(osph_ctx* c) {
         int bits = 1;
         while ((1 << bits) <= maxdim) ++bits;
         G.bits = bits;
-        base += uint32_t(1) << (3 * bits);
+        // the last row (largest Morton code of y, z, as row_key is monotonic in each) ends the class
+        const uint64_t rows_end = (uint64_t(spread2(uint32_t(G.dim[1] - 1)) | (uint64_t(spread2(uint32_t(G.dim[2] - 1))) << 1)) + 1)
+                                  << bits;
+        if (uint64_t(base) + rows_end > (uint64_t(1) << 32) - 2) return c->fail(OSPH_ERR_STATE, "cell count exceeds 32 bits");
+        base += uint32_t(rows_end);
     }
-    int maxbits = 1;
-    for (int cl = 0; cl < g.nclass; ++cl) maxbits = std::max(maxbits, g.c[cl].empty ? 1 : g.c[cl].bits);
-    g.shift = 3 * maxbits;
-    g.key_bits = g.shift + (g.nclass > 2 ? 2 : (g.nclass > 1 ? 1 : 0));
-    if (g.key_bits > 32) return c->fail(OSPH_ERR_STATE, "sort key exceeds 32 bits");
+    g.key_bits = 1;
+    while (g.key_bits < 32 && (uint64_t(1) << g.key_bits) < uint64_t(base)) ++g.key_bits;
     c->grid = g;
     const size_t cells = base;
     if (int e = ensure_cells(c, cells)) return e;
 
     timed(c, "cell_keys", [&] { launch_cell_keys(l, c->s, g, c->keys, c->iota, n); });
+    const size_t sort_need = sort_pairs_temp_bytes(n, g.key_bits);
+    if (sort_need > c->sort_temp_bytes) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->sort_temp);
+        c->sort_temp = nullptr;
+        c->sort_temp_bytes = 0;
+        HIPCHK(c, hipMalloc(&c->sort_temp, sort_need));
+        c->sort_temp_bytes = sort_need;
+    }
+    hipError_t sort_err = hipSuccess;
     timed(c, "sort", [&] {
-        sort_pairs(l, c->sort_temp, c->sort_temp_bytes, c->keys, c->keys_sorted, c->iota, c->order, n, g.key_bits);
+        sort_err = sort_pairs(l, c->sort_temp, c->sort_temp_bytes, c->keys, c->keys_sorted, c->iota, c->order, n,
+                              g.key_bits);
     });
+    HIPCHK(c, sort_err);
     size_t rows = 0;
     for (int cl = 0; cl < g.nclass; ++cl) {
         if (!g.c[cl].empty) rows += size_t(g.c[cl].dim[1]) * size_t(g.c[cl].dim[2]);
@@ -815,7 +835,7 @@ int osph_resize(osph_ctx* c, size_t n) {
         HIPCHK(c, hipMalloc(&c->posf, sizeof(float4) * cap));
         HIPCHK(c, hipMalloc(&c->group, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->count, sizeof(uint32_t) * cap));
-        c->sort_temp_bytes = sort_pairs_temp_bytes(cap);
+        c->sort_temp_bytes = sort_pairs_temp_bytes(cap, 32);
         HIPCHK(c, hipMalloc(&c->sort_temp, std::max<size_t>(c->sort_temp_bytes, 256)));
         c->cap = cap;
         for (int k = 0; k < NDPLANES; ++k) c->s.d[k] = c->dblock + size_t(k) * cap;

@@ -75,8 +75,7 @@ struct Lattice {
 };
 
 // Particles are binned into smoothing-length classes, class = min(nclass - 1, floor(log2(h / h_min))).
-// Each class has its own uniform cell grid (cell edge 2 h_max(class) / divisor) keyed by the Morton code
-// of the cell coordinates; the sort key is (class << shift) | morton, shift = 3 * (largest bits per axis).
+// Each class has its own uniform cell grid (cell edge 2 h_max(class) / divisor) keyed by row_key.
 constexpr int MAX_CLASSES = 4;
 
 struct GridClass {
@@ -84,14 +83,16 @@ struct GridClass {
     double hmax;          // largest h in the class
     double cell, inv_cell;
     int dim[3];
-    int bits;             // bits per axis of the Morton key
-    uint32_t base;        // offset of the class's cells in the cell tables
+    int bits;             // 2^bits > every dim: x slots per row, and the width of y and z in the row key
+    uint32_t base;        // offset of the class's cells in the cell tables and in the sort keys
     int empty;
 };
 
+// The sort key of a particle is its cell index: base(class) + row_key(x, y, z), the classes' key ranges
+// packed back to back (each as long as its largest row key + 1), so the keys are dense and as short
+// as the cell count allows (fewer radix-sort passes).
 struct Grid {
     int nclass;
-    int shift;     // bit position of the class field in the sort key
     int key_bits;  // bits of the sort key that can be nonzero
     double hmin;
     GridClass c[MAX_CLASSES];
@@ -235,8 +236,10 @@ void launch_stream_copy(hipStream_t stream, const void* a, void* b, size_t bytes
 // last 12 doubles.
 size_t integrals_scratch_doubles(size_t n);
 void launch_integrals(const Launch& l, DevState s, double* scratch, size_t n);
-size_t sort_pairs_temp_bytes(size_t n);
-void sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits);
+// Temporary storage of the radix sort of n pairs by the low key_bits bits (the algorithm, and with it
+// the storage, depends on both).
+size_t sort_pairs_temp_bytes(size_t n, int key_bits);
+hipError_t sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                      const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits);
 
 }  // namespace osph

@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_keys(DevState s, Grid g, uint32_
     const int cls = h_class(s.d[H][i], g.hmin, g.nclass);
     int c[3];
     cell_of(g.c[cls], s.d[X][i], s.d[Y][i], s.d[Z][i], c);
-    keys[i] = (uint32_t(cls) << g.shift) | row_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]), g.c[cls].bits);
+    keys[i] = g.c[cls].base + row_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]), g.c[cls].bits);
     idx[i] = uint32_t(i);
 }
 
@@ -234,10 +234,8 @@ __global__ void __launch_bounds__(BLOCK) k_fill_u32(uint32_t* p, size_t count, u
     }
 }
 
-// Cell index of sorted position k (monotonic in k: the class bases and row keys increase with the key).
-__device__ __forceinline__ uint32_t cell_index(const Grid& g, uint32_t key) {
-    return g.c[key >> g.shift].base + (key & ((1u << g.shift) - 1u));
-}
+// Cell index of a sort key (the key itself: keys are dense cell indices, monotonic in sorted order).
+__device__ __forceinline__ uint32_t cell_index(const Grid&, uint32_t key) { return key; }
 
 // First sorted position in [lo, hi) whose cell index is >= c (hi when none is).
 __device__ __forceinline__ uint32_t first_at_least(const uint32_t* __restrict__ keys, const Grid& g, uint32_t c,
@@ -1650,18 +1648,18 @@ void launch_iota(const Launch& l, uint32_t* p, size_t n) {
     k_iota_u32<<<grid_for(n), BLOCK, 0, l.stream>>>(p, n);
 }
 
-size_t sort_pairs_temp_bytes(size_t n) {
+size_t sort_pairs_temp_bytes(size_t n, int key_bits) {
     size_t bytes = 0;
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, int(n), 0, 32);
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, int(n), 0, key_bits);
     return bytes;
 }
 
-void sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits) {
-    if (!n) return;
-    (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, int(n), 0,
-                                             key_bits, l.stream);
+hipError_t sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                      const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits) {
+    if (!n) return hipSuccess;
+    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, int(n), 0,
+                                              key_bits, l.stream);
 }
 
 }  // namespace osph

@@ -79,6 +79,24 @@ def test_pairs_dense_cluster_grows_lists(oracle):
     assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
 
 
+def test_pairs_after_grid_tables_grow(oracle):
+    """Searches on one context whose grid grows between them: first more cells with the same rows (the x
+    extent stretched past a power of two of cells), then more rows.  The cell and row tables are
+    reallocated independently; every pair set stays exact."""
+    st, cfg = initial.fluid_sphere(n_requested=2000, radius=500.0)
+    g = gpu.GpuSolver(st, cfg)
+    for stretch in ((1.0, 1.0), (6.0, 1.0), (6.0, 5.0)):
+        s2 = st.copy()
+        s2["r"][:, 0] *= stretch[0]
+        s2["r"][:, 1] *= stretch[1]
+        _print_case(f"grid_growth x{stretch[0]} y{stretch[1]}", s2)
+        g.upload(s2)
+        pi, pj = g.pairs()
+        oi, oj = oracle.pairs(s2["r"])
+        assert len(pi) > 0
+        assert np.array_equal(pi, oi) and np.array_equal(pj, oj)
+
+
 @pytest.mark.parametrize("name", ["fluid10k", "fluid_exercised", "impact_exercised"])
 def test_integrate_matches_oracle(oracle, name):
     st, cfg = _cases()[name]()
