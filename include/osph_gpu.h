@@ -148,7 +148,9 @@ int osph_integrals(osph_ctx* ctx, double out[OSPH_INTEGRALS]);
  * "pairs_lanes" (1, 2 or 4 lanes share one particle's neighbour list in the pair kernels, default 2),
  * "cell_divisor" (1 to 4: neighbour-grid cells of 2 h_max / divisor, default 3), "lists_screen" (1: FP32-screened
  * neighbour test, 0: FP64 only, -1: screened in mixed precision only), "reorder_period" (the resident state
- * is moved into cell order every this many neighbour searches; 0 never). */
+ * is moved into cell order every this many neighbour searches; 0 never), "fuse_finalize" (1: the PC step's
+ * finalize, corrector and criteria run in the pair kernel, 0: as a separate pass, -1: fused for FP64 only,
+ * the default). */
 int osph_tune(osph_ctx* ctx, const char* key, long long value);
 
 /* ---- kernel table and profiling --------------------------------------------------------------------- */

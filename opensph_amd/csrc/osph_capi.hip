@@ -636,16 +636,19 @@ int build_neighbors(osph_ctx* c, int mode) {
     return build_lists(c);
 }
 
-void launch_pair_sums(osph_ctx* c) {
+// The pair sums; with fuse (the PC step), followed in the same kernel by the finalize, the corrector and
+// the criteria (timed then as "<pairs>_finalize_correct").
+void launch_pair_sums(osph_ctx* c, const FuseArgs* fuse = nullptr) {
     Launch l = L(c);
     const size_t n = c->n;
     if (c->cfg.precision == OSPH_PRECISION_MIXED) {
-        timed(c, "pairs_mixed", [&] {
-            launch_pairs_mixed(l, c->s, c->prm, c->lattice, c->recf, c->order, c->list, c->count, c->kmax, n);
+        timed(c, fuse ? "pairs_mixed_finalize_correct" : "pairs_mixed", [&] {
+            launch_pairs_mixed(l, c->s, c->prm, c->lattice, c->recf, c->order, c->list, c->count, c->kmax, n, fuse);
         });
     } else {
-        timed(c, "pairs_fp64", [&] {
-            launch_pairs_fp64(l, c->s, c->prm, c->rec, c->group, c->order, c->list, c->count, c->lut, c->kmax, n);
+        timed(c, fuse ? "pairs_fp64_finalize_correct" : "pairs_fp64", [&] {
+            launch_pairs_fp64(l, c->s, c->prm, c->rec, c->group, c->order, c->list, c->count, c->lut, c->kmax, n,
+                              fuse);
         });
     }
 }
@@ -1003,10 +1006,18 @@ int step_begin(osph_ctx* c, double dt) {
 }
 
 void launch_second_half(osph_ctx* c, double dt) {
-    launch_pair_sums(c);
-    timed(c, "finalize_correct", [&] {
-        launch_finalize_correct(L(c), c->s, c->mats_dev, c->prm, dt, c->small + SMALL_CRITERIA, overflow_flag(c), c->n);
-    });
+    const bool fuse = c->tune.fuse_finalize == 1 ||
+                      (c->tune.fuse_finalize == -1 && c->cfg.precision != OSPH_PRECISION_MIXED);
+    if (fuse) {
+        const FuseArgs fa{ c->mats_dev, dt, c->small + SMALL_CRITERIA, overflow_flag(c) };
+        launch_pair_sums(c, &fa);
+    } else {
+        launch_pair_sums(c);
+        timed(c, "finalize_correct", [&] {
+            launch_finalize_correct(L(c), c->s, c->mats_dev, c->prm, dt, c->small + SMALL_CRITERIA, overflow_flag(c),
+                                    c->n);
+        });
+    }
     (void)hipMemcpyAsync(c->small_host + SMALL_CRITERIA, c->small + SMALL_CRITERIA,
                          (SMALL_OVERFLOW + 1 - SMALL_CRITERIA) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                          c->stream);
@@ -1088,6 +1099,8 @@ int osph_tune(osph_ctx* c, const char* key, long long value) {
         c->tune.lists_screen = int(value);
     } else if (k == "reorder_period" && value >= 0 && value <= 1000000) {
         c->tune.reorder_period = int(value);
+    } else if (k == "fuse_finalize" && (value == -1 || value == 0 || value == 1)) {
+        c->tune.fuse_finalize = int(value);
     } else {
         return c->fail(OSPH_ERR_ARG, "unknown tuning key or value out of range: " + k);
     }

@@ -161,6 +161,8 @@ struct Tuning {
     int cell_divisor = 3;  // cells of 2 h_max(class) / divisor
     int lists_screen = -1; // FP32 screen in the list kernel: 1 on, 0 off, -1 on for mixed precision only
     int reorder_period = 10;  // move the state into cell order every this many searches (0: never)
+    int fuse_finalize = -1;   // PC step: finalize, corrector and criteria in the pair kernel: 1 always, 0 never,
+                              // -1 for FP64 only (measured faster there, slower in mixed precision)
 };
 
 struct Launch {
@@ -206,13 +208,23 @@ void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order,
 void launch_neighbor_lists(const Launch& l, const double4* pos, const float4* posf, Screen sc, Grid g,
                            const uint32_t* first, uint32_t* list, uint32_t* count, uint32_t kmax, uint32_t* overflow,
                            size_t n);
+// With fuse set, the pair kernels go on, per particle, with what k_finalize_correct does (the finalize
+// of every term, the corrector with time step dt and the time-step criteria into mins), without the
+// round trip of the sums through memory; skipped, like k_finalize_correct, while *overflow is set.
+struct FuseArgs {
+    const osph_material* mats;
+    double dt;
+    unsigned long long* mins;
+    const uint32_t* overflow;
+};
 void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
-                       uint32_t kmax, size_t n);
+                       uint32_t kmax, size_t n, const FuseArgs* fuse = nullptr);
 void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
                        double4* pos, float4* posf, Screen sc, uint32_t* group, size_t n);
 void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
-                        const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n);
+                        const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n,
+                        const FuseArgs* fuse = nullptr);
 void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n);
 void launch_zero_derivatives(const Launch& l, DevState s, size_t n);
 // Finalize + corrections + criteria of a predictor-corrector step; a no-op while *overflow != 0.

@@ -551,6 +551,12 @@ struct PairLane {
     }
 };
 
+__device__ void finalize_at(DevState& s, const osph_material& m, const Params& prm, size_t i);
+__device__ void correct_at(DevState& s, const osph_material& m, const Params& prm, double dt, size_t i);
+__device__ void criteria_values(const DevState& s, const osph_material& m, const Params& prm, size_t i, double out[4]);
+__device__ void reduce_criteria(double v[4], unsigned long long* mins);
+
+
 // Per-wavefront LDS staging of neighbour records: 64 records of CH 16-byte chunks each, at an odd stride
 // of chunks (CH | 1) so that the lanes' 16-byte reads of their own record spread over all banks.
 template <typename Rec>
@@ -657,13 +663,14 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
 // The accumulated buffers replace the storage buffers (Accumulated::store, Accumulated.cpp:87-100).
 // LPP lanes share one particle: each sums its share of the list, then the partial sums are added across
 // the group (fewer particles in flight per XCD, so their neighbour records stay in its L2).
-template <int LPP>
+template <int LPP, bool FUSED>
 __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, const PairRec* __restrict__ rec,
                                                       const uint32_t* __restrict__ group,
                                                       const uint32_t* __restrict__ order,
                                                       const uint32_t* __restrict__ list,
                                                       const uint32_t* __restrict__ count,
-                                                      const double* __restrict__ lut, uint32_t kmax, size_t n) {
+                                                      const double* __restrict__ lut, uint32_t kmax, size_t n,
+                                                      FuseArgs fa) {
     const PairLane<LPP> ln(n);
     const size_t k = ln.k;
     const PairRec me = rec[k];
@@ -736,22 +743,31 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
         gxz = group_sum<LPP>(gxz);
         gyz = group_sum<LPP>(gyz);
     }
-    if (!ln.valid || ln.sub != 0) return;
-    const uint32_t i = order[k];
-    s.d[AX][i] = ax;
-    s.d[AY][i] = ay;
-    s.d[AZ][i] = az;
-    s.d[DU][i] = du;
-    s.d[DIVV][i] = divv;
-    if (solid) {
-        s.d[GXX][i] = gxx;
-        s.d[GYY][i] = gyy;
-        s.d[GZZ][i] = gzz;
-        s.d[GXY][i] = gxy;
-        s.d[GXZ][i] = gxz;
-        s.d[GYZ][i] = gyz;
+    double cv[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
+    if (ln.valid && ln.sub == 0) {
+        const uint32_t i = order[k];
+        s.d[AX][i] = ax;
+        s.d[AY][i] = ay;
+        s.d[AZ][i] = az;
+        s.d[DU][i] = du;
+        s.d[DIVV][i] = divv;
+        if (solid) {
+            s.d[GXX][i] = gxx;
+            s.d[GYY][i] = gyy;
+            s.d[GZZ][i] = gzz;
+            s.d[GXY][i] = gxy;
+            s.d[GXZ][i] = gxz;
+            s.d[GYZ][i] = gyz;
+        }
+        s.u[NCNT][i] = cnt;
+        if (FUSED && !*fa.overflow) {
+            const osph_material m = fa.mats[s.u[MAT][i]];
+            finalize_at(s, m, prm, i);
+            correct_at(s, m, prm, fa.dt, i);
+            criteria_values(s, m, prm, i, cv);
+        }
     }
-    s.u[NCNT][i] = cnt;
+    if (FUSED && !*fa.overflow) reduce_criteria(cv, fa.mins);
 }
 
 // ---- mixed precision ---------------------------------------------------------------------------------
@@ -847,12 +863,13 @@ struct Acc {
 // difference (L_j - L_i) S + (d_j - d_i) is exact in its lattice part and rounds the FP32 offsets once, so
 // it keeps FP32 precision relative to the cell edge S, whatever the magnitudes of the absolute values.
 // Divisions are hardware reciprocals (about 1 ulp) times the numerator.
-template <int LPP>
+template <int LPP, bool FUSED>
 __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
                                                        const PairRecF* __restrict__ rec,
                                                        const uint32_t* __restrict__ order,
                                                        const uint32_t* __restrict__ list,
-                                                       const uint32_t* __restrict__ count, uint32_t kmax, size_t n) {
+                                                       const uint32_t* __restrict__ count, uint32_t kmax, size_t n,
+                                                       FuseArgs fa) {
     const PairLane<LPP> ln(n);
     const size_t k = ln.k;
     const PairRecF me = rec[k];
@@ -926,22 +943,31 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
         g[4] = group_sum<LPP>(gxz.sum);
         g[5] = group_sum<LPP>(gyz.sum);
     }
-    if (!ln.valid || ln.sub != 0) return;
-    const uint32_t i = order[k];
-    s.d[AX][i] = sax;
-    s.d[AY][i] = say;
-    s.d[AZ][i] = saz;
-    s.d[DU][i] = sdu;
-    s.d[DIVV][i] = sdivv;
-    if (solid) {
-        s.d[GXX][i] = g[0];
-        s.d[GYY][i] = g[1];
-        s.d[GZZ][i] = g[2];
-        s.d[GXY][i] = g[3];
-        s.d[GXZ][i] = g[4];
-        s.d[GYZ][i] = g[5];
+    double cv[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
+    if (ln.valid && ln.sub == 0) {
+        const uint32_t i = order[k];
+        s.d[AX][i] = sax;
+        s.d[AY][i] = say;
+        s.d[AZ][i] = saz;
+        s.d[DU][i] = sdu;
+        s.d[DIVV][i] = sdivv;
+        if (solid) {
+            s.d[GXX][i] = g[0];
+            s.d[GYY][i] = g[1];
+            s.d[GZZ][i] = g[2];
+            s.d[GXY][i] = g[3];
+            s.d[GXZ][i] = g[4];
+            s.d[GYZ][i] = g[5];
+        }
+        s.u[NCNT][i] = cnt;
+        if (FUSED && !*fa.overflow) {
+            const osph_material m = fa.mats[s.u[MAT][i]];
+            finalize_at(s, m, prm, i);
+            correct_at(s, m, prm, fa.dt, i);
+            criteria_values(s, m, prm, i, cv);
+        }
     }
-    s.u[NCNT][i] = cnt;
+    if (FUSED && !*fa.overflow) reduce_criteria(cv, fa.mins);
 }
 
 // ScalarGradyKippModel::integrate (core/physics/Damage.cpp:128-170) for particle i.
@@ -1128,7 +1154,12 @@ __device__ void reduce_criteria(double v[4], unsigned long long* mins) {
     if (threadIdx.x < 4) {
         double x = red[0][threadIdx.x];
         for (int w = 1; w < BLOCK / 64; ++w) x = dmin(x, red[w][threadIdx.x]);
-        if (x < INFINITY) atomic_min_pos(&mins[threadIdx.x], x);
+        // the atomic only when it would lower the current minimum (a device-coherent read): most blocks
+        // then skip it, where thousands of same-address atomics would serialise
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+        if (x < INFINITY && bits < __hip_atomic_load(&mins[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            atomic_min_pos(&mins[threadIdx.x], x);
+        }
     }
 }
 
@@ -1551,18 +1582,22 @@ void launch_neighbor_lists(const Launch& l, const double4* pos, const float4* po
 
 void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
-                       uint32_t kmax, size_t n) {
+                       uint32_t kmax, size_t n, const FuseArgs* fuse) {
     if (!n) return;
     const int b = l.tune.pairs_block, lpp = l.tune.pairs_lanes;
     const unsigned grid = grid_for(n * size_t(lpp), b);
     const size_t lds = size_t(l.tune.pairs_lds);
+    const FuseArgs fa = fuse ? *fuse : FuseArgs{};
+#define OSPH_PAIRS64(P, F) \
+    k_pairs_fp64<P, F><<<grid, b, lds, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n, fa)
     if (lpp == 4) {
-        k_pairs_fp64<4><<<grid, b, lds, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n);
+        if (fuse) OSPH_PAIRS64(4, true); else OSPH_PAIRS64(4, false);
     } else if (lpp == 2) {
-        k_pairs_fp64<2><<<grid, b, lds, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n);
+        if (fuse) OSPH_PAIRS64(2, true); else OSPH_PAIRS64(2, false);
     } else {
-        k_pairs_fp64<1><<<grid, b, lds, l.stream>>>(s, prm, rec, group, order, list, count, lut_grads, kmax, n);
+        if (fuse) OSPH_PAIRS64(1, true); else OSPH_PAIRS64(1, false);
     }
+#undef OSPH_PAIRS64
 }
 
 void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
@@ -1572,18 +1607,23 @@ void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, con
 }
 
 void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
-                        const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n) {
+                        const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n,
+                        const FuseArgs* fuse) {
     if (!n) return;
     const int b = l.tune.pairs_block, lpp = l.tune.pairs_lanes;
     const unsigned grid = grid_for(n * size_t(lpp), b);
     const size_t lds = size_t(l.tune.pairs_lds);
+    const FuseArgs fa = fuse ? *fuse : FuseArgs{};
+#define OSPH_PAIRSMX(P, F) \
+    k_pairs_mixed<P, F><<<grid, b, lds, l.stream>>>(s, prm, lat, rec, order, list, count, kmax, n, fa)
     if (lpp == 4) {
-        k_pairs_mixed<4><<<grid, b, lds, l.stream>>>(s, prm, lat, rec, order, list, count, kmax, n);
+        if (fuse) OSPH_PAIRSMX(4, true); else OSPH_PAIRSMX(4, false);
     } else if (lpp == 2) {
-        k_pairs_mixed<2><<<grid, b, lds, l.stream>>>(s, prm, lat, rec, order, list, count, kmax, n);
+        if (fuse) OSPH_PAIRSMX(2, true); else OSPH_PAIRSMX(2, false);
     } else {
-        k_pairs_mixed<1><<<grid, b, lds, l.stream>>>(s, prm, lat, rec, order, list, count, kmax, n);
+        if (fuse) OSPH_PAIRSMX(1, true); else OSPH_PAIRSMX(1, false);
     }
+#undef OSPH_PAIRSMX
 }
 
 void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n) {

@@ -122,6 +122,25 @@ def test_predictor_corrector_matches_oracle(oracle, name, steps):
     _assert_close(g.download(st), so, STATE)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_finalize_equals_separate_pass(precision):
+    """The PC step with the finalize, corrector and criteria fused into the pair kernel gives the same bits
+    as with them in their own pass (same arithmetic per particle, same order)."""
+    st, cfg = _cases()["impact_exercised"]()
+    _print_case(f"fused_finalize/{precision}", st)
+    out = []
+    for fuse in (1, 0):
+        g = gpu.GpuSolver(st, cfg, precision=precision)
+        g.tune(fuse_finalize=fuse)
+        log = [g.step() for _ in range(4)]
+        out.append((log, g.download(st)))
+        g.close()
+    (la, a), (lb, b) = out
+    assert la == lb
+    for k in STATE:
+        assert np.array_equal(a[k], b[k]), k
+
+
 @pytest.mark.parametrize("name", ["fluid.npz", "impact.npz"])
 def test_matches_reference_golden(name):
     from test_oracle import _state_from, golden_case
