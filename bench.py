@@ -91,16 +91,24 @@ def run_precision(gpu, st, cfg, prec, warmup, steps, tune=None):
     for _ in range(warmup):
         solver.step()
     solver.sync()
-    solver.profile(True)
+    # the timed steps carry HIP events around the pair-sum launches only (for the roofline); events
+    # around every launch would add a few microseconds between kernels
+    solver.profile(2)
     t0 = time.perf_counter()
     for _ in range(steps):
         solver.step()
     solver.sync()
     elapsed = time.perf_counter() - t0
-    prof = solver.profile_read()
-    solver.profile(False)
+    prof_pairs = solver.profile_read()
     final = solver.download(st)
-    stats = solver.integrate()  # neighbour statistics of the final state (outside the timed region)
+    # per-kernel breakdown from a few more steps, outside the timed region
+    solver.profile(1)
+    for _ in range(min(steps, 10)):
+        solver.step()
+    prof = solver.profile_read()
+    prof.update(prof_pairs)
+    solver.profile(False)
+    stats = solver.integrate()  # neighbour statistics (outside the timed region)
     solver.close()
     return {"value": st.n * steps / elapsed, "elapsed": elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "prof": prof, "stats": stats, "final": final}

@@ -160,8 +160,9 @@ int osph_kernel_lut(double* values, double* grads);
  * re-measure the HBM roof and to calibrate the FETCH_SIZE / WRITE_SIZE counters (no reference
  * counterpart: measurement support). */
 int osph_stream_copy(size_t bytes, int iters, double* gbs);
-/* When enabled, every kernel launch is bracketed by HIP events on the context's stream and its
- * duration accumulated per kernel name. */
+/* enable 1: every kernel launch is bracketed by HIP events on the context's stream and its duration
+ * accumulated per kernel name; 2: only the pair-sum launches (the events of every launch add gaps of a
+ * few microseconds between kernels); 0: off. */
 int osph_profile_enable(osph_ctx* ctx, int enable);
 /* Copies up to cap (name, total ms, launches) records; returns the count in *count. */
 int osph_profile_read(osph_ctx* ctx, char* names, size_t name_cap, double* ms, uint64_t* launches, int cap,

@@ -78,7 +78,7 @@ struct osph_ctx {
     unsigned long long* bounds_part = nullptr;  // per-block partial grid bounds
     size_t bounds_part_cap = 0;
 
-    bool prof = false;
+    int prof = 0;  // 0 off, 1 every launch, 2 pair-sum launches only
     std::vector<ProfEvent> events;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<double, uint64_t>> prof_acc;
@@ -116,7 +116,7 @@ hipEvent_t take_event(osph_ctx* c) {
 // Runs a launcher; with profiling on, brackets it with events on the context's stream.
 template <typename F>
 void timed(osph_ctx* c, const char* name, F&& f) {
-    if (!c->prof) {
+    if (!c->prof || (c->prof == 2 && std::strncmp(name, "pairs", 5) != 0)) {
         f();
         if (!c->failed_launch && hipPeekAtLastError() != hipSuccess) c->failed_launch = name;
         return;
@@ -1116,7 +1116,8 @@ int osph_profile_enable(osph_ctx* c, int enable) {
     if (!c) return OSPH_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     resolve_events(c);
-    c->prof = enable != 0;
+    if (enable < 0 || enable > 2) return c->fail(OSPH_ERR_ARG, "profile mode must be 0, 1 or 2");
+    c->prof = enable;
     c->prof_acc.clear();
     return OSPH_OK;
 }

@@ -276,7 +276,8 @@ class GpuSolver:
     def sync(self):
         self._check(self.L.osph_sync(self.ctx), "osph_sync")
 
-    def profile(self, enable: bool = True):
+    def profile(self, enable=True):
+        """True / 1: time every launch; 2: the pair-sum launches only; False / 0: off."""
         self._check(self.L.osph_profile_enable(self.ctx, int(enable)), "osph_profile_enable")
 
     def profile_read(self) -> Dict[str, Tuple[float, int]]:
