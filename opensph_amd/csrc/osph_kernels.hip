@@ -8,7 +8,7 @@
 #include "osph_device.h"
 #include "osph_internal.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace osph {
 
@@ -1688,18 +1688,29 @@ void launch_iota(const Launch& l, uint32_t* p, size_t n) {
     k_iota_u32<<<grid_for(n), BLOCK, 0, l.stream>>>(p, n);
 }
 
+// Radix sort of the cell keys: rocPRIM's onesweep with launch shapes measured at the 1M impact (24-bit
+// keys): 1024 threads x 4 items per block 84 us per sort, against 119 us for its default of 1024 x 16
+// (65 blocks per pass on 256 CUs); 512 x 8: 95 us, 256 x 8: 124 us.  12-bit digits exceed the LDS.
+constexpr unsigned SORT_BLOCK = 1024, SORT_ITEMS = 4, SORT_BITS = 8;
+using SortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<SORT_BLOCK, SORT_ITEMS>,
+                                        rocprim::kernel_config<SORT_BLOCK, SORT_ITEMS>, SORT_BITS,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
 size_t sort_pairs_temp_bytes(size_t n, int key_bits) {
     size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, int(n), 0, key_bits);
+    (void)rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u,
+                                                unsigned(key_bits));
     return bytes;
 }
 
 hipError_t sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
                       const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits) {
     if (!n) return hipSuccess;
-    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, int(n), 0,
-                                              key_bits, l.stream);
+    return rocprim::radix_sort_pairs<SortConfig>(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0u,
+                                                 unsigned(key_bits), l.stream);
 }
 
 }  // namespace osph
