@@ -557,6 +557,11 @@ __device__ void criteria_values(const DevState& s, const osph_material& m, const
 __device__ void reduce_criteria(double v[4], unsigned long long* mins);
 
 
+// How the pair kernels gather neighbour records (for_each_neighbor): 0 directly, 1 through the LDS stage,
+// 2 through the LDS stage with the next step's loads issued before the current step's compute.
+constexpr int PAIRS64_STAGE = 1;
+constexpr int PAIRSMX_STAGE = 0;
+
 // Per-wavefront LDS staging of neighbour records: 64 records of CH 16-byte chunks each, at an odd stride
 // of chunks (CH | 1) so that the lanes' 16-byte reads of their own record spread over all banks.
 template <typename Rec>
@@ -571,16 +576,17 @@ struct StageOf {
 // takes quads sub, sub + LPP, ..., in order, the next quad loaded one quad ahead.  f(record, entry) runs
 // for each listed entry of the lane.
 //
-// STAGED: the records are fetched cooperatively.  In each step the wavefront needs one record per lane;
+// STAGED (1, 2): the records are fetched cooperatively.  In each step the wavefront needs one record per lane;
 // pass i of the CH load passes has lane L fetch chunk c of the record of lane r, (r, c) = divmod(64 i +
 // L, CH), into the wavefront's LDS stage, and each lane then reads its own record back.  Consecutive
 // lanes fetch consecutive chunks of one record, so a load instruction touches ~64 / CH records' cache
 // lines instead of one line per lane: CH times fewer line accesses in the texture path, for a longer
 // dependency chain (shuffle, load, LDS write, LDS read).  It pays for the 128-B FP64 record (1.16 ->
 // 1.06 ms at the impact) and not for the 80-B mixed one (0.45 -> 0.51 ms), which gathers directly.
+// With STAGED == 2 the next step's loads are issued before the current step's compute.
 // Every lane of the wavefront must call this (lanes without entries load their own record as a
 // stand-in, and run no f).
-template <int LPP, bool STAGED, typename Rec, typename F>
+template <int LPP, int STAGED, typename Rec, typename F>
 __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, const uint32_t* __restrict__ list,
                                                   size_t k, uint32_t quads, uint32_t listed, uint32_t sub,
                                                   uint4* __restrict__ stage, F&& f) {
@@ -612,13 +618,13 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
         }
         return;
     }
-    // (in the staged loop the step index is uniform across the wavefront, so the entry is taken from
+    // (in the staged loops the step index is uniform across the wavefront, so the entry is taken from
     // rotating registers rather than selected by index, which the compiler would place in scratch)
     const uint4 first = myq > 0 ? row(sub) : make_uint4(0, 0, 0, 0);
     uint32_t q0 = first.x, q1 = first.y, q2 = first.z, q3 = first.w;  // entries of the current quad, next first
     uint4 nquad = myq > 1 ? row(sub + LPP) : first;
-    const uint4* src = reinterpret_cast<const uint4*>(rec);
-    for (uint32_t t = 0; __any(t < total); ++t) {
+    // the entry of step t for this lane (its own index as a stand-in when it has none); steps in order
+    auto entry = [&](uint32_t t) -> uint32_t {
         const bool act = t < total;
         if (act && (t & 3u) == 0 && t > 0) {
             q0 = nquad.x;
@@ -632,13 +638,45 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
         q0 = q1;
         q1 = q2;
         q2 = q3;
-        wave_sync();  // the previous step's reads of the stage come first
+        return e;
+    };
+    const uint4* src = reinterpret_cast<const uint4*>(rec);
+    // pass i: this lane's chunk of the records of every lane's entry e
+    uint4 v[S::CH];
+    auto fetch = [&](uint32_t e) {
 #pragma unroll
         for (uint32_t i = 0; i < S::CH; ++i) {
             const uint32_t q = i * 64u + lane;
             const uint32_t r = q / S::CH, c = q - r * S::CH;
             const uint32_t er = uint32_t(__shfl(int(e), int(r), 64));
-            stage[r * S::STRIDE + c] = src[size_t(er) * S::CH + c];
+            v[i] = src[size_t(er) * S::CH + c];
+        }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (uint32_t i = 0; i < S::CH; ++i) {
+            const uint32_t q = i * 64u + lane;
+            const uint32_t r = q / S::CH, c = q - r * S::CH;
+            stage[r * S::STRIDE + c] = v[i];
+        }
+    };
+    uint32_t e = entry(0);
+    if constexpr (STAGED == 2) fetch(e);
+    for (uint32_t t = 0; __any(t < total); ++t) {
+        const bool act = t < total;
+        const uint32_t ecur = e;
+        wave_sync();  // the previous step's reads of the stage come first
+        if constexpr (STAGED == 2) {
+            put();
+            // the next step's loads leave before this step's compute (pipelined)
+            if (__any(t + 1 < total)) {
+                e = entry(t + 1);
+                fetch(e);
+            }
+        } else {
+            fetch(ecur);
+            put();
+            if (__any(t + 1 < total)) e = entry(t + 1);
         }
         wave_sync();
         union {
@@ -647,7 +685,7 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
         } o;
 #pragma unroll
         for (uint32_t c = 0; c < S::CH; ++c) o.v[c] = stage[lane * S::STRIDE + c];
-        if (act) f(o.r, e);
+        if (act) f(o.r, ecur);
     }
 }
 
@@ -681,9 +719,9 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
     const uint32_t cnt = ln.valid ? count[k] : 0u;
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
-    __shared__ uint4 stage[(BLOCK / 64) * StageOf<PairRec>::WORDS];
-    uint4* my_stage = stage + (threadIdx.x >> 6) * StageOf<PairRec>::WORDS;
-    for_each_neighbor<LPP, true>(rec, list, k, kmax >> 2, listed, ln.sub, my_stage, [&](const PairRec& o, uint32_t k2) {
+    __shared__ uint4 stage[PAIRS64_STAGE ? (BLOCK / 64) * StageOf<PairRec>::WORDS : 1];
+    uint4* my_stage = PAIRS64_STAGE ? stage + (threadIdx.x >> 6) * StageOf<PairRec>::WORDS : stage;
+    for_each_neighbor<LPP, PAIRS64_STAGE>(rec, list, k, kmax >> 2, listed, ln.sub, my_stage, [&](const PairRec& o, uint32_t k2) {
         const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
         const double hbar = 0.5 * (me.h + o.h);
         const double hInv = 1.0 / hbar;
@@ -882,7 +920,10 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
     const uint32_t cnt = ln.valid ? count[k] : 0u;
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
-    for_each_neighbor<LPP, false>(rec, list, k, kmax >> 2, listed, ln.sub, nullptr, [&](const PairRecF& o, uint32_t) {
+    // the LDS stage exists whenever the gather mode uses it (one uint4 otherwise)
+    __shared__ uint4 stage[PAIRSMX_STAGE ? (BLOCK / 64) * StageOf<PairRecF>::WORDS : 1];
+    uint4* my_stage = PAIRSMX_STAGE ? stage + (threadIdx.x >> 6) * StageOf<PairRecF>::WORDS : stage;
+    for_each_neighbor<LPP, PAIRSMX_STAGE>(rec, list, k, kmax >> 2, listed, ln.sub, my_stage, [&](const PairRecF& o, uint32_t) {
         const int lxj = int(o.lat & 1023u), lyj = int((o.lat >> 10) & 1023u), lzj = int(o.lat >> 20);
         const float rx = float(lxi - lxj) * L.Sf + (me.dx - o.dx);
         const float ry = float(lyi - lyj) * L.Sf + (me.dy - o.dy);
