@@ -142,6 +142,26 @@ int osph_sync(osph_ctx* ctx);
  * out = { E, E_kin, E_int, p x y z, L x y z, com x y z, M }, FP64 with a fixed reduction order. */
 enum { OSPH_INTEGRALS = 13 };
 int osph_integrals(osph_ctx* ctx, double out[OSPH_INTEGRALS]);
+
+/* ---- reverse mode (no reference counterpart: OpenSPH has no derivatives of its runs) ---------------- */
+/* Gradients of a function of the final state of a PredictorCorrector run with the gas terms
+ * (PressureForce + StandardAV + ContinuityEquation, Tillotson, constant h; the PC of
+ * core/timestepping/TimeStepping.cpp:286-346) with respect to its initial state and the Tillotson
+ * parameters a, b, A.  osph_adjoint_forward runs `steps` PC steps with the given time steps, keeping a
+ * checkpoint of r, v, a, rho, drho, u, du every `checkpoint_every` steps; the resident state ends at the
+ * final state.  osph_adjoint_backward replays each segment from its checkpoint and runs the reverse
+ * steps: seeds are dJ/d(final r, v) (n x 3, caller order) and dJ/d(final rho, u) (n); each may be null
+ * (zero).  Outputs (each may be null): dJ/d(initial r, v) (n x 3), dJ/d(initial rho, u) (n) and
+ * grad_eos[3 * material + q] = dJ/d(a, b, A).  The time steps are held fixed and the neighbour sets are
+ * those of the replay (identical to the forward's).  *clamped counts particle-steps whose predicted
+ * density or energy sat on its range bound, where the clamped map's derivative is not taken (expect 0).
+ * Afterwards the resident state is the initial state again.  While a forward run is recorded the state is
+ * not reordered; osph_adjoint_clear drops the checkpoints and restores the reorder setting. */
+int osph_adjoint_forward(osph_ctx* ctx, int steps, const double* dt, int checkpoint_every);
+int osph_adjoint_backward(osph_ctx* ctx, const double* seed_r, const double* seed_v, const double* seed_rho,
+                          const double* seed_u, double* grad_r, double* grad_v, double* grad_rho, double* grad_u,
+                          double* grad_eos, unsigned* clamped);
+int osph_adjoint_clear(osph_ctx* ctx);
 /* Launch-shape tuning of the neighbour-list and pair kernels (no reference counterpart: the reference
  * has no device kernels).  Keys: "pairs_block", "lists_block" (64, 128 or 256 threads) and
  * "pairs_lds", "lists_lds" (0..65536 bytes of dynamic LDS per block, which caps resident blocks per CU),

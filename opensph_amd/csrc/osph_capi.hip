@@ -70,6 +70,14 @@ struct osph_ctx {
     Screen screen{};
     double href = 0.0;  // class reference h of the next neighbour search (0: measure it first)
     bool neighbors_valid = false;
+    // reverse-mode run (osph_adjoint_forward / _backward)
+    struct {
+        bool ready = false;
+        int steps = 0, every = 0, saved_reorder = 0;
+        std::vector<double> dt;
+        double* ckpt = nullptr;  // ceil(steps / every) blocks of 13 planes of cap doubles
+        size_t ckpt_blocks = 0, ckpt_cap = 0;
+    } adj;
 
     unsigned char* staging = nullptr;
     size_t staging_cap = 0;
@@ -350,6 +358,10 @@ void plane_ptrs(osph_ctx* c, const QDesc& d, void** out) {
         out[k] = d.u32 ? static_cast<void*>(c->s.u[d.planes[k]]) : static_cast<void*>(c->s.d[d.planes[k]]);
     }
 }
+
+// Adjoint plane indices (the order launch_adj_step expects): lambda of r xyz, v xyz, a xyz, rho, drho,
+// u, du.
+enum { LRX_ = 0, LVX_ = 3, LAX_ = 6, LRHO_ = 9, LDRHO_ = 10, LU_ = 11, LDU_ = 12 };
 
 int sync_check(osph_ctx* c) {
     const hipError_t le = hipGetLastError();
@@ -697,6 +709,13 @@ void select_timestep(const osph_ctx* c, const unsigned long long* mins, double* 
     if (crit) *crit = id;
 }
 
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    ~DevBuf() { (void)hipFree(p); }
+    hipError_t alloc(size_t count) { return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)); }
+};
+
 }  // namespace
 
 extern "C" {
@@ -796,6 +815,7 @@ void osph_destroy(osph_ctx* c) {
     (void)hipFree(c->staging);
     (void)hipFree(c->scratch);
     (void)hipFree(c->bounds_part);
+    (void)hipFree(c->adj.ckpt);
     (void)hipFree(c->mats_dev);
     (void)hipFree(c->small);
     (void)hipFree(c->lut);
@@ -1137,6 +1157,166 @@ int osph_profile_read(osph_ctx* c, char* names, size_t name_cap, double* ms, uin
         ++k;
     }
     *count = k;
+    return OSPH_OK;
+}
+
+// ---- reverse mode ------------------------------------------------------------------------------------
+
+// One PC step with a given time step (the criterion's proposal is ignored).
+static int step_fixed(osph_ctx* c, double dt) {
+    if (int e = step_begin(c, dt)) return e;
+    if (int e = step_middle(c, dt)) return e;
+    return step_end(c, dt, nullptr, nullptr);
+}
+
+// The prediction of a step with time step dt and the derivatives at the predicted state (lists, pair sums
+// and finalize, no correction): what the reverse step differentiates.
+static int predict_and_evaluate(osph_ctx* c, double dt) {
+    if (int e = step_begin(c, dt)) return e;
+    if (int e = search_finish(c)) return e;
+    if (int e = build_lists(c)) return e;
+    launch_pair_sums(c);
+    timed(c, "finalize", [&] { launch_finalize(L(c), c->s, c->mats_dev, c->prm, c->n); });
+    return OSPH_OK;
+}
+
+int osph_adjoint_forward(osph_ctx* c, int steps, const double* dt, int checkpoint_every) {
+    if (!c || steps < 1 || !dt || checkpoint_every < 1) return OSPH_ERR_ARG;
+    if (c->cfg.solid_stress || c->cfg.has_damage || c->cfg.adaptive_h) {
+        return c->fail(OSPH_ERR_UNSUPPORTED,
+                       "the adjoint covers the gas terms: no solid stress, damage or adaptive smoothing length");
+    }
+    if (c->mats.size() > size_t(ADJ_MAX_MATERIALS)) return c->fail(OSPH_ERR_UNSUPPORTED, "too many materials");
+    for (int k = 0; k < steps; ++k) {
+        if (!(dt[k] > 0.0) || !std::isfinite(dt[k])) return c->fail(OSPH_ERR_ARG, "time steps must be positive");
+    }
+    const size_t n = c->n;
+    if (!c->adj.ready) c->adj.saved_reorder = c->tune.reorder_period;
+    c->tune.reorder_period = 0;  // the slot order stays fixed between the checkpoints and the replays
+    c->adj.ready = false;
+    const size_t blocks = size_t((steps + checkpoint_every - 1) / checkpoint_every);
+    if (blocks > c->adj.ckpt_blocks || c->cap > c->adj.ckpt_cap) {
+        (void)hipFree(c->adj.ckpt);
+        c->adj.ckpt = nullptr;
+        c->adj.ckpt_blocks = 0;
+        HIPCHK(c, hipMalloc(&c->adj.ckpt, blocks * 13 * std::max<size_t>(c->cap, 1) * sizeof(double)));
+        c->adj.ckpt_blocks = blocks;
+        c->adj.ckpt_cap = c->cap;
+    }
+    const size_t block_doubles = 13 * c->adj.ckpt_cap;
+    for (int k = 0; k < steps; ++k) {
+        if (k % checkpoint_every == 0) {
+            launch_save_x(L(c), c->s, c->adj.ckpt + size_t(k / checkpoint_every) * block_doubles, c->adj.ckpt_cap, n);
+        }
+        if (int e = step_fixed(c, dt[k])) return e;
+    }
+    c->adj.steps = steps;
+    c->adj.every = checkpoint_every;
+    c->adj.dt.assign(dt, dt + steps);
+    c->adj.ready = true;
+    return sync_check(c);
+}
+
+int osph_adjoint_backward(osph_ctx* c, const double* seed_r, const double* seed_v, const double* seed_rho,
+                          const double* seed_u, double* grad_r, double* grad_v, double* grad_rho, double* grad_u,
+                          double* grad_eos, unsigned* clamped) {
+    if (!c) return OSPH_ERR_ARG;
+    if (!c->adj.ready) return c->fail(OSPH_ERR_STATE, "no forward run recorded (osph_adjoint_forward)");
+    const size_t n = c->n, cap = c->adj.ckpt_cap;
+    const int steps = c->adj.steps, every = c->adj.every;
+    if (cap < n) return c->fail(OSPH_ERR_STATE, "the particle count changed since the forward run");
+    Launch l = L(c);
+    DevBuf<double> lam, gsc, seg, io, part, theta;
+    DevBuf<unsigned char> rec;
+    DevBuf<unsigned> clamp_count;
+    // destroyed before the buffers: no queued kernel still uses them when they are freed (early returns)
+    struct StreamIdle {
+        hipStream_t s;
+        ~StreamIdle() { (void)hipStreamSynchronize(s); }
+    } idle{ c->stream };
+    HIPCHK(c, lam.alloc(13 * cap));
+    HIPCHK(c, gsc.alloc(9 * cap));
+    HIPCHK(c, seg.alloc(size_t(every) * 13 * cap));
+    HIPCHK(c, io.alloc(3 * std::max<size_t>(n, 1)));
+    HIPCHK(c, part.alloc(adj_partials(n)));
+    HIPCHK(c, theta.alloc(ADJ_MAX_MATERIALS * 3));
+    HIPCHK(c, rec.alloc(adj_rec_bytes() * std::max<size_t>(n, 1)));
+    HIPCHK(c, clamp_count.alloc(1));
+    double* lp[13];
+    double* gp[9];
+    for (int k = 0; k < 13; ++k) lp[k] = lam.p + size_t(k) * cap;
+    for (int k = 0; k < 9; ++k) gp[k] = gsc.p + size_t(k) * cap;
+    launch_clear_u32(l, reinterpret_cast<uint32_t*>(theta.p), 2 * ADJ_MAX_MATERIALS * 3, 0u);
+    launch_clear_u32(l, clamp_count.p, 1, 0u);
+    // seeds: lambda of r and v at the final state (caller order, 3 per particle), rho and u (1 each);
+    // the derivative planes start at zero
+    auto seed = [&](const double* host, int nc, double* const* planes) -> int {
+        if (host && n) HIPCHK(c, hipMemcpyAsync(io.p, host, n * nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        launch_scatter_planes(l, host ? io.p : nullptr, nc, planes, c->slot_of, n);
+        return sync_check(c);  // io is reused
+    };
+    if (int e = seed(seed_r, 3, lp + LRX_)) return e;
+    if (int e = seed(seed_v, 3, lp + LVX_)) return e;
+    if (int e = seed(nullptr, 3, lp + LAX_)) return e;
+    if (int e = seed(seed_rho, 1, lp + LRHO_)) return e;
+    if (int e = seed(nullptr, 1, lp + LDRHO_)) return e;
+    if (int e = seed(seed_u, 1, lp + LU_)) return e;
+    if (int e = seed(nullptr, 1, lp + LDU_)) return e;
+
+    const size_t block_doubles = 13 * cap;
+    const int nblocks = (steps + every - 1) / every;
+    for (int b = nblocks - 1; b >= 0; --b) {
+        const int s0 = b * every, s1 = std::min(s0 + every, steps);
+        // replay the segment from its checkpoint, keeping the state before each step
+        launch_restore_x(l, c->s, c->adj.ckpt + size_t(b) * block_doubles, cap, n);
+        launch_save_x(l, c->s, seg.p, cap, n);
+        for (int k = s0; k < s1 - 1; ++k) {
+            if (int e = step_fixed(c, c->adj.dt[k])) return e;
+            launch_save_x(l, c->s, seg.p + size_t(k + 1 - s0) * block_doubles, cap, n);
+        }
+        for (int k = s1 - 1; k >= s0; --k) {
+            launch_restore_x(l, c->s, seg.p + size_t(k - s0) * block_doubles, cap, n);
+            if (int e = predict_and_evaluate(c, c->adj.dt[k])) return e;
+            timed(c, "adjoint", [&] {
+                launch_adj_step(l, c->s, lp, gp, c->order, c->list, c->count, c->kmax, c->lut, c->cfg.av_alpha,
+                                c->cfg.av_beta, c->mats_dev, int(c->mats.size()), c->adj.dt[k], rec.p, part.p,
+                                theta.p, clamp_count.p, n);
+            });
+        }
+    }
+    // gradients with respect to the initial state (caller order)
+    auto out = [&](double* host, int nc, double* const* planes) -> int {
+        if (!host || !n) return OSPH_OK;
+        launch_gather_planes(l, io.p, nc, planes, c->slot_of, n);
+        HIPCHK(c, hipMemcpyAsync(host, io.p, n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        return sync_check(c);
+    };
+    if (int e = out(grad_r, 3, lp + LRX_)) return e;
+    if (int e = out(grad_v, 3, lp + LVX_)) return e;
+    if (int e = out(grad_rho, 1, lp + LRHO_)) return e;
+    if (int e = out(grad_u, 1, lp + LU_)) return e;
+    if (grad_eos) {
+        HIPCHK(c, hipMemcpyAsync(grad_eos, theta.p, c->mats.size() * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    unsigned nclamp = 0;
+    HIPCHK(c, hipMemcpyAsync(&nclamp, clamp_count.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    // the resident state goes back to the initial state of the forward run
+    launch_restore_x(l, c->s, c->adj.ckpt, cap, n);
+    c->neighbors_valid = false;
+    if (int e = sync_check(c)) return e;
+    if (clamped) *clamped = nclamp;
+    return OSPH_OK;
+}
+
+int osph_adjoint_clear(osph_ctx* c) {
+    if (!c) return OSPH_ERR_ARG;
+    if (c->adj.ready) c->tune.reorder_period = c->adj.saved_reorder;
+    c->adj.ready = false;
+    (void)hipFree(c->adj.ckpt);
+    c->adj.ckpt = nullptr;
+    c->adj.ckpt_blocks = 0;
+    c->adj.ckpt_cap = 0;
     return OSPH_OK;
 }
 

@@ -42,10 +42,11 @@ struct alignas(16) PairRec {
 };
 static_assert(sizeof(PairRec) == 128, "PairRec must be 128 bytes");
 
-// Mixed-precision neighbour record, 96 B.  Positions are split into an integer lattice cell (10 bits
+// Mixed-precision neighbour record, 80 B.  Positions are split into an integer lattice cell (10 bits
 // per axis, packed in `lat`) of power-of-two edge S and an FP32 offset inside that cell, so that
 // r_i - r_j = (L_i - L_j) S + (d_i - d_j) keeps FP32 relative precision at any distance from the origin.
-// Velocities stay FP64: bodies move at km/s with m/s internal differences, which FP32 would round away.
+// Velocities get the same split on a second lattice (`vlat`, edge Sv): bodies move at km/s with m/s
+// internal differences, which absolute FP32 velocities would round away.
 struct alignas(16) PairRecF {
     float dx, dy, dz, h;     // position relative to its lattice cell, smoothing length
     float ux, uy, uz, m;     // velocity relative to its velocity-lattice cell, mass
@@ -196,6 +197,29 @@ void launch_advance(const Launch& l, DevState s, const osph_material* mats, Para
                     size_t n);
 // Entries of the per-block partial bounds that launch_advance needs for n particles.
 size_t bounds_partials(size_t n);
+
+// ---- adjoint (osph_adjoint.hip) -----------------------------------------------------------------------
+// Materials whose Tillotson parameters (a, b, A) get a gradient.
+constexpr int ADJ_MAX_MATERIALS = 4;
+// Bytes of one adjoint neighbour record (n of them, in sorted order).
+size_t adj_rec_bytes();
+// Doubles of the per-block partials of the parameter gradient for n particles.
+size_t adj_partials(size_t n);
+// One reverse PC step: lplanes (13: lambda of r xyz, v xyz, a xyz, rho, drho, u, du) hold the adjoints of
+// X_{n+1} and receive those of X_n; the state holds the predicted X* with the derivatives evaluated at it
+// (and its neighbour lists); gplanes (9) are scratch; theta[3 mat + q] accumulates d/d(a, b, A).
+void launch_adj_step(const Launch& l, DevState s, double* const* lplanes, double* const* gplanes,
+                     const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax,
+                     const double* lut, double alpha, double beta, const osph_material* mats, int nmat, double dt,
+                     void* rec, double* part, double* theta, unsigned* clamped, size_t n);
+// The 13 planes of X (r, v, a, rho, drho, u, du) to and from a block of 13 planes of capacity cap.
+void launch_save_x(const Launch& l, DevState s, double* block, size_t cap, size_t n);
+void launch_restore_x(const Launch& l, DevState s, const double* block, size_t cap, size_t n);
+// Device arrays in the caller's order (nc doubles per element; src null: zeros) to and from planes.
+void launch_scatter_planes(const Launch& l, const double* src, int nc, double* const* planes, const uint32_t* slot_of,
+                           size_t n);
+void launch_gather_planes(const Launch& l, double* dst, int nc, double* const* planes, const uint32_t* slot_of,
+                          size_t n);
 void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n);
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value);
 // first[cell] = the first sorted position whose cell index is >= cell, for every cell (x, y, z) of a

@@ -29,7 +29,8 @@ EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info",
            "osph_set_quantity", "osph_get_quantity", "osph_zero_derivatives", "osph_build_neighbors",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
            "osph_stream_copy", "osph_tune", "osph_integrals", "osph_step_pc_batch",
-           "osph_profile_enable", "osph_profile_read"]
+           "osph_profile_enable", "osph_profile_read", "osph_adjoint_forward", "osph_adjoint_backward",
+           "osph_adjoint_clear"]
 
 
 class Material(C.Structure):
@@ -85,6 +86,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_tune.argtypes = [vp, C.c_char_p, C.c_longlong]
     L.osph_integrals.argtypes = [vp, dp]
     L.osph_step_pc_batch.argtypes = [C.POINTER(vp), i, dp, dp, C.POINTER(i)]
+    L.osph_adjoint_forward.argtypes = [vp, i, dp, i]
+    L.osph_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_uint)]
+    L.osph_adjoint_clear.argtypes = [vp]
     L.osph_profile_enable.argtypes = [vp, i]
     L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
     _lib = L
@@ -256,6 +260,37 @@ class GpuSolver:
         self._check(self.L.osph_integrals(self.ctx, out.ctypes.data_as(C.POINTER(C.c_double))), "osph_integrals")
         return {"energy": out[0], "kinetic_energy": out[1], "internal_energy": out[2], "momentum": out[3:6],
                 "angular_momentum": out[6:9], "center_of_mass": out[9:12], "mass": out[12]}
+
+    def adjoint_forward(self, dt, checkpoint_every: int = 10):
+        """Runs len(dt) PC steps with the given time steps, checkpointing every `checkpoint_every` steps for
+        a later adjoint_backward (osph_adjoint_forward)."""
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        self._check(self.L.osph_adjoint_forward(self.ctx, int(dt.size), dt.ctypes.data_as(C.POINTER(C.c_double)),
+                                                int(checkpoint_every)), "osph_adjoint_forward")
+
+    def adjoint_backward(self, seed_r=None, seed_v=None, seed_rho=None, seed_u=None) -> Dict[str, object]:
+        """Gradients of J with respect to the initial r, v, rho, u and the Tillotson (a, b, A) of each
+        material, given dJ/d(final r, v, rho, u) (osph_adjoint_backward).  The resident state returns to the
+        initial state."""
+        n = self.n
+
+        def arr(a, shape):
+            return None if a is None else np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float64)
+
+        sr, sv = arr(seed_r, (n, 3)), arr(seed_v, (n, 3))
+        srho, su = arr(seed_rho, (n,)), arr(seed_u, (n,))
+        out = {"r": np.zeros((n, 3)), "v": np.zeros((n, 3)), "rho": np.zeros(n), "u": np.zeros(n),
+               "eos": np.zeros((len(self.materials), 3))}
+        clamped = C.c_uint()
+        ptr = lambda a: None if a is None else a.ctypes.data
+        self._check(self.L.osph_adjoint_backward(self.ctx, ptr(sr), ptr(sv), ptr(srho), ptr(su), ptr(out["r"]),
+                                                 ptr(out["v"]), ptr(out["rho"]), ptr(out["u"]), ptr(out["eos"]),
+                                                 C.byref(clamped)), "osph_adjoint_backward")
+        out["clamped"] = int(clamped.value)
+        return out
+
+    def adjoint_clear(self):
+        self._check(self.L.osph_adjoint_clear(self.ctx), "osph_adjoint_clear")
 
     def pairs(self) -> Tuple[np.ndarray, np.ndarray]:
         self._check(self.L.osph_build_neighbors(self.ctx), "osph_build_neighbors")

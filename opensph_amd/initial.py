@@ -196,6 +196,40 @@ def fluid_sphere(n_requested: int = 10000, radius: float = 1000.0, sigma_v: floa
     return st, cfg
 
 
+def gas_collision(n_total: int = 500_000, radius: float = 1000.0, ratio: float = 0.5, speed: float = 100.0,
+                  u0: float = 1.0e3, gap_h: float = 0.5, sigma_v: float = 1.0,
+                  seed: int = 5) -> Tuple[ParticleState, RunConfig]:
+    """Config (the adjoint run): two basalt spheres of radii R and ratio R at equal particle spacing,
+    PressureForce + StandardAV + ContinuityEquation, Tillotson, no stress, damage or plasticity
+    (EosMaterial), ConstSmoothingLength, Courant criterion.  The smaller body approaches the larger one
+    along -x at `speed`, its surface `gap_h` smoothing lengths from the other's.  Both start at specific
+    internal energy u0 > 0, away from the energy range's lower bound 0, where the clamp of the integrator
+    has no derivative.  A seeded Gaussian velocity perturbation (sigma_v per component) keeps the pairs
+    off the StandardAV switch (v_i - v_j).(r_i - r_j) = 0, where a rigid body's pairs would all sit and
+    the run has only one-sided derivatives."""
+    mat = basalt()
+    n_t = max(1, int(round(n_total / (1.0 + ratio ** 3))))
+    n_p = max(1, n_total - n_t)
+    rt, mt = sphere_body(n_t, radius, mat)
+    rp, mp = sphere_body(n_p, radius * ratio, mat)
+    h = rt[0, 3]
+    # place the smaller body on +x, its surface gap_h h away from the larger body's surface
+    ext_t = float(np.max(np.sqrt(np.sum(rt[:, :3] ** 2, axis=1))))
+    ext_p = float(np.max(np.sqrt(np.sum(rp[:, :3] ** 2, axis=1))))
+    rp[:, 0] += ext_t + ext_p + gap_h * h
+    nt, npj = rt.shape[0], rp.shape[0]
+    st = ParticleState(nt + npj, [mat])
+    _fill_body(st, slice(0, nt), rt, mt, 0, 0, mat)
+    _fill_body(st, slice(nt, nt + npj), rp, mp, 1, 0, mat)
+    st["v"][nt:, 0] = -speed
+    if sigma_v > 0.0:
+        st["v"][:, :3] += np.random.default_rng(seed).normal(0.0, sigma_v, size=(st.n, 3))
+    st["u"][:] = u0
+    st.materials[0]["rho_lo"] = density_lower_bound(st["r"], st["m"], mat)
+    cfg = RunConfig(solid_stress=0, adaptive_h=0, has_reduce=0, has_damage=0, criteria=1)
+    return st, cfg
+
+
 def impact(n_target: int = 950_000, n_projectile: int = 50_000, r_target: float = 50e3, r_projectile: float = 5e3,
            speed: float = 5e3, angle_deg: float = 45.0, offset_h: float = 4.0, seed: int = 1234,
            flaws: bool = True) -> Tuple[ParticleState, RunConfig]:

@@ -183,6 +183,18 @@ class Oracle:
         return out, log
 
 
+    def run_pc_fixed(self, st, cfg, dts):
+        """PredictorCorrector steps with the given time steps (the criterion's proposals ignored)."""
+        P, I = to_planes(st)
+        c = make_config(cfg, st.materials)
+        crit = C.c_int(0)
+        for dt in dts:
+            self.lib.orc_step_pc(C.byref(c), _ptr(P), _ptr(I, C.c_uint32), st.n, float(dt), C.byref(crit))
+        out = st.copy()
+        from_planes(out, P, I)
+        return out
+
+
 class RefRun:
     """The real OpenSPH SymmetricSolver<3> + PredictorCorrector on a copy of the given state."""
 
