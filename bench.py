@@ -252,6 +252,63 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint(args):
+    """Gradient of the outer kinetic energy of a two-sphere gas collision (PressureForce + StandardAV +
+    ContinuityEquation, Tillotson) with respect to the initial state and the Tillotson a, b, A: forward
+    run with checkpoints, then the checkpointed reverse sweep (one replay plus one reverse step per step)."""
+    from opensph_amd import gpu, initial
+
+    t0 = time.perf_counter()
+    st, cfg = initial.gas_collision(n_total=args.adjoint_n, radius=1000.0, speed=100.0)
+    log(f"gas collision of {st.n} particles set up in {time.perf_counter() - t0:.1f}s")
+    steps, every = args.adjoint_steps, args.adjoint_every
+    out = {}
+    for prec in ("fp64", "mixed"):
+        g = gpu.GpuSolver(st, cfg, precision=prec)
+        g.integrate()  # the sound speed the Courant criterion needs
+        dt = 0.5 * g.timestep()[0]
+        g.upload(st)  # back to the initial state (zero derivatives)
+        dts = [dt] * steps
+        g.adjoint_forward(dts[:2], 1)  # warm-up (allocations, kernel loads)
+        g.adjoint_backward(seed_v=np.zeros((st.n, 3)))
+        g.sync()
+        t0 = time.perf_counter()
+        g.adjoint_forward(dts, every)
+        g.sync()
+        fwd = time.perf_counter() - t0
+        fin = g.download(st)
+        r = fin["r"][:, :3]
+        R = float(np.max(np.linalg.norm(st["r"][:, :3] - st["r"][:, :3].mean(axis=0), axis=1)))
+        com = np.sum(fin["m"][:, None] * r, axis=0) / np.sum(fin["m"])
+        S = np.linalg.norm(r - com, axis=1) > 0.5 * R
+        seed_v = np.where(S[:, None], st["m"][:, None] * fin["v"][:, :3], 0.0)
+        t0 = time.perf_counter()
+        grads = g.adjoint_backward(seed_v=seed_v)
+        g.sync()
+        bwd = time.perf_counter() - t0
+        g.close()
+        out[prec] = {"forward_s": fwd, "backward_s": bwd, "grads": grads, "dt": dt, "outer": int(S.sum())}
+    a, b = out["mixed"]["grads"], out["fp64"]["grads"]
+    rel = {k: float(np.linalg.norm(a[k] - b[k]) / np.linalg.norm(b[k])) for k in ("v", "r", "rho", "u")}
+    name, _ = gpu.device_info()
+    main_run = out["mixed"]
+    total = main_run["forward_s"] + main_run["backward_s"]
+    res = {"metric": "particle-steps/s, gas-collision gradient (forward + checkpointed reverse sweep)",
+           "value": st.n * steps / total, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": 2,
+           "ms_per_step": 1e3 * total / steps, "higher_is_better": True, "dtype": "f32+f64",
+           "data": "synthetic",
+           "config": {"workload": "two basalt spheres R 1 km and 0.5 km, 100 m/s, PressureForce + StandardAV + "
+                                  "ContinuityEquation, Tillotson, constant h, dt fixed at half the Courant step",
+                      "particles": st.n, "checkpoint_every": every, "device": name},
+           "objective": f"kinetic energy of the {main_run['outer']} particles beyond R/2 of the centre of mass",
+           "forward_s": {p: out[p]["forward_s"] for p in out}, "backward_s": {p: out[p]["backward_s"] for p in out},
+           "fp64_value": st.n * steps / (out["fp64"]["forward_s"] + out["fp64"]["backward_s"]),
+           "mixed_vs_fp64_gradient_rel_l2": rel, "clamped": {p: out[p]["grads"]["clamped"] for p in out},
+           "dJ_dtheta_mixed": out["mixed"]["grads"]["eos"][0].tolist(),
+           "dJ_dtheta_fp64": out["fp64"]["grads"]["eos"][0].tolist()}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,9 +321,12 @@ def main():
                     help="launch-shape setting key=value passed to osph_tune (e.g. pairs_lds=40960)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the 1 GiB device copy (HBM roof, FETCH_SIZE/WRITE_SIZE calibration)")
-    ap.add_argument("--workload", default="impact", choices=["impact", "nbody", "ensemble"],
+    ap.add_argument("--workload", default="impact", choices=["impact", "nbody", "ensemble", "adjoint"],
                     help="impact: the SPH headline; nbody: self-gravity Barnes-Hut + LeapFrog; "
-                         "ensemble: batched impact ensemble")
+                         "ensemble: batched impact ensemble; adjoint: gas-collision gradient")
+    ap.add_argument("--adjoint-n", type=int, default=500_000)
+    ap.add_argument("--adjoint-steps", type=int, default=50)
+    ap.add_argument("--adjoint-every", type=int, default=10)
     ap.add_argument("--nbody-n", type=int, default=8_000_000)
     ap.add_argument("--ensemble-n", type=int, default=64)
     args = ap.parse_args()
@@ -280,6 +340,8 @@ def main():
     if args.workload == "nbody":
         run_nbody(args)
         return
+    if args.workload == "adjoint":
+        return run_adjoint(args)
     if args.workload == "ensemble":
         run_ensemble(args)
         return
