@@ -85,10 +85,16 @@ def total_energy(st):
     return float(np.sum(st["m"] * (0.5 * v2 + st["u"])))
 
 
-def run_precision(gpu, st, cfg, prec, warmup, steps, tune=None):
+def run_precision(gpu, st, cfg, prec, warmup, steps, tune=None, tuner=False):
     solver = gpu.GpuSolver(st, cfg, precision=prec)
     if tune:
         solver.tune(**tune)
+    tuned = None
+    if tuner:
+        # launch shapes picked by the autotuner on this state (it leaves the state as it found it)
+        from opensph_amd import autotune
+        tuned = autotune.autotune(solver, steps=10, log=log)
+        log(f"{prec}: autotuned {tuned['settings']} ({tuned['ms_per_step']:.3f} ms/step)")
     for _ in range(warmup):
         solver.step()
     solver.sync()
@@ -112,7 +118,7 @@ def run_precision(gpu, st, cfg, prec, warmup, steps, tune=None):
     stats = solver.integrate()  # neighbour statistics (outside the timed region)
     solver.close()
     return {"value": st.n * steps / elapsed, "elapsed": elapsed, "ms_per_step": 1e3 * elapsed / steps,
-            "prof": prof, "stats": stats, "final": final}
+            "prof": prof, "stats": stats, "final": final, "tuned": tuned["settings"] if tuned else None}
 
 
 def compare_precisions(gpu, st, cfg, runs):
@@ -319,6 +325,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="append", default=[],
                     help="launch-shape setting key=value passed to osph_tune (e.g. pairs_lds=40960)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch shapes with opensph_amd.autotune before the warmup")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the 1 GiB device copy (HBM roof, FETCH_SIZE/WRITE_SIZE calibration)")
     ap.add_argument("--workload", default="impact", choices=["impact", "nbody", "ensemble", "adjoint"],
@@ -360,7 +368,7 @@ def main():
     runs = {}
     for prec in precisions:
         tune = dict((kv.split("=")[0], int(kv.split("=")[1])) for kv in args.tune)
-        runs[prec] = run_precision(gpu, st, cfg, prec, args.warmup, args.steps, tune)
+        runs[prec] = run_precision(gpu, st, cfg, prec, args.warmup, args.steps, tune, args.autotune)
         log(f"{prec}: {runs[prec]['value']:.4g} particle-steps/s, {runs[prec]['ms_per_step']:.3f} ms/step")
     main_prec = precisions[-1]
     r = runs[main_prec]
@@ -412,6 +420,7 @@ def main():
                                     if "fp64" in runs else None),
         "mixed_over_fp64": (runs["mixed"]["value"] / runs["fp64"]["value"]) if len(runs) == 2 else None,
         "accuracy": accuracy,
+        "autotuned": {p: runs[p]["tuned"] for p in runs} if args.autotune else None,
         "roofline": roofline,
         "cpu_baseline": baseline,
     }

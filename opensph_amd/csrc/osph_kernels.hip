@@ -1193,8 +1193,10 @@ __device__ void reduce_criteria(double v[4], unsigned long long* mins) {
     }
     __syncthreads();
     if (threadIdx.x < 4) {
+        // the pair kernels run with blocks of 64, 128 or 256 threads: only the block's own waves hold values
+        const int waves = int(blockDim.x >> 6);
         double x = red[0][threadIdx.x];
-        for (int w = 1; w < BLOCK / 64; ++w) x = dmin(x, red[w][threadIdx.x]);
+        for (int w = 1; w < waves; ++w) x = dmin(x, red[w][threadIdx.x]);
         // the atomic only when it would lower the current minimum (a device-coherent read): most blocks
         // then skip it, where thousands of same-address atomics would serialise
         const unsigned long long bits = (unsigned long long)__double_as_longlong(x);

@@ -122,16 +122,17 @@ def test_predictor_corrector_matches_oracle(oracle, name, steps):
     _assert_close(g.download(st), so, STATE)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_fused_finalize_equals_separate_pass(precision):
+@pytest.mark.parametrize("precision,block", [("fp64", 256), ("mixed", 256), ("fp64", 128), ("mixed", 64)])
+def test_fused_finalize_equals_separate_pass(precision, block):
     """The PC step with the finalize, corrector and criteria fused into the pair kernel gives the same bits
-    as with them in their own pass (same arithmetic per particle, same order)."""
+    as with them in their own pass (same arithmetic per particle, same order), for every pair-kernel block
+    size (the fused criteria minimum is reduced over the block's own waves)."""
     st, cfg = _cases()["impact_exercised"]()
-    _print_case(f"fused_finalize/{precision}", st)
+    _print_case(f"fused_finalize/{precision}/block{block}", st)
     out = []
     for fuse in (1, 0):
         g = gpu.GpuSolver(st, cfg, precision=precision)
-        g.tune(fuse_finalize=fuse)
+        g.tune(fuse_finalize=fuse, pairs_block=block if fuse else 256)
         log = [g.step() for _ in range(4)]
         out.append((log, g.download(st)))
         g.close()
