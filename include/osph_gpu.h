@@ -165,7 +165,7 @@ int osph_adjoint_clear(osph_ctx* ctx);
 /* Launch-shape tuning of the neighbour-list and pair kernels (no reference counterpart: the reference
  * has no device kernels).  Keys: "pairs_block", "lists_block" (64, 128 or 256 threads) and
  * "pairs_lds", "lists_lds" (0..65536 bytes of dynamic LDS per block, which caps resident blocks per CU),
- * "pairs_lanes" (1, 2 or 4 lanes share one particle's neighbour list in the pair kernels, default 2),
+ * "pairs_lanes" (1, 2 or 4 lanes share one particle's neighbour list in the pair kernels, default 1),
  * "cell_divisor" (1 to 4: neighbour-grid cells of 2 h_max / divisor, default 3), "lists_screen" (1: FP32-screened
  * neighbour test, 0: FP64 only, -1: screened in mixed precision only), "reorder_period" (the resident state
  * is moved into cell order every this many neighbour searches; 0 never), "fuse_finalize" (1: the PC step's

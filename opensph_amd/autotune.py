@@ -19,7 +19,7 @@ from typing import Dict, Iterable, List, Sequence, Tuple
 # key -> candidate values, in the order the descent visits them (the first value of each list is the
 # library's default)
 DEFAULT_SPACE: List[Tuple[str, Sequence[int]]] = [
-    ("pairs_lanes", (2, 1, 4)),
+    ("pairs_lanes", (1, 2, 4)),
     ("cell_divisor", (3, 2, 4)),
     ("lists_block", (256, 128)),
     ("pairs_block", (256, 128)),

@@ -156,7 +156,7 @@ __host__ __device__ inline size_t list_quad(size_t k, uint32_t j, uint32_t quads
 struct Tuning {
     int pairs_block = 256;
     int pairs_lds = 0;
-    int pairs_lanes = 2;  // lanes per particle in the pair kernels (1, 2 or 4)
+    int pairs_lanes = 1;  // lanes per particle in the pair kernels (1, 2 or 4)
     int lists_block = 256;
     int lists_lds = 0;
     int cell_divisor = 3;  // cells of 2 h_max(class) / divisor

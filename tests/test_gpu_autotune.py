@@ -7,7 +7,7 @@ from opensph_amd import autotune, gpu
 
 pytestmark = pytest.mark.gpu
 
-SPACE = [("pairs_lanes", (2, 1, 4)), ("cell_divisor", (3, 2)), ("fuse_finalize", (-1, 1))]
+SPACE = [("pairs_lanes", (1, 2, 4)), ("cell_divisor", (3, 2)), ("fuse_finalize", (-1, 1))]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

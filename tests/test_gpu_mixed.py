@@ -18,16 +18,18 @@ def total_energy(st):
     return float(np.sum(st["m"] * (0.5 * v2 + st["u"])))
 
 
-@pytest.mark.parametrize("name", ["impact_exercised", "fluid_exercised"])
-def test_mixed_derivatives_match_fp64(name):
+@pytest.mark.parametrize("name,lanes", [("impact_exercised", 1), ("fluid_exercised", 1), ("impact_exercised", 2),
+                                        ("impact_exercised", 4)])
+def test_mixed_derivatives_match_fp64(name, lanes):
     from cases import exercised_fluid, exercised_impact
 
     st, cfg = {"impact_exercised": exercised_impact, "fluid_exercised": exercised_fluid}[name]()
-    print(f"[case] mixed {name} n={st.n}", flush=True)
+    print(f"[case] mixed {name} lanes={lanes} n={st.n}", flush=True)
     a = gpu.GpuSolver(st, cfg, precision="fp64")
     a.integrate()
     da = a.download(st)
     b = gpu.GpuSolver(st, cfg, precision="mixed")
+    b.tune(pairs_lanes=lanes)
     b.integrate()
     db = b.download(st)
     assert np.array_equal(da["ncnt"], db["ncnt"])

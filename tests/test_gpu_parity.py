@@ -109,6 +109,21 @@ def test_integrate_matches_oracle(oracle, name):
     assert np.array_equal(a["ncnt"], b["ncnt"])
 
 
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_integrate_lane_split_matches_oracle(oracle, lanes):
+    """Two and four lanes sharing each particle's neighbour list in the pair kernel (the default is one):
+    FP64 derivatives within the parity bar of the oracle, neighbour counts exact."""
+    st, cfg = _cases()["impact_exercised"]()
+    _print_case(f"impact_exercised/lanes{lanes}", st)
+    g = gpu.GpuSolver(st, cfg)
+    g.tune(pairs_lanes=lanes)
+    g.integrate()
+    a = g.download(st)
+    b = oracle.integrate(st, cfg)
+    _assert_close(a, b, DERIV)
+    assert np.array_equal(a["ncnt"], b["ncnt"])
+
+
 @pytest.mark.parametrize("name,steps", [("fluid10k", 20), ("impact_exercised", 6)])
 def test_predictor_corrector_matches_oracle(oracle, name, steps):
     st, cfg = _cases()[name]()
