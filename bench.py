@@ -154,13 +154,37 @@ def nbody_state(n, seed=7):
 
 
 def run_nbody(args):
-    """Self-gravity N-body: Barnes-Hut (theta 0.5, quadrupole, leaf 25, M4-softened) + LeapFrog, dt = 1 s."""
+    """Self-gravity N-body: Barnes-Hut (theta 0.5, quadrupole, leaf 25, M4-softened) + LeapFrog, dt = 1 s.
+    --precision both (the default) runs FP64 and then the FP32 evaluation from the same start and reports
+    the mixed run, with the FP64 run and the acceleration difference beside it."""
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    runs = {p: nbody_once(args, p) for p in precs}
+    out = runs[precs[-1]]
+    if len(runs) == 2:
+        a64, amx = runs["fp64"].pop("_dv0"), runs["mixed"].pop("_dv0")
+        scale = np.linalg.norm(a64, axis=1).max()
+        out["fp64_value"] = runs["fp64"]["value"]
+        out["fp64_kernel_ms_per_step"] = runs["fp64"]["kernel_ms_per_step"]
+        out["fp64_roofline"] = runs["fp64"]["roofline"]
+        out["mixed_over_fp64"] = out["value"] / runs["fp64"]["value"]
+        out["mixed_vs_fp64_first_eval"] = {
+            "err_over_max_max": float(np.linalg.norm(amx - a64, axis=1).max() / scale),
+            "rel_l2": float(np.linalg.norm(amx - a64) / np.linalg.norm(a64))}
+    out.pop("_dv0", None)
+    print(json.dumps(out), flush=True)
+
+
+def nbody_once(args, prec):
     from opensph_amd import gpu
     from opensph_amd.gravity import GpuGravity
 
     n = args.nbody_n
     r, m = nbody_state(n)
-    g = GpuGravity(r, m, solver="barnes_hut", kernel="m4", theta=0.5, order=2, leaf_size=25)
+    g = GpuGravity(r, m, solver="barnes_hut", kernel="m4", theta=0.5, order=2, leaf_size=25, precision=prec)
+    # the accelerations of the first evaluation, for the FP32 / FP64 comparison (untimed)
+    g.build()
+    dv0, _ = g.eval_self()
+    g.set_particles(r, m)
     g.leapfrog(1.0, args.warmup)
     g.sync()
     g.profile(True)
@@ -180,27 +204,32 @@ def run_nbody(args):
     g.build()
     g.profile(True)
     _, st = g.eval_self()
-    eval_ms = g.profile_read().get("eval", (float("nan"), 1))[0]
+    kname = "eval_mixed" if prec == "mixed" else "eval"
+    eval_ms = g.profile_read().get(kname, (float("nan"), 1))[0]
     g.profile(False)
+    g.close()
     name, cus = gpu.device_info()
+    peak = FP32_PEAK_TFLOPS if prec == "mixed" else FP64_PEAK_TFLOPS
     out = {"metric": "particle-steps/s, 8M N-body Barnes-Hut + LeapFrog", "value": n * args.steps / elapsed,
            "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "dtype": "f64",
-           "data": "synthetic",
+           "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+           "dtype": "f32+f64" if prec == "mixed" else "f64", "data": "synthetic",
            "config": {"workload": "uniform sphere R 100 km, equal masses, M4 softening h 10 m, theta 0.5, "
-                                  "quadrupole, leaf 25, LeapFrog dt 1 s", "particles": n, "device": name},
+                                  "quadrupole, leaf 25, LeapFrog dt 1 s", "particles": n, "precision": prec,
+                      "device": name},
            "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in sorted(prof.items())},
            "accuracy": {"sample": int(sample.size), "rel_err_median": float(np.median(rel)),
                         "rel_err_max": float(rel.max()),
                         "err_over_rms_max": float(np.max(np.linalg.norm(bh - direct, axis=1)) / rms)},
            "interactions": {"nodes_approx": int(st["nodes_approx"]), "leaves_exact": int(st["nodes_exact"]),
                             "expansions": int(st["expansions"]), "pairs": int(st["pairs"]), "eval_ms": eval_ms},
-           "roofline": {"bound": "mfma", "kernel": "eval",
+           "roofline": {"bound": "mfma", "kernel": kname,
                         "achieved": (FLOP_PER_GRAVITY_PAIR * st["pairs"] + FLOP_PER_QUADRUPOLE * st["expansions"])
                         / (eval_ms * 1e-3) / 1e12,
-                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "traffic": None}}
-    out["roofline"]["frac"] = out["roofline"]["achieved"] / FP64_PEAK_TFLOPS
-    print(json.dumps(out), flush=True)
+                        "peak": peak, "unit": "TFLOP/s", "traffic": None},
+           "_dv0": dv0}
+    out["roofline"]["frac"] = out["roofline"]["achieved"] / peak
+    return out
 
 
 def ensemble_members(count, seed=11):

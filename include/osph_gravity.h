@@ -6,6 +6,7 @@
  *
  *   osph_gravity_create                     <- Factory::getGravity (core/system/Factory.cpp:361-413):
  *                                              BarnesHut or BruteForceGravity with a GravityLutKernel
+ *   osph_gravity_set_precision              (none: FP32 evaluation of the interaction lists)
  *   osph_gravity_set_particles              <- the POSITION / MASS buffers that IGravity::build reads
  *   osph_gravity_build                      <- IGravity::build (BarnesHut.cpp:50-84, BruteForceGravity.h:44-47)
  *   osph_gravity_eval_self                  <- IGravity::evalSelfGravity (BarnesHut.cpp:86-99,212-312;
@@ -64,6 +65,14 @@ typedef struct {
 int osph_gravity_create(osph_gravity** g, const osph_gravity_config* cfg);
 void osph_gravity_destroy(osph_gravity* g);
 const char* osph_gravity_last_error(const osph_gravity* g);
+
+/* Arithmetic of the Barnes-Hut evaluation (no reference counterpart: the reference evaluates in FP64).
+ * OSPH_PRECISION_FP64 (the default) follows the reference's FP64 sums.  OSPH_PRECISION_MIXED evaluates
+ * the interaction lists in FP32 on coordinates relative to each target leaf, with lengths and G m
+ * scaled by powers of two (the root box extent and the total G m), and adds each batch of up to 64
+ * terms to FP64 accumulators; the tree, its moments, the walk and the integration stay FP64.  The
+ * brute-force solver, evaluations at points and the direct sums are FP64 either way. */
+int osph_gravity_set_precision(osph_gravity* g, int precision);
 
 /* Particles in the reference's element layouts: r and v are Vectors (x, y, z, h; stride bytes, 0 = 32),
  * m doubles.  v may be NULL (zero velocities).  Accelerations start at zero. */

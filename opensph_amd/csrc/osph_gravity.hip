@@ -730,6 +730,269 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
     }
 }
 
+// ---- mixed-precision evaluation ---------------------------------------------------------------------
+
+// Largest value over the wavefront (every lane gets it).
+__device__ __forceinline__ float wave_max_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Powers of two that bring lengths and G m into FP32's comfortable range: lengths are divided by
+// 2^e with the root box extent in [2^(e-1), 2^e), G m by 2^f with the total G m in [2^(f-1), 2^f).
+// Both divisions are exact in FP64; the results are scaled back by 2^(f - 2e).
+struct EvalScale {
+    double invL, invM, back;
+};
+
+// A wave-uniform double kept in scalar registers.
+__device__ __forceinline__ double uniform_d(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(b)), hi = __builtin_amdgcn_readfirstlane(uint32_t(b >> 32));
+    return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
+}
+
+__device__ __forceinline__ EvalScale eval_scale(const Tree& t) {
+    EvalScale sc{ 1.0, 1.0, 1.0 };
+    if (t.n < 2) return sc;
+    const double4 lo = t.lo[0], hi = t.hi[0];
+    const double ext = fmax(fmax(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+    int e = 0, f = 0;
+    if (ext > 0.0) (void)frexp(ext, &e);
+    const double M = t.mom[0].M;
+    if (M > 0.0) (void)frexp(M, &f);
+    sc.invL = uniform_d(ldexp(1.0, -e));
+    sc.invM = uniform_d(ldexp(1.0, -f));
+    sc.back = uniform_d(ldexp(1.0, f - 2 * e));
+    return sc;
+}
+
+// The multipole expansion of add_multipole in FP32, written so that no intermediate leaves FP32's range:
+// with d the vector from the target to the node's centre of mass, il = 1/|d|, u = d il and w = il^2,
+//   monopole   il^3 M d                                  = w M u
+//   quadrupole il^4 (7.5 u (u.q2.u) - 3 q2.u)            = w^2 t2
+//   octupole   il^5 (7.5 (q3:uu) - 17.5 u (q3:uuu))      = w^2 il t3
+// and an approximated node has size < theta |d|, so w |q2| < theta^2 M and w il |q3| < theta^3 M.
+template <int ORDER>
+__device__ __forceinline__ void add_multipole_f(float M, const float* q2, const float* q3, float dx, float dy, float dz,
+                                                float a[3]) {
+    const float il = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+    const float w = il * il;
+    const float u0 = dx * il, u1 = dy * il, u2 = dz * il;
+    float hx = 0.f, hy = 0.f, hz = 0.f;
+    if (ORDER >= 2) {
+        const float xx = q2[0], xy = q2[1], xz = q2[2], yy = q2[3], yz = q2[4], zz = -xx - yy;
+        const float Q1x = xx * u0 + xy * u1 + xz * u2;
+        const float Q1y = xy * u0 + yy * u1 + yz * u2;
+        const float Q1z = xz * u0 + yz * u1 + zz * u2;
+        const float Q0 = 7.5f * (Q1x * u0 + Q1y * u1 + Q1z * u2);
+        hx = Q0 * u0 - 3.f * Q1x;
+        hy = Q0 * u1 - 3.f * Q1y;
+        hz = Q0 * u2 - 3.f * Q1z;
+    }
+    if (ORDER >= 3) {
+        const float xxx = q3[0], xxy = q3[1], xxz = q3[2], xyy = q3[3], xyz = q3[4], yyy = q3[5], yyz = q3[6];
+        const float xzz = -xxx - xyy, yzz = -xxy - yyy, zzz = -xxz - yyz;
+        const float Q1x = xxx * u0 * u0 + xyy * u1 * u1 + xzz * u2 * u2 + 2.f * (xxy * u0 * u1 + xxz * u0 * u2 + xyz * u1 * u2);
+        const float Q1y = xxy * u0 * u0 + yyy * u1 * u1 + yzz * u2 * u2 + 2.f * (xyy * u0 * u1 + xyz * u0 * u2 + yyz * u1 * u2);
+        const float Q1z = xxz * u0 * u0 + yyz * u1 * u1 + zzz * u2 * u2 + 2.f * (xyz * u0 * u1 + xzz * u0 * u2 + yzz * u1 * u2);
+        const float Q0 = 17.5f * (Q1x * u0 + Q1y * u1 + Q1z * u2);
+        hx += il * (7.5f * Q1x - Q0 * u0);
+        hy += il * (7.5f * Q1y - Q0 * u1);
+        hz += il * (7.5f * Q1z - Q0 * u2);
+    }
+    a[0] += w * (M * u0 + w * hx);
+    a[1] += w * (M * u1 + w * hy);
+    a[2] += w * (M * u2 + w * hz);
+}
+
+// add_pair<false> in FP32 on scaled coordinates (r = source - target, h and m scaled).  The LUT index is
+// clamped: an FP32 q^2 may round up to 4 where the FP64 test admitted it.
+__device__ __forceinline__ void add_pair_f(const double* __restrict__ grads, float rx, float ry, float rz, float h,
+                                           float m, float a[3]) {
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    if (grads && d2 < 4.f * (h * h)) {
+        const float hInv = __builtin_amdgcn_rcpf(h);
+        const float qSqr = d2 * (hInv * hInv);
+        const float fi = float(LUT_ENTRIES / 4) * qSqr;
+        const uint32_t i1 = min(uint32_t(fi), uint32_t(LUT_ENTRIES - 1));
+        const float ratio = fi - float(i1);
+        const float gr = float(grads[i1]) * (1.f - ratio) + float(grads[i1 + 1]) * ratio;
+        const float f = m * (hInv * hInv * hInv) * gr;
+        a[0] += rx * f;
+        a[1] += ry * f;
+        a[2] += rz * f;
+        return;
+    }
+    const float il = __builtin_amdgcn_rsqf(d2);
+    const float f = m * (il * il * il);
+    a[0] += rx * f;
+    a[1] += ry * f;
+    a[2] += rz * f;
+}
+
+// Per-wavefront staging area of k_eval_mixed: the FP32 half of EvalStage.
+struct EvalStageF {
+    float c[3][64];
+    float M[64];
+    float q2[5][64];
+    float q3[7][64];
+    float4 src[64];
+    float sm[64];
+    uint32_t sidx[64];
+};
+
+// k_eval with FP32 arithmetic on offsets from the first particle of the target leaf (R), lengths and
+// G m scaled by eval_scale.  A pair separation taken as the difference of two FP32 offsets is off by at
+// most 2^-23 O, with O the largest offset among the batch's sources and the wavefront's targets: pairs
+// closer than O / 8 (where that is more than 1e-6 of the separation) take theirs in FP64 from the
+// source's FP64 position instead.  The lanes, lists, splits and the order of the staged entries are
+// those of k_eval.  Each batch of up to 64 staged entries is summed in FP32 and added to FP64
+// accumulators; the accelerations are scaled back and added to the state planes in FP64.
+template <int ORDER>
+__global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const uint32_t* __restrict__ order,
+                                                      const uint32_t* __restrict__ targets, uint32_t ntargets,
+                                                      const uint32_t* __restrict__ nlist, uint32_t cap_n,
+                                                      const uint32_t* __restrict__ xlist, uint32_t cap_x,
+                                                      const uint32_t* __restrict__ ncount,
+                                                      const uint32_t* __restrict__ xcount,
+                                                      const double* __restrict__ grads) {
+    __shared__ EvalStageF stage[BLOCK / 64];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    EvalStageF& S_ = stage[wave];
+    const uint32_t q = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64)) + wave;
+    if (q >= ntargets) return;  // uniform over the wavefront
+    const EvalScale sc = eval_scale(t);
+    const uint32_t T = targets[q];
+    uint32_t tf, tc;
+    node_range(t, T, tf, tc);
+    uint32_t P = 1;
+    while (P < tc) P <<= 1;
+    const uint32_t S = 64 / P;
+    const uint32_t p = lane & (P - 1), sp = lane / P;
+    const bool active = p < tc;
+    const uint32_t k = tf + (active ? p : 0);
+    const double4 R = t.pos[tf];
+    float mx, my, mz, mh;
+    {
+        const double4 me = t.pos[k];
+        mx = float((me.x - R.x) * sc.invL);
+        my = float((me.y - R.y) * sc.invL);
+        mz = float((me.z - R.z) * sc.invL);
+        mh = float(me.w * sc.invL);
+    }
+    const float me2 = wave_max_f(active ? mx * mx + my * my + mz * mz : 0.f);
+    double a[3] = { 0.0, 0.0, 0.0 };
+    const uint32_t nn = min(ncount[q], cap_n), nx = min(xcount[q], cap_x);
+    const uint32_t* nl = nlist + size_t(q) * cap_n;
+    const uint32_t* xl = xlist + size_t(q) * cap_x;
+    const double s2 = sc.invM * sc.invL * sc.invL, s3 = s2 * sc.invL;
+
+    // node list
+    for (uint32_t base = 0; base < nn; base += 64) {
+        const uint32_t cnt = min(64u, nn - base);
+        if (lane < cnt) {
+            const uint32_t id = nl[base + lane];
+            const double4 g = t.geo[id];
+            const NodeMom& mm = t.mom[id];
+            S_.c[0][lane] = float((g.x - R.x) * sc.invL);
+            S_.c[1][lane] = float((g.y - R.y) * sc.invL);
+            S_.c[2][lane] = float((g.z - R.z) * sc.invL);
+            S_.M[lane] = float(mm.M * sc.invM);
+            if (ORDER >= 2) {
+                for (int c = 0; c < 5; ++c) S_.q2[c][lane] = float(mm.q2[c] * s2);
+            }
+            if (ORDER >= 3) {
+                for (int c = 0; c < 7; ++c) S_.q3[c][lane] = float(mm.q3[c] * s3);
+            }
+        }
+        wave_sync();
+        float b[3] = { 0.f, 0.f, 0.f };
+        for (uint32_t j = sp; j < cnt; j += S) {
+            float q2[5], q3[7];
+            if (ORDER >= 2) {
+                for (int c = 0; c < 5; ++c) q2[c] = S_.q2[c][j];
+            }
+            if (ORDER >= 3) {
+                for (int c = 0; c < 7; ++c) q3[c] = S_.q3[c][j];
+            }
+            add_multipole_f<ORDER>(S_.M[j], q2, q3, S_.c[0][j] - mx, S_.c[1][j] - my, S_.c[2][j] - mz, b);
+        }
+        for (int c = 0; c < 3; ++c) a[c] += double(b[c]);
+        wave_sync();
+    }
+
+    // source particles: entries e < nx are the listed leaves, entry nx is the target leaf
+    const uint32_t nent = nx + 1;
+    for (uint32_t eb = 0; eb < nent; eb += 64) {
+        uint32_t f = 0, c = 0;
+        const uint32_t e = eb + lane;
+        if (e < nx) {
+            node_range(t, xl[e], f, c);
+        } else if (e == nx) {
+            f = tf;
+            c = tc;
+        }
+        uint32_t incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if (lane >= uint32_t(o)) incl += v;
+        }
+        const uint32_t excl = incl - c;
+        const uint32_t total = __shfl(incl, 63, 64);
+        for (uint32_t s0 = 0; s0 < total; s0 += 64) {
+            const uint32_t slot = s0 + lane;
+            const uint32_t cnt = min(64u, total - s0);
+            int lo = 0;
+            for (int step = 32; step > 0; step >>= 1) {
+                const uint32_t ex = __shfl(excl, (lo + step) & 63, 64);
+                if (lo + step < 64 && ex <= slot) lo += step;
+            }
+            const uint32_t fe = __shfl(f, lo, 64), xe = __shfl(excl, lo, 64);
+            if (lane < cnt) {
+                const uint32_t src = fe + (slot - xe);
+                const double4 o = t.pos[src];
+                S_.src[lane] = make_float4(float((o.x - R.x) * sc.invL), float((o.y - R.y) * sc.invL),
+                                           float((o.z - R.z) * sc.invL), float(o.w * sc.invL));
+                S_.sm[lane] = float(t.gm[src] * sc.invM);
+                S_.sidx[lane] = src;
+            }
+            wave_sync();
+            float o2 = 0.f;
+            if (lane < cnt) {
+                const float4 o = S_.src[lane];
+                o2 = o.x * o.x + o.y * o.y + o.z * o.z;
+            }
+            const float close = fmaxf(wave_max_f(o2), me2) * (1.f / 64.f);  // (O / 8)^2
+            float b[3] = { 0.f, 0.f, 0.f };
+            for (uint32_t j = sp; j < cnt; j += S) {
+                const uint32_t src = S_.sidx[j];
+                if (src == k) continue;
+                const float4 o = S_.src[j];
+                float rx = o.x - mx, ry = o.y - my, rz = o.z - mz;
+                if (rx * rx + ry * ry + rz * rz < close) {
+                    const double4 od = t.pos[src], me = t.pos[k];
+                    rx = float((od.x - me.x) * sc.invL);
+                    ry = float((od.y - me.y) * sc.invL);
+                    rz = float((od.z - me.z) * sc.invL);
+                }
+                add_pair_f(grads, rx, ry, rz, 0.5f * (o.w + mh), S_.sm[j], b);
+            }
+            for (int c = 0; c < 3; ++c) a[c] += double(b[c]);
+            wave_sync();
+        }
+    }
+    for (uint32_t o = P; o < 64; o <<= 1) {
+        for (int c = 0; c < 3; ++c) a[c] += __shfl_xor(a[c], int(o), 64);
+    }
+    if (active && sp == 0) {
+        const uint32_t i = order[k];
+        s.d[GAX][i] += a[0] * sc.back;
+        s.d[GAY][i] += a[1] * sc.back;
+        s.d[GAZ][i] += a[2] * sc.back;
+    }
+}
+
 // BarnesHut::evalImpl (BarnesHut.cpp:136-178) at arbitrary points: a node is approximated when the point
 // lies outside its box and size^2 / (distance^2 + EPS) < theta^2; leaves are summed with the unsymmetrised
 // kernel of the source particle.
@@ -998,6 +1261,7 @@ using namespace osph;
 
 struct osph_gravity {
     osph_gravity_config cfg{};
+    int precision = OSPH_PRECISION_FP64;  // the tree walk's arithmetic (osph_gravity_set_precision)
     hipStream_t stream = nullptr;
     size_t n = 0, cap = 0;
     double* dblock = nullptr;
@@ -1354,7 +1618,21 @@ int eval_bh(osph_gravity* g, osph_gravity_stats* stats) {
     const int ord = g->cfg.order;
     const double* grads = g->cfg.kernel == OSPH_GRAVITY_KERNEL_M4 ? g->lut_grads : nullptr;
     const unsigned blocks = unsigned((nt + (BLOCK / 64) - 1) / (BLOCK / 64));
-    timed(g, "eval", [&] {
+    if (g->precision == OSPH_PRECISION_MIXED) {
+        timed(g, "eval_mixed", [&] {
+            const unsigned nb = blocks > 0 ? blocks : 1;
+            if (ord == 3) {
+                k_eval_mixed<3><<<nb, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist,
+                                                      g->cap_x, g->ncount, g->xcount, grads);
+            } else if (ord == 2) {
+                k_eval_mixed<2><<<nb, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist,
+                                                      g->cap_x, g->ncount, g->xcount, grads);
+            } else {
+                k_eval_mixed<0><<<nb, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist,
+                                                      g->cap_x, g->ncount, g->xcount, grads);
+            }
+        });
+    } else timed(g, "eval", [&] {
         const unsigned nb = blocks > 0 ? blocks : 1;
         if (ord == 3) {
             k_eval<3><<<nb, BLOCK, 0, st>>>(t, g->s, g->order, g->targets, nt, g->nlist, g->cap_n, g->xlist, g->cap_x,
@@ -1508,6 +1786,15 @@ void osph_gravity_destroy(osph_gravity* g) {
 }
 
 const char* osph_gravity_last_error(const osph_gravity* g) { return g ? g->err.c_str() : "null context"; }
+
+int osph_gravity_set_precision(osph_gravity* g, int precision) {
+    if (!g) return ERR_ARG;
+    if (precision != OSPH_PRECISION_FP64 && precision != OSPH_PRECISION_MIXED) {
+        return g->fail(ERR_ARG, "precision must be OSPH_PRECISION_FP64 or OSPH_PRECISION_MIXED");
+    }
+    g->precision = precision;
+    return OK;
+}
 
 int osph_gravity_set_particles(osph_gravity* g, size_t n, const double* r, const double* v, size_t vec_stride,
                                const double* m) {

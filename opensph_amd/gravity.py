@@ -20,7 +20,8 @@ SOLVER = {"brute_force": 0, "barnes_hut": 1}
 KERNEL = {"point": 0, "m4": 1}
 ORDER = {"monopole": 0, "quadrupole": 2, "octupole": 3, 0: 0, 2: 2, 3: 3}
 
-EXPORTS = ["osph_gravity_create", "osph_gravity_destroy", "osph_gravity_last_error", "osph_gravity_set_particles",
+EXPORTS = ["osph_gravity_create", "osph_gravity_destroy", "osph_gravity_last_error", "osph_gravity_set_precision",
+           "osph_gravity_set_particles",
            "osph_gravity_get", "osph_gravity_build", "osph_gravity_eval_self", "osph_gravity_eval_points",
            "osph_gravity_direct", "osph_gravity_energy", "osph_gravity_moments", "osph_gravity_step_leapfrog",
            "osph_gravity_integrals", "osph_gravity_sync", "osph_gravity_profile_enable",
@@ -50,6 +51,7 @@ def _bind(L):
     L.osph_gravity_destroy.restype = None
     L.osph_gravity_last_error.argtypes = [vp]
     L.osph_gravity_last_error.restype = C.c_char_p
+    L.osph_gravity_set_precision.argtypes = [vp, i]
     L.osph_gravity_set_particles.argtypes = [vp, sz, vp, vp, sz, vp]
     L.osph_gravity_get.argtypes = [vp, i, vp, sz]
     L.osph_gravity_build.argtypes = [vp]
@@ -78,7 +80,8 @@ class GpuGravity:
     """HBM-resident self-gravity of a particle set (positions with h in lane 3, masses, velocities)."""
 
     def __init__(self, r: np.ndarray, m: np.ndarray, v: Optional[np.ndarray] = None, solver: str = "barnes_hut",
-                 kernel: str = "m4", theta: float = 0.5, order="quadrupole", leaf_size: int = 25, G: float = G_SI):
+                 kernel: str = "m4", theta: float = 0.5, order="quadrupole", leaf_size: int = 25, G: float = G_SI,
+                 precision: str = "fp64"):
         self.L = _bind(load_library())
         cfg = GravityConfig(SOLVER[solver], KERNEL[kernel], float(theta), ORDER[order], int(leaf_size), float(G))
         self.ctx = C.c_void_p()
@@ -86,6 +89,9 @@ class GpuGravity:
         if rc != 0:
             raise GpuError(f"osph_gravity_create failed ({rc}): invalid settings or no usable GPU")
         self.G = G
+        self.precision = precision
+        self._check(self.L.osph_gravity_set_precision(self.ctx, {"fp64": 0, "mixed": 1}[precision]),
+                    "osph_gravity_set_precision")
         self.set_particles(r, m, v)
 
     def _check(self, rc, what):
