@@ -599,11 +599,14 @@ __device__ __forceinline__ void add_multipole(const NodeMom& q, double drx, doub
 }
 
 // Per-wavefront staging area of k_eval: 64 node expansions (SoA) or 64 source particles.
+// Moments of the orders below ORDER take no LDS (one unused row keeps the arrays legal): the stage's size
+// bounds the wavefronts per CU.
+template <int ORDER>
 struct EvalStage {
     double c[3][64];
     double M[64];
-    double q2[5][64];
-    double q3[7][64];
+    double q2[ORDER >= 2 ? 5 : 1][64];
+    double q3[ORDER >= 3 ? 7 : 1][64];
     double4 src[64];
     double sm[64];
     uint32_t sidx[64];
@@ -621,9 +624,9 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
                                                 const uint32_t* __restrict__ xlist, uint32_t cap_x,
                                                 const uint32_t* __restrict__ ncount,
                                                 const uint32_t* __restrict__ xcount, const double* __restrict__ grads) {
-    __shared__ EvalStage stage[BLOCK / 64];
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    EvalStage& S_ = stage[wave];
+    __shared__ EvalStage<ORDER> stage[BLOCK / 64];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    EvalStage<ORDER>& S_ = stage[wave];
     const uint32_t q = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64)) + wave;
     if (q >= ntargets) return;  // uniform over the wavefront
     const uint32_t T = targets[q];
@@ -652,10 +655,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
             S_.c[1][lane] = g.y;
             S_.c[2][lane] = g.z;
             S_.M[lane] = mm.M;
-            if (ORDER >= 2) {
+            if constexpr (ORDER >= 2) {
                 for (int c = 0; c < 5; ++c) S_.q2[c][lane] = mm.q2[c];
             }
-            if (ORDER >= 3) {
+            if constexpr (ORDER >= 3) {
                 for (int c = 0; c < 7; ++c) S_.q3[c][lane] = mm.q3[c];
             }
         }
@@ -663,10 +666,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
         for (uint32_t j = sp; j < cnt; j += S) {
             NodeMom mm;
             mm.M = S_.M[j];
-            if (ORDER >= 2) {
+            if constexpr (ORDER >= 2) {
                 for (int c = 0; c < 5; ++c) mm.q2[c] = S_.q2[c][j];
             }
-            if (ORDER >= 3) {
+            if constexpr (ORDER >= 3) {
                 for (int c = 0; c < 7; ++c) mm.q3[c] = S_.q3[c][j];
             }
             add_multipole(mm, me.x - S_.c[0][j], me.y - S_.c[1][j], me.z - S_.c[2][j], ORDER, a);
@@ -780,7 +783,7 @@ __device__ __forceinline__ void add_multipole_f(float M, const float* q2, const 
     const float w = il * il;
     const float u0 = dx * il, u1 = dy * il, u2 = dz * il;
     float hx = 0.f, hy = 0.f, hz = 0.f;
-    if (ORDER >= 2) {
+    if constexpr (ORDER >= 2) {
         const float xx = q2[0], xy = q2[1], xz = q2[2], yy = q2[3], yz = q2[4], zz = -xx - yy;
         const float Q1x = xx * u0 + xy * u1 + xz * u2;
         const float Q1y = xy * u0 + yy * u1 + yz * u2;
@@ -790,7 +793,7 @@ __device__ __forceinline__ void add_multipole_f(float M, const float* q2, const 
         hy = Q0 * u1 - 3.f * Q1y;
         hz = Q0 * u2 - 3.f * Q1z;
     }
-    if (ORDER >= 3) {
+    if constexpr (ORDER >= 3) {
         const float xxx = q3[0], xxy = q3[1], xxz = q3[2], xyy = q3[3], xyz = q3[4], yyy = q3[5], yyz = q3[6];
         const float xzz = -xxx - xyy, yzz = -xxy - yyy, zzz = -xxz - yyz;
         const float Q1x = xxx * u0 * u0 + xyy * u1 * u1 + xzz * u2 * u2 + 2.f * (xxy * u0 * u1 + xxz * u0 * u2 + xyz * u1 * u2);
@@ -832,11 +835,12 @@ __device__ __forceinline__ void add_pair_f(const double* __restrict__ grads, flo
 }
 
 // Per-wavefront staging area of k_eval_mixed: the FP32 half of EvalStage.
+template <int ORDER>
 struct EvalStageF {
     float c[3][64];
     float M[64];
-    float q2[5][64];
-    float q3[7][64];
+    float q2[ORDER >= 2 ? 5 : 1][64];
+    float q3[ORDER >= 3 ? 7 : 1][64];
     float4 src[64];
     float sm[64];
     uint32_t sidx[64];
@@ -857,9 +861,9 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const ui
                                                       const uint32_t* __restrict__ ncount,
                                                       const uint32_t* __restrict__ xcount,
                                                       const double* __restrict__ grads) {
-    __shared__ EvalStageF stage[BLOCK / 64];
+    __shared__ EvalStageF<ORDER> stage[BLOCK / 64];
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    EvalStageF& S_ = stage[wave];
+    EvalStageF<ORDER>& S_ = stage[wave];
     const uint32_t q = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / 64)) + wave;
     if (q >= ntargets) return;  // uniform over the wavefront
     const EvalScale sc = eval_scale(t);
@@ -899,10 +903,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const ui
             S_.c[1][lane] = float((g.y - R.y) * sc.invL);
             S_.c[2][lane] = float((g.z - R.z) * sc.invL);
             S_.M[lane] = float(mm.M * sc.invM);
-            if (ORDER >= 2) {
+            if constexpr (ORDER >= 2) {
                 for (int c = 0; c < 5; ++c) S_.q2[c][lane] = float(mm.q2[c] * s2);
             }
-            if (ORDER >= 3) {
+            if constexpr (ORDER >= 3) {
                 for (int c = 0; c < 7; ++c) S_.q3[c][lane] = float(mm.q3[c] * s3);
             }
         }
@@ -910,10 +914,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const ui
         float b[3] = { 0.f, 0.f, 0.f };
         for (uint32_t j = sp; j < cnt; j += S) {
             float q2[5], q3[7];
-            if (ORDER >= 2) {
+            if constexpr (ORDER >= 2) {
                 for (int c = 0; c < 5; ++c) q2[c] = S_.q2[c][j];
             }
-            if (ORDER >= 3) {
+            if constexpr (ORDER >= 3) {
                 for (int c = 0; c < 7; ++c) q3[c] = S_.q3[c][j];
             }
             add_multipole_f<ORDER>(S_.M[j], q2, q3, S_.c[0][j] - mx, S_.c[1][j] - my, S_.c[2][j] - mz, b);
