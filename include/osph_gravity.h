@@ -7,6 +7,7 @@
  *   osph_gravity_create                     <- Factory::getGravity (core/system/Factory.cpp:361-413):
  *                                              BarnesHut or BruteForceGravity with a GravityLutKernel
  *   osph_gravity_set_precision              (none: FP32 evaluation of the interaction lists)
+ *   osph_gravity_tune                       (none: FP32-screened walk)
  *   osph_gravity_set_particles              <- the POSITION / MASS buffers that IGravity::build reads
  *   osph_gravity_build                      <- IGravity::build (BarnesHut.cpp:50-84, BruteForceGravity.h:44-47)
  *   osph_gravity_eval_self                  <- IGravity::evalSelfGravity (BarnesHut.cpp:86-99,212-312;
@@ -73,6 +74,12 @@ const char* osph_gravity_last_error(const osph_gravity* g);
  * terms to FP64 accumulators; the tree, its moments, the walk and the integration stay FP64.  The
  * brute-force solver, evaluations at points and the direct sums are FP64 either way. */
 int osph_gravity_set_precision(osph_gravity* g, int precision);
+
+/* Launch tuning (no reference counterpart).  Key "walk_screen": 1 takes the Barnes-Hut walk's opening
+ * tests in FP32 on 32-byte node records, with the FP64 test wherever the FP32 margin is within its error
+ * bound, so the interaction lists are those of the FP64 walk, entry for entry; 0 walks in FP64 only; -1
+ * (the default) screens in mixed precision only. */
+int osph_gravity_tune(osph_gravity* g, const char* key, long long value);
 
 /* Particles in the reference's element layouts: r and v are Vectors (x, y, z, h; stride bytes, 0 = 32),
  * m doubles.  v may be NULL (zero velocities).  Accelerations start at zero. */

@@ -60,6 +60,16 @@ struct WalkNode {
 };
 static_assert(sizeof(WalkNode) == 64, "WalkNode is 64 bytes");
 
+// The screened walk's record of an internal node, in FP32: the centre of mass as an offset from the root
+// box's centre and r_open, both divided by the power of two 2^e at or above the root box's extent (so the
+// offsets lie within [-1/2, 1/2]); count's top bit marks r_open == 0.
+struct WalkNodeF {
+    float4 geo;
+    uint32_t first, count, left, skip;
+};
+static_assert(sizeof(WalkNodeF) == 32, "WalkNodeF is 32 bytes");
+constexpr uint32_t WALK_ZERO_R = 0x80000000u;
+
 struct Tree {
     uint32_t n;        // particles; internal nodes 0..n-2, particle k is node n-1+k
     uint32_t leaf;     // leaf size
@@ -74,6 +84,9 @@ struct Tree {
     double4* hi;
     NodeMom* mom;      // [n-1]
     WalkNode* wn;      // [n-1]
+    WalkNodeF* wnf;    // [n-1], for the screened walk
+    double ctr[3];     // the root box's centre and 2^-e, the scale of wnf
+    double invL;
     uint32_t* flag;    // [n-1]: node depths while the moments are built
     const double4* pos;  // sorted x, y, z, h
     const double* gm;    // sorted G m
@@ -494,6 +507,140 @@ __global__ void __launch_bounds__(BLOCK) k_walk(Tree t, const uint32_t* __restri
     xcount[q] = nx;
     work[2 * size_t(q)] = tc * nn;                        // particle-node expansions
     work[2 * size_t(q) + 1] = tc * xsum + tc * (tc - 1);  // particle pairs
+    if (nn > cap_n) atomicMax(&need[0], nn);
+    if (nx > cap_x) atomicMax(&need[1], nx);
+}
+
+// The target's box: its leaf's box, or the particle itself.
+__device__ __forceinline__ void target_box(const Tree& t, uint32_t T, uint32_t tf, double lo[3], double hi[3]) {
+    if (is_particle(t, T)) {
+        const double4 p = t.pos[tf];
+        lo[0] = hi[0] = p.x;
+        lo[1] = hi[1] = p.y;
+        lo[2] = hi[2] = p.z;
+    } else {
+        const double4 a = t.lo[T], b = t.hi[T];
+        lo[0] = a.x;
+        lo[1] = a.y;
+        lo[2] = a.z;
+        hi[0] = b.x;
+        hi[1] = b.y;
+        hi[2] = b.z;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_walk_records(Tree t) {
+    const uint32_t i = uint32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i + 1 >= t.n) return;
+    const WalkNode w = t.wn[i];
+    WalkNodeF f;
+    f.geo = make_float4(float((w.geo.x - t.ctr[0]) * t.invL), float((w.geo.y - t.ctr[1]) * t.invL),
+                        float((w.geo.z - t.ctr[2]) * t.invL), float(w.geo.w * t.invL));
+    f.first = w.first;
+    f.count = w.count | (w.geo.w == 0.0 ? WALK_ZERO_R : 0u);
+    f.left = w.left;
+    f.skip = w.skip;
+    t.wnf[i] = f;
+}
+
+// k_walk with the opening test screened in FP32: the same interaction lists, in the same order, from
+// 32-byte node records instead of 64-byte ones.  In units of the root box (offsets within [-1/2, 1/2]),
+// the rounding of the box, the centre and r_open to FP32 moves S = |box - centre|^2 by at most
+// 7 delta sqrt(S) (delta = 2^-24) and the FP32 arithmetic adds a few ulps of Q = r_open^2 + S; where
+// |r_open^2 - S| is within 16 delta sqrt(Q) + 2e-6 Q, the FP64 test of k_walk decides.
+__global__ void __launch_bounds__(BLOCK) k_walk_screened(Tree t, const uint32_t* __restrict__ targets,
+                                                         uint32_t ntargets, uint32_t* __restrict__ nlist,
+                                                         uint32_t cap_n, uint32_t* __restrict__ xlist, uint32_t cap_x,
+                                                         uint32_t* __restrict__ ncount, uint32_t* __restrict__ xcount,
+                                                         uint32_t* __restrict__ work, uint32_t* need) {
+    const uint32_t q = uint32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= ntargets) return;
+    const uint32_t T = targets[q];
+    uint32_t tf, tc;
+    node_range(t, T, tf, tc);
+    float lof[3], hif[3];
+    {
+        double lo[3], hi[3];
+        target_box(t, T, tf, lo, hi);
+        for (int k = 0; k < 3; ++k) {
+            lof[k] = float((lo[k] - t.ctr[k]) * t.invL);
+            hif[k] = float((hi[k] - t.ctr[k]) * t.invL);
+        }
+    }
+    uint32_t nn = 0, nx = 0, xsum = 0;
+    uint32_t* nl = nlist + size_t(q) * cap_n;
+    uint32_t* xl = xlist + size_t(q) * cap_x;
+    uint32_t id = 0;
+    for (size_t visits = 0; id != END; ++visits) {
+        if (visits > 2 * size_t(t.n)) {
+            atomicOr(t.error, 2u);
+            break;
+        }
+        if (id == T) {
+            id = t.skip[id];
+            continue;
+        }
+        if (is_particle(t, id)) {
+            if (nx < cap_x) xl[nx] = id;
+            ++nx;
+            ++xsum;
+            id = t.skip[id];
+            continue;
+        }
+        const WalkNodeF w = t.wnf[id];
+        const uint32_t cnt = w.count & ~WALK_ZERO_R;
+        if (w.first <= tf && tf + tc <= w.first + cnt) {  // an ancestor of the target
+            id = w.left;
+            continue;
+        }
+        const bool is_leaf = cnt <= t.leaf;
+        const bool zero_r = (w.count & WALK_ZERO_R) != 0u;
+        if (is_leaf && zero_r) {
+            if (nx < cap_x) xl[nx] = id;
+            ++nx;
+            xsum += cnt;
+            id = w.skip;
+            continue;
+        }
+        if (zero_r) {
+            id = w.left;
+            continue;
+        }
+        float S = 0.f;
+        {
+            const float c[3] = { w.geo.x, w.geo.y, w.geo.z };
+            for (int k = 0; k < 3; ++k) {
+                const float l = fmaxf(lof[k] - c[k], 0.f), r = fmaxf(c[k] - hif[k], 0.f);
+                S += l * l + r * r;
+            }
+        }
+        const float R2 = w.geo.w * w.geo.w;
+        const float m = R2 - S, Q = R2 + S;
+        bool overlaps;
+        if (fabsf(m) > 1e-6f * __builtin_amdgcn_sqrtf(Q) + 2e-6f * Q) {
+            overlaps = m > 0.f;
+        } else {
+            double lo[3], hi[3];
+            target_box(t, T, tf, lo, hi);
+            overlaps = ball_overlaps_box(t.geo[id], lo, hi);
+        }
+        if (!overlaps) {
+            if (nn < cap_n) nl[nn] = id;
+            ++nn;
+            id = w.skip;
+        } else if (is_leaf) {
+            if (nx < cap_x) xl[nx] = id;
+            ++nx;
+            xsum += cnt;
+            id = w.skip;
+        } else {
+            id = w.left;
+        }
+    }
+    ncount[q] = nn;
+    xcount[q] = nx;
+    work[2 * size_t(q)] = tc * nn;
+    work[2 * size_t(q) + 1] = tc * xsum + tc * (tc - 1);
     if (nn > cap_n) atomicMax(&need[0], nn);
     if (nx > cap_x) atomicMax(&need[1], nx);
 }
@@ -1282,6 +1429,10 @@ struct osph_gravity {
     double4 *geo = nullptr, *lo = nullptr, *hi = nullptr;
     NodeMom* mom = nullptr;
     WalkNode* wn = nullptr;
+    WalkNodeF* wnf = nullptr;  // filled on the first screened walk after a build
+    bool wnf_ready = false;
+    double ctr[3] = { 0.0, 0.0, 0.0 }, invL = 1.0;
+    int walk_screen = -1;  // 1: FP32-screened walk, 0: FP64 walk, -1: screened in mixed precision only
     uint32_t *leaf_at = nullptr, *targets = nullptr, *ntargets_dev = nullptr;
     void* select_temp = nullptr;
     size_t select_temp_bytes = 0;
@@ -1382,7 +1533,7 @@ int check_traversal(osph_gravity* g) {
 void free_particles(osph_gravity* g) {
     void* ptrs[] = { g->dblock, g->keys, g->keys_sorted, g->iota, g->order, g->sort_temp, g->pos, g->gm, g->left,
                      g->right, g->parent, g->skip, g->first, g->count, g->flag, g->geo, g->lo, g->hi, g->mom, g->wn,
-                     g->leaf_at, g->targets, g->ntargets_dev, g->select_temp };
+                     g->wnf, g->leaf_at, g->targets, g->ntargets_dev, g->select_temp };
     for (void* p : ptrs) (void)hipFree(p);
     g->dblock = nullptr;
     g->keys = g->keys_sorted = nullptr;
@@ -1394,6 +1545,8 @@ void free_particles(osph_gravity* g) {
     g->geo = g->lo = g->hi = nullptr;
     g->mom = nullptr;
     g->wn = nullptr;
+    g->wnf = nullptr;
+    g->wnf_ready = false;
     g->leaf_at = g->targets = g->ntargets_dev = nullptr;
     g->cap = 0;
 }
@@ -1420,6 +1573,7 @@ int ensure_particles(osph_gravity* g, size_t n) {
     GCHK(g, hipMalloc(&g->hi, sizeof(double4) * nodes));
     GCHK(g, hipMalloc(&g->mom, sizeof(NodeMom) * nodes));
     GCHK(g, hipMalloc(&g->wn, sizeof(WalkNode) * nodes));
+    GCHK(g, hipMalloc(&g->wnf, sizeof(WalkNodeF) * nodes));
     GCHK(g, hipMalloc(&g->parent, sizeof(uint32_t) * ids));
     GCHK(g, hipMalloc(&g->skip, sizeof(uint32_t) * ids));
     GCHK(g, hipMalloc(&g->leaf_at, sizeof(uint32_t) * c));
@@ -1493,6 +1647,9 @@ Tree tree_of(osph_gravity* g) {
     t.hi = g->hi;
     t.mom = g->mom;
     t.wn = g->wn;
+    t.wnf = g->wnf;
+    for (int k = 0; k < 3; ++k) t.ctr[k] = g->ctr[k];
+    t.invL = g->invL;
     t.flag = g->flag;
     t.pos = g->pos;
     t.gm = g->gm;
@@ -1560,6 +1717,12 @@ int build_tree(osph_gravity* g) {
         ext = std::max(ext, hi - lo[k]);
     }
     const double scale = ext > 0.0 ? 2097151.0 / ext : 0.0;
+    // the screened walk's frame: the root box's centre, lengths divided by 2^e >= the extent
+    for (int k = 0; k < 3; ++k) g->ctr[k] = lo[k] + 0.5 * (decode(g->small_host[3 + k]) - lo[k]);
+    int e = 0;
+    if (ext > 0.0) (void)std::frexp(ext, &e);
+    g->invL = std::ldexp(1.0, -e);
+    g->wnf_ready = false;
     Tree t = tree_of(g);
     timed(g, "morton", [&] { k_morton<<<grid_for(n), BLOCK, 0, st>>>(g->s, lo[0], lo[1], lo[2], scale, g->keys, g->iota, n); });
     timed(g, "sort", [&] {
@@ -1601,7 +1764,18 @@ int eval_bh(osph_gravity* g, osph_gravity_stats* stats) {
     for (int attempt = 0; attempt < 4; ++attempt) {
         if (int e = ensure_lists(g, nt)) return e;
         uint32_t* need = reinterpret_cast<uint32_t*>(g->small + 6);
-        timed(g, "walk", [&] {
+        const bool screened = g->walk_screen == 1 || (g->walk_screen == -1 && g->precision == OSPH_PRECISION_MIXED);
+        if (screened && !g->wnf_ready && n > 1) {
+            timed(g, "walk_records", [&] { k_walk_records<<<grid_for(n - 1), BLOCK, 0, st>>>(t); });
+            g->wnf_ready = true;
+        }
+        if (screened && n > 1) {
+            timed(g, "walk_screened", [&] {
+                k_fill_u32g<<<1, 64, 0, st>>>(need, 2, 0u);
+                k_walk_screened<<<grid_for(nt), BLOCK, 0, st>>>(t, g->targets, nt, g->nlist, g->cap_n, g->xlist,
+                                                                g->cap_x, g->ncount, g->xcount, g->work, need);
+            });
+        } else timed(g, "walk", [&] {
             k_fill_u32g<<<1, 64, 0, st>>>(need, 2, 0u);
             k_walk<<<grid_for(nt), BLOCK, 0, st>>>(t, g->targets, nt, g->nlist, g->cap_n, g->xlist, g->cap_x, g->ncount,
                                                    g->xcount, g->work, need);
@@ -1798,6 +1972,16 @@ int osph_gravity_set_precision(osph_gravity* g, int precision) {
     }
     g->precision = precision;
     return OK;
+}
+
+int osph_gravity_tune(osph_gravity* g, const char* key, long long value) {
+    if (!g || !key) return ERR_ARG;
+    const std::string k(key);
+    if (k == "walk_screen" && (value == -1 || value == 0 || value == 1)) {
+        g->walk_screen = int(value);
+        return OK;
+    }
+    return g->fail(ERR_ARG, "unknown tuning key or value out of range: " + k);
 }
 
 int osph_gravity_set_particles(osph_gravity* g, size_t n, const double* r, const double* v, size_t vec_stride,

@@ -21,7 +21,7 @@ KERNEL = {"point": 0, "m4": 1}
 ORDER = {"monopole": 0, "quadrupole": 2, "octupole": 3, 0: 0, 2: 2, 3: 3}
 
 EXPORTS = ["osph_gravity_create", "osph_gravity_destroy", "osph_gravity_last_error", "osph_gravity_set_precision",
-           "osph_gravity_set_particles",
+           "osph_gravity_tune", "osph_gravity_set_particles",
            "osph_gravity_get", "osph_gravity_build", "osph_gravity_eval_self", "osph_gravity_eval_points",
            "osph_gravity_direct", "osph_gravity_energy", "osph_gravity_moments", "osph_gravity_step_leapfrog",
            "osph_gravity_integrals", "osph_gravity_sync", "osph_gravity_profile_enable",
@@ -52,6 +52,7 @@ def _bind(L):
     L.osph_gravity_last_error.argtypes = [vp]
     L.osph_gravity_last_error.restype = C.c_char_p
     L.osph_gravity_set_precision.argtypes = [vp, i]
+    L.osph_gravity_tune.argtypes = [vp, C.c_char_p, C.c_longlong]
     L.osph_gravity_set_particles.argtypes = [vp, sz, vp, vp, sz, vp]
     L.osph_gravity_get.argtypes = [vp, i, vp, sz]
     L.osph_gravity_build.argtypes = [vp]
@@ -108,6 +109,11 @@ class GpuGravity:
             self.close()
         except Exception:
             pass
+
+    def tune(self, **kw):
+        """Launch tuning (see osph_gravity_tune): walk_screen = -1, 0 or 1."""
+        for k, v in kw.items():
+            self._check(self.L.osph_gravity_tune(self.ctx, k.encode(), int(v)), f"osph_gravity_tune({k})")
 
     def set_particles(self, r, m, v=None):
         r4 = _vec4(r)
