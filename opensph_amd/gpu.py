@@ -229,6 +229,24 @@ class GpuSolver:
                 a[...] = host
         return st
 
+    def save(self, path, run_time: float = 0.0, template: ParticleState | None = None):
+        """BinaryOutput::dump of the resident state (core/io/Output.cpp:481-595): the quantities of the path,
+        the materials' parameters and the current time step, in the reference's .ssf format."""
+        from . import ssf
+        st = self.download(template)
+        ssf.write(path, ssf.from_state(st, run_time=run_time, time_step=self.dt, solid=bool(self.cfg.solid_stress),
+                                       damage=bool(self.cfg.has_damage)))
+
+    @classmethod
+    def load(cls, path, cfg: RunConfig, precision: str = "fp64") -> "GpuSolver":
+        """BinaryInput::load (core/io/Output.cpp:697-803) of an .ssf file into a new resident solver, which
+        continues with the file's time step."""
+        from . import ssf
+        f = ssf.read(path)
+        solver = cls(ssf.to_state(f), cfg, precision=precision)
+        solver.dt = f.time_step
+        return solver
+
     def zero_derivatives(self):
         self._check(self.L.osph_zero_derivatives(self.ctx), "osph_zero_derivatives")
 
