@@ -354,6 +354,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="append", default=[],
                     help="launch-shape setting key=value passed to osph_tune (e.g. pairs_lds=40960)")
+    ap.add_argument("--gravity", type=int, default=0, choices=[0, 1, 2],
+                    help="impact with self-gravity (GravitySolver): 1 direct sum, 2 Barnes-Hut theta 0.5 octupole")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch shapes with opensph_amd.autotune before the warmup")
     ap.add_argument("--calibrate", action="store_true",
@@ -391,6 +393,9 @@ def main():
 
     t0 = time.perf_counter()
     st, cfg = initial.impact()
+    if args.gravity:
+        import dataclasses
+        cfg = dataclasses.replace(cfg, gravity=args.gravity)
     log(f"setup {st.n} particles in {time.perf_counter() - t0:.1f}s")
     name, cus = gpu.device_info()
     precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
@@ -426,7 +431,9 @@ def main():
     if "fp64" in runs and "mixed" in runs:
         accuracy = compare_precisions(gpu, st, cfg, runs)
 
-    baseline = None if args.no_cpu else cpu_baseline(st, cfg)
+    # the reference driver's self-gravity is the direct sum: O(N^2) per step at 1M particles, not a
+    # bounded sample, so the gravity variants report no CPU baseline
+    baseline = None if (args.no_cpu or args.gravity) else cpu_baseline(st, cfg)
     out = {
         "metric": "particle-steps/s, 1M impact: mixed vs FP64 GPU at <=1e-5 accel err",
         "value": value,
@@ -443,6 +450,7 @@ def main():
         "config": {"workload": "basalt two-body impact (R 50 km + 5 km, 5 km/s, 45 deg), solid stress + AV + "
                                "continuity + adaptive h, von Mises + Grady-Kipp damage, predictor-corrector",
                    "particles": st.n, "precision": main_prec, "device": name, "compute_units": cus,
+                   "self_gravity": {0: None, 1: "direct sum", 2: "Barnes-Hut theta 0.5 octupole leaf 25"}[args.gravity],
                    "neighbors_mean": stats["neighbors_mean"]},
         "fp64_value": runs["fp64"]["value"] if "fp64" in runs else None,
         "fp64_kernel_ms_per_step": ({k: v[0] / max(v[1], 1) for k, v in sorted(runs["fp64"]["prof"].items())}

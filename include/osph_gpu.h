@@ -136,6 +136,17 @@ int osph_timestep(osph_ctx* ctx, double* dt, int* criterion);
  * the first failing context's code is returned and its osph_last_error() holds the message.  No
  * reference counterpart: the reference runs ensembles as separate jobs. */
 int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* next_dt, int* criterion);
+
+/* Replaces GravitySolver<SymmetricSolver<3>> (core/sph/solvers/GravitySolver.cpp:64-91): with a self-gravity
+ * context attached (osph_gravity.h), every derivative evaluation of ctx (osph_integrate, osph_step_pc and
+ * the batch) first evaluates the self-gravity of the current positions, smoothing lengths and masses
+ * with it, and the pair sums then add to those accelerations.  The particles are copied device to device
+ * each evaluation (the gravity context's own particles are overwritten); set the gravity context up as
+ * Factory::getGravity would (core/system/Factory.cpp:361-413; BruteForceGravity takes Constants::gravity,
+ * BarnesHut GRAVITY_CONSTANT).  g = NULL detaches.  The gravity context must outlive the attachment;
+ * the adjoint is not available while one is attached. */
+struct osph_gravity;
+int osph_attach_gravity(osph_ctx* ctx, struct osph_gravity* g);
 int osph_sync(osph_ctx* ctx);
 /* Integrals of the resident state (TotalEnergy, TotalKineticEnergy, TotalInternalEnergy, TotalMomentum,
  * TotalAngularMomentum about the origin, CenterOfMass, TotalMass; core/physics/Integrals.cpp:9-180):

@@ -1,6 +1,7 @@
 // C ABI of the MI355X OpenSPH hot path (declared in include/osph_gpu.h): context, HBM-resident state,
 // neighbour build orchestration, SymmetricSolver<3>::integrate and PredictorCorrector::step.
 #include "osph_internal.h"
+#include "osph_gravity.h"
 
 #include <algorithm>
 #include <cmath>
@@ -78,6 +79,11 @@ struct osph_ctx {
         double* ckpt = nullptr;  // ceil(steps / every) blocks of 13 planes of cap doubles
         size_t ckpt_blocks = 0, ckpt_cap = 0;
     } adj;
+    // attached self-gravity (osph_attach_gravity): evaluated once per derivative evaluation, copied into
+    // the accelerations before every pair-sum launch
+    osph_gravity* grav = nullptr;
+    hipEvent_t grav_ready = nullptr, grav_done = nullptr;
+    const double* grav_acc[3] = { nullptr, nullptr, nullptr };
 
     unsigned char* staging = nullptr;
     size_t staging_cap = 0;
@@ -648,11 +654,35 @@ int build_neighbors(osph_ctx* c, int mode) {
     return build_lists(c);
 }
 
+// GravitySolver::loop (core/sph/solvers/GravitySolver.cpp:64-91): the attached context's self-gravity of
+// the current (predicted) positions, evaluated on its own stream after this context's queued work; this
+// stream then waits for it.
+int gravity_eval(osph_ctx* c) {
+    if (!c->grav || !c->n) return OSPH_OK;
+    HIPCHK(c, hipEventRecord(c->grav_ready, c->stream));
+    const double* src[5] = { c->s.d[X], c->s.d[Y], c->s.d[Z], c->s.d[H], c->s.d[M] };
+    if (gravity_self_from_device(c->grav, c->n, src, c->grav_ready, c->grav_done, c->grav_acc) != 0) {
+        return c->fail(OSPH_ERR_HIP, std::string("self-gravity: ") + osph_gravity_last_error(c->grav));
+    }
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->grav_done, 0));
+    return OSPH_OK;
+}
+
 // The pair sums; with fuse (the PC step), followed in the same kernel by the finalize, the corrector and
 // the criteria (timed then as "<pairs>_finalize_correct").
 void launch_pair_sums(osph_ctx* c, const FuseArgs* fuse = nullptr) {
     Launch l = L(c);
     const size_t n = c->n;
+    if (c->grav && n) {
+        // the pair kernels add their sums to the self-gravity (GravitySolver::loop evaluates gravity
+        // into the accumulated accelerations first)
+        timed(c, "gravity_copy", [&] {
+            for (int k = 0; k < 3; ++k) {
+                (void)hipMemcpyAsync(c->s.d[AX + k], c->grav_acc[k], n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     c->stream);
+            }
+        });
+    }
     if (c->cfg.precision == OSPH_PRECISION_MIXED) {
         timed(c, fuse ? "pairs_mixed_finalize_correct" : "pairs_mixed", [&] {
             launch_pairs_mixed(l, c->s, c->prm, c->lattice, c->recf, c->order, c->list, c->count, c->kmax, n, fuse);
@@ -669,6 +699,7 @@ int integrate(osph_ctx* c, osph_stats* stats) {
     const size_t n = c->n;
     Launch l = L(c);
     if (int e = build_neighbors(c, 1)) return e;
+    if (int e = gravity_eval(c)) return e;
     launch_pair_sums(c);
     timed(c, "finalize", [&] { launch_finalize(l, c->s, c->mats_dev, c->prm, n); });
     if (stats) {
@@ -808,6 +839,8 @@ void osph_destroy(osph_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     resolve_events(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->grav_ready) (void)hipEventDestroy(c->grav_ready);
+    if (c->grav_done) (void)hipEventDestroy(c->grav_done);
     free_state(c);
     (void)hipFree(c->cell_first);
     (void)hipFree(c->row_range);
@@ -1046,6 +1079,7 @@ void launch_second_half(osph_ctx* c, double dt) {
 int step_middle(osph_ctx* c, double dt) {
     if (c->n == 0) return OSPH_OK;
     if (int e = search_finish(c)) return e;
+    if (int e = gravity_eval(c)) return e;
     if (int e = launch_lists(c)) return e;
     launch_second_half(c, dt);
     return OSPH_OK;
@@ -1095,6 +1129,20 @@ int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* nex
     for (int k = 0; k < count; ++k) {
         if (int e = step_end(ctxs[k], dt[k], next_dt ? next_dt + k : nullptr, crit ? crit + k : nullptr)) return e;
     }
+    return OSPH_OK;
+}
+
+int osph_attach_gravity(osph_ctx* c, osph_gravity* g) {
+    if (!c) return OSPH_ERR_ARG;
+    if (g && c->adj.ready) return c->fail(OSPH_ERR_UNSUPPORTED, "the adjoint has no self-gravity");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (g && !c->grav_ready) {
+        HIPCHK(c, hipEventCreateWithFlags(&c->grav_ready, hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->grav_done, hipEventDisableTiming));
+    }
+    c->grav = g;
+    c->prm.add_accel = g ? 1 : 0;
+    c->neighbors_valid = false;
     return OSPH_OK;
 }
 
@@ -1187,6 +1235,7 @@ int osph_adjoint_forward(osph_ctx* c, int steps, const double* dt, int checkpoin
                        "the adjoint covers the gas terms: no solid stress, damage or adaptive smoothing length");
     }
     if (c->mats.size() > size_t(ADJ_MAX_MATERIALS)) return c->fail(OSPH_ERR_UNSUPPORTED, "too many materials");
+    if (c->grav) return c->fail(OSPH_ERR_UNSUPPORTED, "the adjoint has no self-gravity: detach it first");
     for (int k = 0; k < steps; ++k) {
         if (!(dt[k] > 0.0) || !std::isfinite(dt[k])) return c->fail(OSPH_ERR_ARG, "time steps must be positive");
     }

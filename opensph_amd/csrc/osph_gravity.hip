@@ -1910,6 +1910,50 @@ int download_vectors(osph_gravity* g, double* host, size_t stride, int plane0, i
 
 }  // namespace
 
+namespace {
+
+// Positions, smoothing lengths and masses of another context's particles (device planes, n each) into
+// this context's planes; accelerations and velocities start at zero.
+__global__ void __launch_bounds__(BLOCK) k_load_planes(Planes s, const double* __restrict__ x,
+                                                       const double* __restrict__ y, const double* __restrict__ z,
+                                                       const double* __restrict__ h, const double* __restrict__ m,
+                                                       size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s.d[GX][i] = x[i];
+    s.d[GY][i] = y[i];
+    s.d[GZ][i] = z[i];
+    s.d[GH][i] = h[i];
+    s.d[GM][i] = m[i];
+    for (int k = GVX; k <= GVH; ++k) s.d[k][i] = 0.0;
+    for (int k = GAX; k <= GAZ; ++k) s.d[k][i] = 0.0;
+}
+
+}  // namespace
+
+// GravitySolver::loop's gravity half (core/sph/solvers/GravitySolver.cpp:64-80) for particles held in
+// another context's device planes: once `ready` has fired, the planes are copied in on this context's
+// stream, the tree is built and the self-gravity evaluated; `done` is recorded after the last kernel and
+// accel[0..2] then point to the accelerations (n doubles each, in the order of the source planes).
+int gravity_self_from_device(osph_gravity* g, size_t n, const double* const src[5], hipEvent_t ready,
+                             hipEvent_t done, const double* accel[3]) {
+    if (!g || (n && !src)) return ERR_ARG;
+    if (n >= (size_t(1) << 31)) return g->fail(ERR_ARG, "particle count must be below 2^31");
+    if (int e = ensure_particles(g, n > 0 ? n : 1)) return e;
+    g->n = n;
+    g->built = false;
+    for (int k = 0; k < 3; ++k) accel[k] = g->s.d[GAX + k];
+    GCHK(g, hipStreamWaitEvent(g->stream, ready, 0));
+    if (n) {
+        k_load_planes<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, src[0], src[1], src[2], src[3], src[4], n);
+        GCHK(g, hipPeekAtLastError());
+        if (int e = osph_gravity_build(g)) return e;
+        if (int e = eval_self(g, nullptr)) return e;
+    }
+    GCHK(g, hipEventRecord(done, g->stream));
+    return OK;
+}
+
 extern "C" {
 
 int osph_gravity_create(osph_gravity** out, const osph_gravity_config* cfg) {

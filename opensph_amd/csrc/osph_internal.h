@@ -105,6 +105,7 @@ struct Params {
     double av_alpha, av_beta, h_lo, h_hi;
     int criteria;
     double courant, deriv_factor, div_factor, max_dt;
+    int add_accel;  // the pair sums add to the accelerations already in the planes (attached self-gravity)
 };
 
 constexpr int LUT_ENTRIES = 40000;
@@ -279,3 +280,11 @@ hipError_t sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint
                       const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits);
 
 }  // namespace osph
+
+// The gravity half of GravitySolver::loop for particles in another context's device planes (defined with
+// the self-gravity, osph_gravity.hip): src = x, y, z, h, m (n doubles each); the copy waits for `ready`,
+// `done` is recorded on the gravity context's stream after the evaluation, and accel[0..2] then point
+// to the accelerations in the order of the source planes.  Returns a status of osph_gravity.h.
+struct osph_gravity;
+int gravity_self_from_device(osph_gravity* g, size_t n, const double* const src[5], hipEvent_t ready,
+                             hipEvent_t done, const double* accel[3]);

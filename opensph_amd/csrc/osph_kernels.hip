@@ -784,6 +784,11 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
     double cv[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
     if (ln.valid && ln.sub == 0) {
         const uint32_t i = order[k];
+        if (prm.add_accel) {
+            ax += s.d[AX][i];
+            ay += s.d[AY][i];
+            az += s.d[AZ][i];
+        }
         s.d[AX][i] = ax;
         s.d[AY][i] = ay;
         s.d[AZ][i] = az;
@@ -987,9 +992,15 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
     double cv[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
     if (ln.valid && ln.sub == 0) {
         const uint32_t i = order[k];
-        s.d[AX][i] = sax;
-        s.d[AY][i] = say;
-        s.d[AZ][i] = saz;
+        if (prm.add_accel) {
+            s.d[AX][i] = sax + s.d[AX][i];
+            s.d[AY][i] = say + s.d[AY][i];
+            s.d[AZ][i] = saz + s.d[AZ][i];
+        } else {
+            s.d[AX][i] = sax;
+            s.d[AY][i] = say;
+            s.d[AZ][i] = saz;
+        }
         s.d[DU][i] = sdu;
         s.d[DIVV][i] = sdivv;
         if (solid) {

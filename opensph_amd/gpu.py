@@ -30,7 +30,7 @@ EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
            "osph_stream_copy", "osph_tune", "osph_integrals", "osph_step_pc_batch",
            "osph_profile_enable", "osph_profile_read", "osph_adjoint_forward", "osph_adjoint_backward",
-           "osph_adjoint_clear"]
+           "osph_adjoint_clear", "osph_attach_gravity"]
 
 
 class Material(C.Structure):
@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_adjoint_forward.argtypes = [vp, i, dp, i]
     L.osph_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_uint)]
     L.osph_adjoint_clear.argtypes = [vp]
+    L.osph_attach_gravity.argtypes = [vp, vp]
     L.osph_profile_enable.argtypes = [vp, i]
     L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
     _lib = L
@@ -171,7 +172,28 @@ class GpuSolver:
             raise GpuError(f"osph_create failed ({rc}): no usable GPU or HIP runtime error")
         self.n = 0
         self.dt = cfg.initial_dt
+        self.gravity = None
         self.upload(st)
+        if getattr(cfg, "gravity", 0):
+            self.attach_gravity_from_config()
+
+    def attach_gravity_from_config(self):
+        """GravitySolver<SymmetricSolver> with the gravity Factory::getGravity builds from the run settings
+        (core/system/Factory.cpp:361-413): the direct sum with Constants::gravity, or Barnes-Hut with the
+        configured opening angle, multipole order, leaf size and constant; M4 (SPH-kernel) softening."""
+        from .gravity import G_SI, GpuGravity
+        st = self.download()
+        brute = self.cfg.gravity == 1
+        g = GpuGravity(st["r"], st["m"], solver="brute_force" if brute else "barnes_hut", kernel="m4",
+                       theta=self.cfg.gravity_theta, order=self.cfg.gravity_order, leaf_size=self.cfg.gravity_leaf,
+                       G=G_SI if brute else self.cfg.gravity_G, precision=self.precision)
+        self.attach_gravity(g)
+
+    def attach_gravity(self, g):
+        """osph_attach_gravity: g (a GpuGravity, or None to detach) evaluates the self-gravity of every
+        derivative evaluation."""
+        self._check(self.L.osph_attach_gravity(self.ctx, g.ctx if g is not None else None), "osph_attach_gravity")
+        self.gravity = g
 
     def _check(self, rc, what):
         if rc != 0:
@@ -182,6 +204,9 @@ class GpuSolver:
         if getattr(self, "ctx", None) and self.ctx.value:
             self.L.osph_destroy(self.ctx)
             self.ctx = C.c_void_p()
+        if getattr(self, "gravity", None) is not None:
+            self.gravity.close()  # after the context it is attached to
+            self.gravity = None
 
     def __del__(self):
         try:

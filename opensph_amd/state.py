@@ -66,6 +66,16 @@ class RunConfig:
     div_factor: float = 0.005
     max_dt: float = 10.0
     initial_dt: float = 0.03
+    # GravitySolver<SymmetricSolver> (core/sph/solvers/GravitySolver.cpp:64-91): self-gravity added to the
+    # accelerations of every derivative evaluation; 0 none, 1 BruteForceGravity, 2 BarnesHut, both with the
+    # SPH-kernel (M4) softening (RunSettingsId::GRAVITY_*, defaults core/system/Settings.cpp:633-642).
+    # As in Factory::getGravity (core/system/Factory.cpp:361-413), gravity_G applies to Barnes-Hut only;
+    # the direct sum always uses Constants::gravity.
+    gravity: int = 0
+    gravity_G: float = 6.67408e-11  # Constants::gravity (core/physics/Constants.h:32)
+    gravity_theta: float = 0.5
+    gravity_order: int = 3
+    gravity_leaf: int = 25
 
 
 class ParticleState:

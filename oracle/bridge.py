@@ -54,7 +54,8 @@ class Config(C.Structure):
                 ("av_alpha", C.c_double), ("av_beta", C.c_double), ("h_lo", C.c_double), ("h_hi", C.c_double),
                 ("criteria", C.c_int), ("courant", C.c_double), ("deriv_factor", C.c_double),
                 ("div_factor", C.c_double), ("max_dt", C.c_double),
-                ("n_mat", C.c_int), ("mat", C.POINTER(Material))]
+                ("n_mat", C.c_int), ("mat", C.POINTER(Material)),
+                ("gravity", C.c_int)]
 
 
 def make_config(cfg, materials):

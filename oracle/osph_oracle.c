@@ -6,6 +6,7 @@
  * only differences to the reference are the summation order of pair contributions.
  */
 #include "osph_oracle.h"
+#include "osph_oracle_gravity.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -493,6 +494,18 @@ void orc_integrate(const orc_config* cfg, double* P, uint32_t* I, size_t n) {
                                ORC_GXY, ORC_GXZ, ORC_GYZ };
     for (size_t k = 0; k < sizeof(acc) / sizeof(acc[0]); ++k) memset(PL(acc[k]), 0, n * sizeof(double));
     memset(IPL(ORC_NCNT), 0, n * sizeof(uint32_t));
+    /* GravitySolver::loop (core/sph/solvers/GravitySolver.cpp:64-91): the self-gravity goes into the
+     * accumulated accelerations first (BruteForceGravity::evalSelfGravity, BruteForceGravity.h:49-57),
+     * then the SPH pair sums add to them; the planes AX, AY, AZ are consecutive */
+    if (cfg->gravity) {
+        double* gvalues = (double*)malloc((ORC_GRAVITY_LUT_ENTRIES + 1) * sizeof(double));
+        double* ggrads = (double*)malloc((ORC_GRAVITY_LUT_ENTRIES + 1) * sizeof(double));
+        orc_gravity_lut(gvalues, ggrads);
+        const double G = 6.67408e-11; /* Constants::gravity (core/physics/Constants.h:32) */
+        orc_gravity_direct(n, PL(ORC_X), PL(ORC_Y), PL(ORC_Z), PL(ORC_H), PL(ORC_M), ggrads, G, NULL, n, PL(ORC_AX));
+        free(gvalues);
+        free(ggrads);
+    }
 
     loop_ctx* c = (loop_ctx*)malloc(sizeof(loop_ctx));
     double* values = (double*)malloc((ORC_LUT_ENTRIES + 1) * sizeof(double));

@@ -70,6 +70,11 @@ typedef struct {
     double courant, deriv_factor, div_factor, max_dt;
     int n_mat;
     const orc_material* mat;
+    /* GravitySolver<SymmetricSolver> (core/sph/solvers/GravitySolver.cpp:64-91): 1 adds the self-gravity of
+     * BruteForceGravity with the GravityLutKernel (GRAVITY_KERNEL = SPH_KERNEL) to the accelerations
+     * before the pair sums; 0 none.  Factory::getGravity (core/system/Factory.cpp:383-385) builds
+     * BruteForceGravity with its default constant, Constants::gravity, whatever GRAVITY_CONSTANT holds. */
+    int gravity;
 } orc_config;
 
 /* M4 look-up table (LutKernel<3> built from CubicSpline<3>): values and grads, 40001 entries each. */

@@ -11,6 +11,7 @@
 #include "quantities/Quantity.h"
 #include "quantities/Storage.h"
 #include "sph/Materials.h"
+#include "sph/solvers/GravitySolver.h"
 #include "sph/solvers/StandardSets.h"
 #include "sph/solvers/SymmetricSolver.h"
 #include "system/Factory.h"
@@ -172,7 +173,14 @@ void* ref_create(const orc_config* cfg, const double* P, const uint32_t* I, size
     }
 
     EquationHolder eqs = getStandardEquations(s);
-    run->solver = makeAuto<SymmetricSolver<3>>(*run->scheduler, s, eqs);
+    if (cfg->gravity) {
+        // the self-gravity of the reference's GravitySolver, with the direct sum
+        s.set(RunSettingsId::GRAVITY_SOLVER, GravityEnum::BRUTE_FORCE);
+        s.set(RunSettingsId::GRAVITY_KERNEL, GravityKernelEnum::SPH_KERNEL);
+        run->solver = makeAuto<GravitySolver<SymmetricSolver<3>>>(*run->scheduler, s, eqs);
+    } else {
+        run->solver = makeAuto<SymmetricSolver<3>>(*run->scheduler, s, eqs);
+    }
 
     // one body (and material instance) per run of equal flags, as addMonolithicBody creates them
     run->storage = makeShared<Storage>();
