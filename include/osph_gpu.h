@@ -47,12 +47,14 @@ enum {
 typedef struct {
     /* Tillotson EoS (TillotsonEos, core/physics/Eos.cpp:184-238) */
     double rho0, A, B, a, b, alpha, beta, u0, uiv, ucv;
-    int yielding;          /* 0 = NONE (EosMaterial), 1 = VON_MISES (SolidMaterial) */
+    int yielding;          /* 0 = NONE (EosMaterial), 1 = VON_MISES, 2 = DRUCKER_PRAGER (SolidMaterial) */
     int damage;            /* 0 = NONE, 1 = SCALAR_GRADY_KIPP */
     double shear_modulus, elasticity_limit, melt_energy, young_modulus;
     /* IMaterial::range (clamping) and IMaterial::minimal (derivative criterion) */
     double rho_lo, rho_hi, u_lo, u_hi, d_lo, d_hi;
     double rho_min, u_min, s_min, d_min;
+    /* DruckerPragerRheology (core/physics/Rheology.cpp:119-181): COHESION, INTERNAL_FRICTION, DRY_FRICTION */
+    double cohesion, internal_friction, dry_friction;
 } osph_material;
 
 /* Run settings read by the hot path (RunSettings, core/system/Settings.cpp:486-715). */

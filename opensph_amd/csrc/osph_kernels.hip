@@ -97,7 +97,8 @@ __device__ void eigenvalues(const double t[6], double out[3]) {
 }
 
 // EosMaterial::initialize (core/sph/Materials.cpp:47-58), VonMisesRheology::initialize
-// (core/physics/Rheology.cpp:36-83) and the h clamp of AdaptiveSmoothingLength::initialize
+// (core/physics/Rheology.cpp:36-83) or DruckerPragerRheology::initialize (core/physics/Rheology.cpp:119-181,
+// without acoustic fluidization) and the h clamp of AdaptiveSmoothingLength::initialize
 // (core/sph/equations/EquationTerm.cpp:356-364).
 __device__ void material_init_at(DevState& s, const osph_material& m, const Params& prm, size_t i) {
 #pragma clang fp contract(off)
@@ -125,6 +126,35 @@ __device__ void material_init_at(DevState& s, const osph_material& m, const Para
             const double dd = (xx * xx + yy * yy + zz * zz) + 2.0 * (xy * xy + xz * xz + yz * yz);
             const double J2 = 0.5 * dd + 1.e-15;
             const double red = dmin(Y / sqrt(3.0 * J2), 1.0);
+            s.d[RED][i] = red;
+            s.d[SXX][i] = xx * red;
+            s.d[SYY][i] = yy * red;
+            s.d[SXY][i] = xy * red;
+            s.d[SXZ][i] = xz * red;
+            s.d[SYZ][i] = yz * red;
+        }
+    } else if (m.yielding == 2) {
+        const double d = prm.has_damage ? s.d[D][i] * s.d[D][i] * s.d[D][i] : 0.0;
+        if (p < 0.0) p = (1.0 - d) * p;
+        const double Y0 = m.cohesion, YM = m.elasticity_limit, mui = m.internal_friction, mud = m.dry_friction;
+        const double Yi = Y0 + mui * p / (1.0 + mui * dmax(p, 0.0) / (YM - Y0));
+        const double Yd = mud * p;
+        double Y = Yd > Yi ? Yi : (1.0 - d) * Yi + d * Yd;
+        const double u = s.d[U][i];
+        Y = u < 1.e-5 * m.melt_energy ? Y : Y * dmax(1.0 - u / m.melt_energy, 0.0);
+        if (Y < EPS_D) {
+            s.d[RED][i] = 0.0;
+            s.d[SXX][i] = 0.0;
+            s.d[SYY][i] = 0.0;
+            s.d[SXY][i] = 0.0;
+            s.d[SXZ][i] = 0.0;
+            s.d[SYZ][i] = 0.0;
+        } else {
+            const double xx = s.d[SXX][i], yy = s.d[SYY][i], zz = -xx - yy;
+            const double xy = s.d[SXY][i], xz = s.d[SXZ][i], yz = s.d[SYZ][i];
+            const double dd = (xx * xx + yy * yy + zz * zz) + 2.0 * (xy * xy + xz * xz + yz * yz);
+            const double J2 = 0.5 * dd + EPS_D;
+            const double red = dmin(Y / sqrt(J2), 1.0);
             s.d[RED][i] = red;
             s.d[SXX][i] = xx * red;
             s.d[SYY][i] = yy * red;
@@ -1054,7 +1084,7 @@ __device__ void finalize_at(DevState& s, const osph_material& m, const Params& p
     s.d[VH][i] = (prm.adaptive_h && h > 2.0 * prm.h_lo) ? h / 3.0 * divv : 0.0;
     s.d[DRHO][i] += -rho * divv;
     double du = s.d[DU][i];
-    if (prm.solid_stress && m.yielding == 1) {
+    if (prm.solid_stress && m.yielding != 0) {
         const double sxx = s.d[SXX][i], syy = s.d[SYY][i], szz = -sxx - syy;
         const double gxx = s.d[GXX][i], gyy = s.d[GYY][i], gzz = s.d[GZZ][i];
         const double gxy = s.d[GXY][i], gxz = s.d[GXZ][i], gyz = s.d[GYZ][i];

@@ -38,7 +38,8 @@ class Material(C.Structure):
         ("yielding", C.c_int), ("damage", C.c_int)] + [
         (k, C.c_double) for k in ("shear_modulus", "elasticity_limit", "melt_energy", "young_modulus",
                                   "rho_lo", "rho_hi", "u_lo", "u_hi", "d_lo", "d_hi",
-                                  "rho_min", "u_min", "s_min", "d_min")]
+                                  "rho_min", "u_min", "s_min", "d_min",
+                                  "cohesion", "internal_friction", "dry_friction")]
 
 
 class Config(C.Structure):
@@ -106,9 +107,11 @@ def kernel_lut() -> Tuple[np.ndarray, np.ndarray]:
 
 
 def _material(m: dict) -> Material:
+    from .state import basalt
+    defaults = basalt()  # parameters a material dict leaves out take the reference's defaults
     out = Material()
     for name, t in Material._fields_:
-        setattr(out, name, t(m[name]).value)
+        setattr(out, name, t(m.get(name, defaults[name])).value)
     return out
 
 

@@ -55,9 +55,9 @@ B_ENERGY_RANGE, B_ENERGY_MIN, B_STRESS_TENSOR_MIN, B_DAMAGE_RANGE, B_DAMAGE_MIN 
 B_BULK_MODULUS, B_TILLOTSON_B, B_TILLOTSON_A_SMALL, B_TILLOTSON_B_SMALL = 21, 22, 23, 24
 B_TILLOTSON_ALPHA, B_TILLOTSON_BETA, B_TILLOTSON_U0, B_TILLOTSON_UIV, B_TILLOTSON_UCV = 25, 26, 27, 28, 29
 B_YIELDING, B_DAMAGE_MODEL, B_SHEAR_MODULUS, B_YOUNG_MODULUS = 33, 34, 35, 36
-B_ELASTICITY_LIMIT, B_MELT_ENERGY = 38, 39
+B_ELASTICITY_LIMIT, B_MELT_ENERGY, B_COHESION, B_INTERNAL_FRICTION, B_DRY_FRICTION = 38, 39, 40, 41, 42
 EOS_TILLOTSON = 4                                   # EosEnum::TILLOTSON
-YIELDING_NONE, YIELDING_VON_MISES = 0, 2            # YieldingEnum
+YIELDING_NONE, YIELDING_VON_MISES, YIELDING_DRUCKER_PRAGER = 0, 2, 3  # YieldingEnum
 FRACTURE_NONE, FRACTURE_SCALAR_GRADY_KIPP = 0, 1    # FractureEnum
 
 FLOAT_PARAMS = {B_DENSITY: "rho0", B_BULK_MODULUS: "A", B_TILLOTSON_B: "B", B_TILLOTSON_A_SMALL: "a",
@@ -66,7 +66,8 @@ FLOAT_PARAMS = {B_DENSITY: "rho0", B_BULK_MODULUS: "A", B_TILLOTSON_B: "B", B_TI
                 B_SHEAR_MODULUS: "shear_modulus", B_YOUNG_MODULUS: "young_modulus",
                 B_ELASTICITY_LIMIT: "elasticity_limit", B_MELT_ENERGY: "melt_energy",
                 B_DENSITY_MIN: "rho_min", B_ENERGY_MIN: "u_min", B_STRESS_TENSOR_MIN: "s_min",
-                B_DAMAGE_MIN: "d_min"}
+                B_DAMAGE_MIN: "d_min", B_COHESION: "cohesion", B_INTERNAL_FRICTION: "internal_friction",
+                B_DRY_FRICTION: "dry_friction"}
 RANGE_PARAMS = {B_DENSITY_RANGE: ("rho_lo", "rho_hi"), B_ENERGY_RANGE: ("u_lo", "u_hi"),
                 B_DAMAGE_RANGE: ("d_lo", "d_hi")}
 
@@ -300,7 +301,8 @@ def material_dict(m: Material) -> dict:
         if pid in ps and ps[pid].type == T_INTERVAL:
             out[lo], out[hi] = (float(x) for x in ps[pid].value)
     if B_YIELDING in ps:
-        out["yielding"] = 1 if int(ps[B_YIELDING].value[0]) == YIELDING_VON_MISES else 0
+        y = int(ps[B_YIELDING].value[0])
+        out["yielding"] = {YIELDING_VON_MISES: 1, YIELDING_DRUCKER_PRAGER: 2}.get(y, 0)
     if B_DAMAGE_MODEL in ps:
         out["damage"] = 1 if int(ps[B_DAMAGE_MODEL].value[0]) == FRACTURE_SCALAR_GRADY_KIPP else 0
     return out
@@ -340,9 +342,11 @@ def _float_param(pid, x):
 def material_params(mat: dict) -> List[Param]:
     """BodySettings entries for one of this package's materials (Tillotson EoS)."""
     ps = [Param(B_EOS, T_ENUM, (EOS_TILLOTSON, 0))]
-    ps += [_float_param(pid, mat[key]) for pid, key in sorted(FLOAT_PARAMS.items())]
+    defaults = basalt()
+    ps += [_float_param(pid, mat.get(key, defaults[key])) for pid, key in sorted(FLOAT_PARAMS.items())]
     ps += [Param(pid, T_INTERVAL, (float(mat[lo]), float(mat[hi]))) for pid, (lo, hi) in sorted(RANGE_PARAMS.items())]
-    ps.append(Param(B_YIELDING, T_ENUM, (YIELDING_VON_MISES if mat.get("yielding") else YIELDING_NONE, 0)))
+    yielding = {1: YIELDING_VON_MISES, 2: YIELDING_DRUCKER_PRAGER}.get(int(mat.get("yielding", 0)), YIELDING_NONE)
+    ps.append(Param(B_YIELDING, T_ENUM, (yielding, 0)))
     ps.append(Param(B_DAMAGE_MODEL, T_ENUM,
                     (FRACTURE_SCALAR_GRADY_KIPP if mat.get("damage") else FRACTURE_NONE, 0)))
     return sorted(ps, key=lambda p: p.id)

@@ -43,6 +43,7 @@ def basalt(**overrides) -> dict:
         shear_modulus=2.27e10, elasticity_limit=3.5e9, melt_energy=3.4e6, young_modulus=5.7e10,
         rho_lo=50.0, rho_hi=INFTY, u_lo=0.0, u_hi=INFTY, d_lo=0.0, d_hi=1.0,
         rho_min=100.0, u_min=1.0, s_min=1.0e5, d_min=0.03,
+        cohesion=9.0e7, internal_friction=2.0, dry_friction=0.8,
     )
     m.update(overrides)
     return m

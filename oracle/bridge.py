@@ -45,7 +45,8 @@ class Material(C.Structure):
         ("yielding", C.c_int), ("damage", C.c_int)] + [
         (k, C.c_double) for k in ("shear_modulus", "elasticity_limit", "melt_energy", "young_modulus",
                                   "rho_lo", "rho_hi", "u_lo", "u_hi", "d_lo", "d_hi",
-                                  "rho_min", "u_min", "s_min", "d_min")]
+                                  "rho_min", "u_min", "s_min", "d_min",
+                                  "cohesion", "internal_friction", "dry_friction")]
 
 
 class Config(C.Structure):
@@ -60,9 +61,11 @@ class Config(C.Structure):
 
 def make_config(cfg, materials):
     mats = (Material * len(materials))()
+    from opensph_amd.state import basalt
+    defaults = basalt()  # parameters a material dict leaves out take the reference's defaults
     for i, m in enumerate(materials):
         for name, _ in Material._fields_:
-            setattr(mats[i], name, type(getattr(mats[i], name))(m[name]))
+            setattr(mats[i], name, type(getattr(mats[i], name))(m.get(name, defaults[name])))
     c = Config()
     for name, _ in Config._fields_:
         if name in ("n_mat", "mat"):

@@ -173,14 +173,55 @@ void orc_eigenvalues(const double t[6], double out[3]) {
     out[2] = (t1 * cos(v2) - p3) * n;
 }
 
+/* DruckerPragerRheology::initialize (core/physics/Rheology.cpp:119-181) for particle i, without acoustic
+ * fluidization (USE_ACOUSTIC_FLUDIZATION defaults to false): the pressure-dependent yield strength
+ * Y_i = Y_0 + mu_i p / (1 + mu_i max(p, 0) / (Y_M - Y_0)), the dry-friction strength mu_d p, their blend
+ * by damage, the melting reduction and the deviatoric stress reduced to Y / sqrt(J2). */
+static void drucker_prager(const orc_config* cfg, const orc_material* m, double* P, size_t n, size_t i) {
+    const double d = cfg->has_damage ? pow3(PL(ORC_D)[i]) : 0.0;
+    if (PL(ORC_P)[i] < 0.0) {
+        PL(ORC_P)[i] = (1.0 - d) * PL(ORC_P)[i];
+    }
+    const double p = PL(ORC_P)[i];
+    const double Y_0 = m->cohesion, Y_M = m->elasticity_limit, mu_i = m->internal_friction, mu_d = m->dry_friction;
+    const double Y_i = Y_0 + mu_i * p / (1.0 + mu_i * maxd(p, 0.0) / (Y_M - Y_0));
+    const double Y_d = mu_d * p;
+    double Y;
+    if (Y_d > Y_i) {
+        Y = Y_i;
+    } else {
+        Y = (1.0 - d) * Y_i + d * Y_d;
+    }
+    const double u = PL(ORC_U)[i];
+    Y = (u < 1.e-5 * m->melt_energy) ? Y : Y * maxd(1.0 - u / m->melt_energy, 0.0);
+    double* s[5] = { PL(ORC_SXX), PL(ORC_SYY), PL(ORC_SXY), PL(ORC_SXZ), PL(ORC_SYZ) };
+    if (Y < EPS_) {
+        PL(ORC_RED)[i] = 0.0;
+        for (int k = 0; k < 5; ++k) s[k][i] = 0.0;
+        return;
+    }
+    const double xx = s[0][i], yy = s[1][i], zz = -s[0][i] - s[1][i];
+    const double dd = (xx * xx + yy * yy + zz * zz) +
+                      2.0 * (s[2][i] * s[2][i] + s[3][i] * s[3][i] + s[4][i] * s[4][i]);
+    const double J2 = 0.5 * dd + EPS_;
+    const double red = mind(Y / sqrt(J2), 1.0);
+    PL(ORC_RED)[i] = red;
+    for (int k = 0; k < 5; ++k) s[k][i] = s[k][i] * red;
+}
+
 /* EosMaterial::initialize (core/sph/Materials.cpp:47-58) followed by VonMisesRheology::initialize
- * (core/physics/Rheology.cpp:36-83) for particle i. */
+ * (core/physics/Rheology.cpp:36-83) or DruckerPragerRheology::initialize (core/physics/Rheology.cpp:119-181,
+ * without acoustic fluidization) for particle i. */
 static void material_initialize(const orc_config* cfg, double* P, size_t n, const uint32_t* I, size_t i) {
     const orc_material* m = &cfg->mat[IPL(ORC_MAT)[i]];
     double p, cs;
     orc_tillotson(m, PL(ORC_RHO)[i], PL(ORC_U)[i], &p, &cs);
     PL(ORC_P)[i] = p;
     PL(ORC_CS)[i] = cs;
+    if (m->yielding == 2) {
+        drucker_prager(cfg, m, P, n, i);
+        return;
+    }
     if (m->yielding != 1) return;
 
     const double d = cfg->has_damage ? pow3(PL(ORC_D)[i]) : 0.0;
@@ -532,7 +573,7 @@ void orc_integrate(const orc_config* cfg, double* P, uint32_t* I, size_t n) {
         /* ContinuityEquation::finalize, STANDARD mode (EquationTerm.cpp:289-299) */
         PL(ORC_DRHO)[i] += -rho * divv;
         /* SolidStressForce::finalize (EquationTerm.cpp:177-203), skipped for YieldingEnum::NONE */
-        if (cfg->solid_stress && m->yielding == 1) {
+        if (cfg->solid_stress && m->yielding != 0) {
             const double sxx = PL(ORC_SXX)[i], syy = PL(ORC_SYY)[i], szz = -sxx - syy;
             const double gxx = PL(ORC_GXX)[i], gyy = PL(ORC_GYY)[i], gzz = PL(ORC_GZZ)[i];
             const double gxy = PL(ORC_GXY)[i], gxz = PL(ORC_GXZ)[i], gyz = PL(ORC_GYZ)[i];

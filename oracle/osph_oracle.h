@@ -49,13 +49,15 @@ enum { ORC_FLAG, ORC_MAT, ORC_NCNT, ORC_NFLAWS, ORC_NI };
 typedef struct {
     /* Tillotson EoS (core/physics/Eos.cpp:184-196) */
     double rho0, A, B, a, b, alpha, beta, u0, uiv, ucv;
-    int yielding;          /* 0 = NONE (EosMaterial), 1 = VON_MISES (SolidMaterial) */
+    int yielding;          /* 0 = NONE (EosMaterial), 1 = VON_MISES, 2 = DRUCKER_PRAGER (SolidMaterial) */
     int damage;            /* 0 = NONE, 1 = SCALAR_GRADY_KIPP */
     double shear_modulus;  /* mu, Hooke's law */
     double elasticity_limit, melt_energy, young_modulus;
     /* IMaterial ranges (clamping) and minimals (derivative criterion) */
     double rho_lo, rho_hi, u_lo, u_hi, d_lo, d_hi;
     double rho_min, u_min, s_min, d_min;
+    /* DruckerPragerRheology (core/physics/Rheology.cpp:119-181): COHESION, INTERNAL_FRICTION, DRY_FRICTION */
+    double cohesion, internal_friction, dry_friction;
 } orc_material;
 
 typedef struct {

@@ -58,7 +58,12 @@ BodySettings bodySettings(const orc_material& m) {
     body.set(BodySettingsId::TILLOTSON_SUBLIMATION, m.u0);
     body.set(BodySettingsId::TILLOTSON_ENERGY_IV, m.uiv);
     body.set(BodySettingsId::TILLOTSON_ENERGY_CV, m.ucv);
-    body.set(BodySettingsId::RHEOLOGY_YIELDING, m.yielding == 1 ? YieldingEnum::VON_MISES : YieldingEnum::NONE);
+    body.set(BodySettingsId::RHEOLOGY_YIELDING, m.yielding == 1   ? YieldingEnum::VON_MISES
+                                                : m.yielding == 2 ? YieldingEnum::DRUCKER_PRAGER
+                                                                  : YieldingEnum::NONE);
+    body.set(BodySettingsId::COHESION, m.cohesion);
+    body.set(BodySettingsId::INTERNAL_FRICTION, m.internal_friction);
+    body.set(BodySettingsId::DRY_FRICTION, m.dry_friction);
     body.set(BodySettingsId::RHEOLOGY_DAMAGE, m.damage == 1 ? FractureEnum::SCALAR_GRADY_KIPP : FractureEnum::NONE);
     body.set(BodySettingsId::SHEAR_MODULUS, m.shear_modulus);
     body.set(BodySettingsId::ELASTICITY_LIMIT, m.elasticity_limit);
@@ -194,7 +199,7 @@ void* ref_create(const orc_config* cfg, const double* P, const uint32_t* I, size
         }
         const orc_material& mat = cfg->mat[matIdx[begin]];
         run->solid = run->solid || (cfg->solid_stress != 0);
-        run->reduce = run->reduce || (mat.yielding == 1);
+        run->reduce = run->reduce || (mat.yielding != 0);
         run->damage = run->damage || (mat.damage == 1);
         SharedPtr<IMaterial> material(Factory::getMaterial(bodySettings(mat)));
         Storage body(material);

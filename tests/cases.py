@@ -38,3 +38,11 @@ def exercised_fluid(seed=3):
     st["rho"][:] = st.materials[0]["rho0"] * rng.uniform(0.9, 1.1, size=st.n)
     st["u"][:] = rng.uniform(0.0, 1.0e7, size=st.n)
     return st, cfg
+
+
+def exercised_impact_drucker_prager(seed=7):
+    """exercised_impact with DruckerPragerRheology: pressures of both signs around the cohesion, the
+    dry-friction branch (mu_d p above the intact strength at high pressure), damage, melting."""
+    st, cfg = exercised_impact(seed=seed)
+    st.materials = [dict(m, yielding=2) for m in st.materials]
+    return st, cfg

@@ -264,3 +264,33 @@ def test_oracle_gravity_solver_matches_live_reference(oracle):
     got = ref.read()
     for k in ("r", "v", "rho", "u"):
         assert rel_l2(so[k], got[k]) < 1e-12, k
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref",
+                                                    "libosph_ref.so")), reason="reference core not built here")
+def test_oracle_drucker_prager_matches_live_reference(oracle):
+    """DruckerPragerRheology (core/physics/Rheology.cpp:119-181) in the oracle against the real reference:
+    one evaluation and three predictor-corrector steps of the exercised impact, 1e-12."""
+    from cases import exercised_impact_drucker_prager
+    from oracle.bridge import RefRun
+
+    st, cfg = exercised_impact_drucker_prager()
+    ref = RefRun(st, cfg)
+    b = ref.integrate()
+    a = oracle.integrate(st, cfg)
+    for k in ("dv", "du", "ds", "p", "reduce", "drho", "ddamage"):
+        x, y = a[k], b[k]
+        if k == "dv":
+            x, y = x[:, :3], y[:, :3]
+        assert rel_l2(x, y) < 1e-12, k
+    red = a["reduce"]
+    assert np.any(red < 1.0) and np.any(red == 1.0)  # yielding and elastic particles
+    ref = RefRun(st, cfg)
+    rlog = [ref.step() for _ in range(3)]
+    so, olog = oracle.run_pc(st, cfg, 3)
+    for (dt, nxt, crit), (odt, onxt, ocrit) in zip(rlog, olog):
+        assert onxt == pytest.approx(nxt, rel=1e-12)
+        assert crit == ocrit
+    got = ref.read()
+    for k in ("r", "v", "rho", "u", "s", "damage"):
+        assert rel_l2(so[k], got[k]) < 1e-12, k

@@ -78,6 +78,7 @@ def test_state_from_reference_file():
 
 def test_state_written_and_read_back(tmp_path):
     st, cfg = initial.impact(n_target=3000, n_projectile=300, seed=3)
+    st.materials = [dict(m, yielding=2, cohesion=8.5e7) for m in st.materials]  # Drucker-Prager
     rng = np.random.default_rng(4)
     for k in ("v", "dv", "drho", "du", "s", "ds", "ddamage", "p", "cs", "gradv", "divv", "reduce"):
         st.a[k][...] = rng.normal(size=st.a[k].shape)
@@ -95,7 +96,8 @@ def test_state_written_and_read_back(tmp_path):
     for a, b in zip(back.materials, st.materials):
         for key in ("rho0", "A", "B", "a", "b", "alpha", "beta", "u0", "uiv", "ucv", "shear_modulus",
                     "elasticity_limit", "melt_energy", "young_modulus", "rho_lo", "rho_hi", "u_lo", "u_hi",
-                    "d_lo", "d_hi", "rho_min", "u_min", "s_min", "d_min", "yielding", "damage"):
+                    "d_lo", "d_hi", "rho_min", "u_min", "s_min", "d_min", "yielding", "damage", "cohesion",
+                    "internal_friction", "dry_friction"):
             assert a[key] == b[key], key
     # the ranges the reference's material reads back (Interval(lower, upper) where lower < upper)
     dens = [r for r in g.materials[0].ranges if r[0] == ssf.DENSITY][0]
