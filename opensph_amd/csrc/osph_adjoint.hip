@@ -191,64 +191,72 @@ struct PairGrad {
 //   da = m_q (-(p_o/rho_o^2 + p_q/rho_q^2) - Pi) G,  ddivv = m_q/rho_o w.G,  dheat = -m_q Pi w.G / 2,
 //   G = grad W(r) = r h^-5 g(|r|^2/h^2),  w = v_q - v_o,
 //   Pi = (-alpha cbar mu + beta mu^2)/rhobar when (v_o - v_q).r < 0, mu = h (v_o - v_q).r/(|r|^2 + eps h^2).
-__device__ void pair_adjoint(const AdjRec& o, const AdjRec& q, const double* __restrict__ lut, double alpha,
-                             double beta, PairGrad& out) {
+// T is the arithmetic: double for the FP64 adjoint; float for the mixed one, whose separation r and
+// velocity difference w are taken in FP64 and rounded once (a close pair's separation is a small
+// difference of large coordinates), and whose results the caller adds to FP64 sums.
+template <typename T>
+__device__ void pair_adjoint(const AdjRec& o, const AdjRec& q, const double* __restrict__ lut, double alpha_d,
+                             double beta_d, PairGrad& out) {
 #pragma clang fp contract(off)
-    const double r[3] = { o.x - q.x, o.y - q.y, o.z - q.z };
-    const double w[3] = { q.vx - o.vx, q.vy - o.vy, q.vz - o.vz };
-    const double hbar = 0.5 * (o.h + q.h);
-    const double hInv = 1.0 / hbar;
-    const double hInv2 = hInv * hInv;
-    const double s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-    double g, dg;
-    lut_grad_slope(lut, s * hInv2, g, dg);
-    const double h5 = (hInv2 * hInv2) * hInv;
-    const double K = h5 * g, Kp = h5 * dg * hInv2;  // G = K r, dK/ds = Kp
-    const double G[3] = { K * r[0], K * r[1], K * r[2] };
-    const double wg = w[0] * G[0] + w[1] * G[1] + w[2] * G[2];
-    const double dvdr = -(w[0] * r[0] + w[1] * r[1] + w[2] * r[2]);
-    const double Po = o.p / (o.rho * o.rho), Pq = q.p / (q.rho * q.rho);
-    const double mq = q.m;
-    double Pi = 0.0, mu = 0.0, den = 1.0, rhobar = 1.0, csbar = 0.0;
-    const bool conv = dvdr < 0.0;
+    const T alpha = T(alpha_d), beta = T(beta_d);
+    const T r[3] = { T(o.x - q.x), T(o.y - q.y), T(o.z - q.z) };
+    const T w[3] = { T(q.vx - o.vx), T(q.vy - o.vy), T(q.vz - o.vz) };
+    const T oh = T(o.h), qh = T(q.h), orho = T(o.rho), qrho = T(q.rho), op = T(o.p), qp = T(q.p);
+    const T hbar = T(0.5) * (oh + qh);
+    const T hInv = T(1) / hbar;
+    const T hInv2 = hInv * hInv;
+    const T s = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double gd, dgd;
+    lut_grad_slope(lut, double(s * hInv2), gd, dgd);
+    const T g = T(gd), dg = T(dgd);
+    const T h5 = (hInv2 * hInv2) * hInv;
+    const T K = h5 * g, Kp = h5 * dg * hInv2;  // G = K r, dK/ds = Kp
+    const T G[3] = { K * r[0], K * r[1], K * r[2] };
+    const T wg = w[0] * G[0] + w[1] * G[1] + w[2] * G[2];
+    const T dvdr = -(w[0] * r[0] + w[1] * r[1] + w[2] * r[2]);
+    const T Po = op / (orho * orho), Pq = qp / (qrho * qrho);
+    const T mq = T(q.m);
+    T Pi = T(0), mu = T(0), den = T(1), rhobar = T(1), csbar = T(0);
+    const bool conv = dvdr < T(0);
     if (conv) {
-        rhobar = 0.5 * (o.rho + q.rho);
-        csbar = 0.5 * (o.cs + q.cs);
-        den = s + 1.e-2 * (hbar * hbar);
+        rhobar = T(0.5) * (orho + qrho);
+        csbar = T(0.5) * (T(o.cs) + T(q.cs));
+        den = s + T(1.e-2) * (hbar * hbar);
         mu = hbar * dvdr / den;
         Pi = (-alpha * csbar * mu + beta * (mu * mu)) / rhobar;
     }
-    const double la[3] = { o.lax, o.lay, o.laz };
-    const double A = la[0] * G[0] + la[1] * G[1] + la[2] * G[2];
-    const double C = -(Po + Pq) - Pi;
-    const double gPi = -mq * A - 0.5 * mq * o.lh * wg;
-    const double gP = -mq * A;
-    const double cw = mq * o.ld / o.rho - 0.5 * mq * o.lh * Pi;  // coefficient of w in dL/dG and of G in dL/dw
-    double gG[3], gw[3];
+    const T la[3] = { T(o.lax), T(o.lay), T(o.laz) };
+    const T old = T(o.ld), olh = T(o.lh);
+    const T A = la[0] * G[0] + la[1] * G[1] + la[2] * G[2];
+    const T C = -(Po + Pq) - Pi;
+    const T gPi = -mq * A - T(0.5) * mq * olh * wg;
+    const T gP = -mq * A;
+    const T cw = mq * old / orho - T(0.5) * mq * olh * Pi;  // coefficient of w in dL/dG and of G in dL/dw
+    T gG[3], gw[3];
     for (int k = 0; k < 3; ++k) {
         gG[k] = mq * C * la[k] + cw * w[k];
         gw[k] = cw * G[k];
     }
-    double gmu = 0.0, gc = 0.0, grb = 0.0, gdvdr = 0.0, gs = 0.0;
+    T gmu = T(0), gc = T(0), grb = T(0), gdvdr = T(0), gs = T(0);
     if (conv) {
-        gmu = gPi * (-alpha * csbar + 2.0 * beta * mu) / rhobar;
+        gmu = gPi * (-alpha * csbar + T(2) * beta * mu) / rhobar;
         gc = gPi * (-alpha * mu / rhobar);
         grb = gPi * (-Pi / rhobar);
         gdvdr = gmu * hbar / den;
         gs = gmu * (-mu / den);
     }
-    const double gGr = gG[0] * r[0] + gG[1] * r[1] + gG[2] * r[2];
+    const T gGr = gG[0] * r[0] + gG[1] * r[1] + gG[2] * r[2];
     for (int k = 0; k < 3; ++k) {
-        out.gr[k] = K * gG[k] + 2.0 * Kp * gGr * r[k] + 2.0 * gs * r[k] - gdvdr * w[k];
-        out.gw[k] = gw[k] - gdvdr * r[k];
+        out.gr[k] = double(K * gG[k] + T(2) * Kp * gGr * r[k] + T(2) * gs * r[k] - gdvdr * w[k]);
+        out.gw[k] = double(gw[k] - gdvdr * r[k]);
     }
-    const double ro3 = o.rho * o.rho * o.rho, rq3 = q.rho * q.rho * q.rho;
-    out.p_o = gP / (o.rho * o.rho);
-    out.p_q = gP / (q.rho * q.rho);
-    out.rho_o = -(mq * o.ld / (o.rho * o.rho)) * wg - 2.0 * o.p * gP / ro3 + 0.5 * grb;
-    out.rho_q = -2.0 * q.p * gP / rq3 + 0.5 * grb;
-    out.c_o = 0.5 * gc;
-    out.c_q = 0.5 * gc;
+    const T ro3 = orho * orho * orho, rq3 = qrho * qrho * qrho;
+    out.p_o = double(gP / (orho * orho));
+    out.p_q = double(gP / (qrho * qrho));
+    out.rho_o = double(-(mq * old / (orho * orho)) * wg - T(2) * op * gP / ro3 + T(0.5) * grb);
+    out.rho_q = double(-T(2) * qp * gP / rq3 + T(0.5) * grb);
+    out.c_o = double(T(0.5) * gc);
+    out.c_q = double(T(0.5) * gc);
 }
 
 // The adjoint planes of one run (slot order): lambda of X (13), the gathered gradients (9).
@@ -299,6 +307,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_seed(DevState s, AdjPlanes ap, co
 
 // F's vector-Jacobian product gathered per particle over its neighbour list: its own pair terms (owner
 // = itself) and the terms each neighbour receives from it (owner = the neighbour).
+template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_adj_pairs(AdjPlanes ap, const AdjRec* __restrict__ rec,
                                                      const uint32_t* __restrict__ order,
                                                      const uint32_t* __restrict__ list,
@@ -320,8 +329,8 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pairs(AdjPlanes ap, const AdjRec*
         for (uint32_t t = 0; t < 4 && c + t < listed; ++t) {
             const AdjRec o = rec[ent[t]];
             PairGrad a, b;
-            pair_adjoint(me, o, lut, alpha, beta, a);  // this particle's own terms
-            pair_adjoint(o, me, lut, alpha, beta, b);  // the neighbour's terms from this particle
+            pair_adjoint<T>(me, o, lut, alpha, beta, a);  // this particle's own terms
+            pair_adjoint<T>(o, me, lut, alpha, beta, b);  // the neighbour's terms from this particle
             for (int d = 0; d < 3; ++d) {
                 gr[d] += a.gr[d] - b.gr[d];  // r = r_me - r_o in a, r_o - r_me in b
                 gv[d] += -a.gw[d] + b.gw[d];  // w = v_o - v_me in a, v_me - v_o in b
@@ -454,14 +463,18 @@ size_t adj_rec_bytes() { return sizeof(AdjRec); }
 void launch_adj_step(const Launch& l, DevState s, double* const* lplanes, double* const* gplanes,
                      const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax,
                      const double* lut, double alpha, double beta, const osph_material* mats, int nmat, double dt,
-                     void* rec, double* part, double* theta, unsigned* clamped, size_t n) {
+                     void* rec, double* part, double* theta, unsigned* clamped, size_t n, bool fp32) {
     if (!n) return;
     AdjPlanes ap;
     for (int c = 0; c < 13; ++c) ap.l[c] = lplanes[c];
     for (int c = 0; c < 9; ++c) ap.g[c] = gplanes[c];
     AdjRec* r = static_cast<AdjRec*>(rec);
     k_adj_seed<<<grid_for(n), BLOCK, 0, l.stream>>>(s, ap, order, dt, r, n);
-    k_adj_pairs<<<grid_for(n), BLOCK, 0, l.stream>>>(ap, r, order, list, count, lut, alpha, beta, kmax, n);
+    if (fp32) {
+        k_adj_pairs<float><<<grid_for(n), BLOCK, 0, l.stream>>>(ap, r, order, list, count, lut, alpha, beta, kmax, n);
+    } else {
+        k_adj_pairs<double><<<grid_for(n), BLOCK, 0, l.stream>>>(ap, r, order, list, count, lut, alpha, beta, kmax, n);
+    }
     const unsigned g = grid_for(n);
     k_adj_finish<<<g, BLOCK, 0, l.stream>>>(s, ap, mats, nmat, dt, part, clamped, n);
     k_adj_theta<<<ADJ_MAX_MATERIALS * 3, BLOCK, 0, l.stream>>>(part, g, theta);

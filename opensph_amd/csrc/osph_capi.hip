@@ -1329,7 +1329,7 @@ int osph_adjoint_backward(osph_ctx* c, const double* seed_r, const double* seed_
             timed(c, "adjoint", [&] {
                 launch_adj_step(l, c->s, lp, gp, c->order, c->list, c->count, c->kmax, c->lut, c->cfg.av_alpha,
                                 c->cfg.av_beta, c->mats_dev, int(c->mats.size()), c->adj.dt[k], rec.p, part.p,
-                                theta.p, clamp_count.p, n);
+                                theta.p, clamp_count.p, n, c->cfg.precision == OSPH_PRECISION_MIXED);
             });
         }
     }

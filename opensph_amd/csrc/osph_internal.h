@@ -208,11 +208,12 @@ size_t adj_rec_bytes();
 size_t adj_partials(size_t n);
 // One reverse PC step: lplanes (13: lambda of r xyz, v xyz, a xyz, rho, drho, u, du) hold the adjoints of
 // X_{n+1} and receive those of X_n; the state holds the predicted X* with the derivatives evaluated at it
-// (and its neighbour lists); gplanes (9) are scratch; theta[3 mat + q] accumulates d/d(a, b, A).
+// (and its neighbour lists); gplanes (9) are scratch; theta[3 mat + q] accumulates d/d(a, b, A).  fp32:
+// the pair terms in FP32 (FP64 separations, FP64 sums), the mixed adjoint.
 void launch_adj_step(const Launch& l, DevState s, double* const* lplanes, double* const* gplanes,
                      const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax,
                      const double* lut, double alpha, double beta, const osph_material* mats, int nmat, double dt,
-                     void* rec, double* part, double* theta, unsigned* clamped, size_t n);
+                     void* rec, double* part, double* theta, unsigned* clamped, size_t n, bool fp32);
 // The 13 planes of X (r, v, a, rho, drho, u, du) to and from a block of 13 planes of capacity cap.
 void launch_save_x(const Launch& l, DevState s, double* block, size_t cap, size_t n);
 void launch_restore_x(const Launch& l, DevState s, const double* block, size_t cap, size_t n);

@@ -123,3 +123,19 @@ def test_mixed_precision_gradients_close_to_fp64(fp64_run):
     for k in ("r", "v", "rho", "u"):
         err = np.linalg.norm(mixed[k] - out[k]) / np.linalg.norm(out[k])
         assert err < 1e-3, (k, err)
+
+
+def test_mixed_adjoint_matches_central_differences(fp64_run, oracle):
+    """The mixed adjoint (FP32 pair terms with FP64 separations and sums, FP64 elsewhere) against the
+    oracle's central differences, for the initial velocities and densities."""
+    st, cfg, out, S, fin, back = fp64_run
+    mixed, _, _, _ = _gpu_gradients(st, cfg, precision="mixed", S=S)
+    rng = np.random.default_rng(5)
+    for what, eps in (("v", 1e-4), ("rho", 1e-3)):
+        grad = mixed[what]
+        scale = np.abs(grad).max()
+        for c in _components(grad, 3, rng):
+            def perturb(s, e, c=c, what=what):
+                s[what][c] += e
+            fd = _fd(oracle, st, cfg, S, perturb, eps)
+            assert abs(fd - grad[c]) <= 5e-5 * scale, (what, c, fd, grad[c])
