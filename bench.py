@@ -232,9 +232,10 @@ def nbody_once(args, prec):
     return out
 
 
-def ensemble_members(count, seed=11):
+def ensemble_members(count, seed=11, flaws=True):
     """Impact ensemble: N ~ exp(U(ln 5e3, ln 2e5)), R_p/R_t ~ U(0.05, 0.2) at equal particle spacing,
-    angle ~ U(0, 90) deg, speed ~ U(3, 7) km/s, R_t = 50 km, basalt; one seed per member."""
+    angle ~ U(0, 90) deg, speed ~ U(3, 7) km/s, R_t = 50 km, basalt; one seed per member.  flaws=False
+    leaves the Grady-Kipp flaws to the device (initial.device_flaws after the upload)."""
     from opensph_amd import initial
 
     rng = np.random.default_rng(seed)
@@ -246,7 +247,7 @@ def ensemble_members(count, seed=11):
         n_t = max(100, int(n) - n_p)
         st, cfg = initial.impact(n_target=n_t, n_projectile=n_p, r_target=50e3, r_projectile=50e3 * ratio,
                                  speed=rng.uniform(3e3, 7e3), angle_deg=rng.uniform(0.0, 90.0),
-                                 seed=int(rng.integers(1 << 30)), flaws=True)
+                                 seed=int(rng.integers(1 << 30)), flaws=flaws)
         out.append((st, cfg))
     return out
 
@@ -255,11 +256,22 @@ def run_ensemble(args):
     """Impact ensemble: all members stepped together (osph_step_pc_batch) against one after another."""
     from opensph_amd import gpu
 
+    from opensph_amd import initial
+
+    # set-up: lattices and placement on the host, the Grady-Kipp flaws drawn on the device
     t0 = time.perf_counter()
-    ms = ensemble_members(args.ensemble_n)
-    total = sum(st.n for st, _ in ms)
-    log(f"ensemble of {len(ms)} members, {total} particles, set up in {time.perf_counter() - t0:.1f}s")
+    ms = ensemble_members(args.ensemble_n, flaws=False)
+    t_host = time.perf_counter() - t0
     solvers = [gpu.GpuSolver(st, cfg, precision="mixed") for st, cfg in ms]
+    t_upload = time.perf_counter() - t0 - t_host
+    for k, (s, (st, _)) in enumerate(zip(solvers, ms)):
+        initial.device_flaws(s, st, seed=1000 * k)
+    for s in solvers:
+        s.sync()
+    t_setup = time.perf_counter() - t0
+    total = sum(st.n for st, _ in ms)
+    log(f"ensemble of {len(ms)} members, {total} particles, set up in {t_setup:.1f}s (host {t_host:.1f}s, "
+        f"upload {t_upload:.1f}s, device flaws {t_setup - t_host - t_upload:.2f}s)")
     for _ in range(args.warmup):
         gpu.step_batch(solvers)
     for s in solvers:
@@ -283,7 +295,8 @@ def run_ensemble(args):
            "warmup": args.warmup, "ms_per_step": 1e3 * batched / args.steps, "higher_is_better": True,
            "dtype": "f32+f64", "data": "synthetic",
            "config": {"workload": "impact ensemble", "members": len(ms), "particles": total, "device": name},
-           "sequential_value": total * args.steps / sequential}
+           "sequential_value": total * args.steps / sequential,
+           "setup_s": {"host_lattices": t_host, "upload": t_upload, "device_flaws": t_setup - t_host - t_upload}}
     print(json.dumps(out), flush=True)
 
 

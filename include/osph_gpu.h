@@ -139,6 +139,18 @@ int osph_timestep(osph_ctx* ctx, double* dt, int* criterion);
  * reference counterpart: the reference runs ensembles as separate jobs. */
 int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* next_dt, int* criterion);
 
+/* Replaces ScalarGradyKippModel::setFlaws (core/physics/Damage.cpp:17-126, accumulate mode) for the
+ * particles [first, first + count) of one body, on the resident state: flaws p = 1, 2, ... each hit a
+ * uniformly drawn particle of the body until every particle holds one (the draw that flaws the last
+ * particle ends the sequence); a particle's EPS_MIN / M_ZERO / N_FLAWS follow from its first and last
+ * flaw and their count, eps(p) = (p / (k_weibull volume))^(1/m_weibull), and EXPLICIT_GROWTH =
+ * growth_speed / (2 h).  volume = the body's sum of m / rho; growth_speed = the Rayleigh fraction of the
+ * longitudinal sound speed.  The draws come from a counter-based generator (splitmix64 of seed + p times
+ * the golden-ratio increment), not from the reference's BenzAsphaugRng: the same seed gives the same flaws
+ * on every device, whatever the particle order in HBM.  The state must hold damage quantities. */
+int osph_set_flaws(osph_ctx* ctx, size_t first, size_t count, double volume, double k_weibull, double m_weibull,
+                   double growth_speed, uint64_t seed);
+
 /* Replaces GravitySolver<SymmetricSolver<3>> (core/sph/solvers/GravitySolver.cpp:64-91): with a self-gravity
  * context attached (osph_gravity.h), every derivative evaluation of ctx (osph_integrate, osph_step_pc and
  * the batch) first evaluates the self-gravity of the current positions, smoothing lengths and masses

@@ -1132,6 +1132,48 @@ int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* nex
     return OSPH_OK;
 }
 
+int osph_set_flaws(osph_ctx* c, size_t first, size_t count, double volume, double k_weibull, double m_weibull,
+                    double growth_speed, uint64_t seed) {
+    if (!c) return OSPH_ERR_ARG;
+    if (first > c->n || count > c->n - first) return c->fail(OSPH_ERR_ARG, "particle range outside the state");
+    if (!c->cfg.has_damage) return c->fail(OSPH_ERR_STATE, "the state holds no damage quantities");
+    if (!(volume > 0.0) || !(k_weibull > 0.0) || !(m_weibull > 0.0) || !(growth_speed >= 0.0)) {
+        return c->fail(OSPH_ERR_ARG, "volume, Weibull coefficient and exponent must be positive");
+    }
+    if (count == 0) return OSPH_OK;
+    if (count >= (size_t(1) << 27)) return c->fail(OSPH_ERR_ARG, "a body must hold fewer than 2^27 particles");
+    const uint32_t nb = uint32_t(count);
+    Launch l = L(c);
+    DevBuf<uint32_t> buf;
+    struct StreamIdle {
+        hipStream_t s;
+        ~StreamIdle() { (void)hipStreamSynchronize(s); }
+    } idle{ c->stream };
+    HIPCHK(c, buf.alloc(3 * size_t(nb) + 2));
+    uint32_t *fst = buf.p, *lst = buf.p + nb, *cnt = buf.p + 2 * size_t(nb), *acc = buf.p + 3 * size_t(nb);
+    launch_clear_u32(l, fst, nb, 0xffffffffu);
+    launch_clear_u32(l, lst, 2 * size_t(nb), 0u);  // last and count
+    // draw flaws in batches until every particle holds one; the draw that flaws the last particle ends
+    // the sequence (K = the largest first flaw), as ScalarGradyKippModel::setFlaws stops there
+    uint64_t drawn = 0;
+    uint32_t host[2] = { 0, 0 };
+    for (;;) {
+        const uint64_t batch = 4 * uint64_t(nb);
+        if (drawn + batch >= 0xffffffffull) return c->fail(OSPH_ERR_STATE, "flaw count exceeds 2^32");
+        flaws_first_pass(l, seed, drawn + 1, batch, nb, fst);
+        drawn += batch;
+        flaws_cover(l, fst, nb, acc);
+        HIPCHK(c, hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (host[0] == 0) break;
+    }
+    const uint32_t K = host[1];
+    flaws_count_pass(l, seed, K, nb, cnt, lst);
+    const double denom = 1.0 / (std::pow(k_weibull, 1.0 / m_weibull) * std::pow(volume, 1.0 / m_weibull));
+    flaws_finish(l, c->s, c->slot_of, first, nb, fst, lst, cnt, denom, 1.0 / m_weibull, growth_speed);
+    return sync_check(c);
+}
+
 int osph_attach_gravity(osph_ctx* c, osph_gravity* g) {
     if (!c) return OSPH_ERR_ARG;
     if (g && c->adj.ready) return c->fail(OSPH_ERR_UNSUPPORTED, "the adjoint has no self-gravity");

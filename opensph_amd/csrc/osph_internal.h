@@ -280,6 +280,17 @@ size_t sort_pairs_temp_bytes(size_t n, int key_bits);
 hipError_t sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
                       const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits);
 
+// Grady-Kipp flaws of one body on the device (osph_set_flaws): flaw p (1-based) hits particle
+// flaw_target(seed, p, nb); the first pass records first[i] = min p (first initialised to ~0), cover
+// counts the particles still without a flaw and the largest first flaw into acc[0], acc[1]; the count
+// pass records the count and the last flaw of p <= K; finish writes the damage planes.
+int flaws_first_pass(const Launch& l, uint64_t seed, uint64_t p0, uint64_t count, uint32_t nb, uint32_t* first);
+void flaws_cover(const Launch& l, const uint32_t* first, uint32_t nb, uint32_t* acc);
+void flaws_count_pass(const Launch& l, uint64_t seed, uint32_t K, uint32_t nb, uint32_t* cnt, uint32_t* last);
+void flaws_finish(const Launch& l, DevState s, const uint32_t* slot_of, size_t body_first, uint32_t nb,
+                  const uint32_t* first, const uint32_t* last, const uint32_t* cnt, double denom, double inv_m,
+                  double cg);
+
 }  // namespace osph
 
 // The gravity half of GravitySolver::loop for particles in another context's device planes (defined with

@@ -1802,3 +1802,102 @@ hipError_t sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint
 }
 
 }  // namespace osph
+
+// ---- Grady-Kipp flaws on the device ------------------------------------------------------------------
+
+namespace osph {
+namespace {
+
+// Uniform particle index of flaw p (1-based) of a body of nb particles: splitmix64 of seed + p times the
+// golden-ratio increment, mapped to [0, nb) by the high word of a 64 x 64-bit product.
+__device__ __forceinline__ uint32_t flaw_target(uint64_t seed, uint64_t p, uint32_t nb) {
+    uint64_t x = seed + p * 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return uint32_t(__umul64hi(x, uint64_t(nb)));
+}
+
+// Flaws p0 .. p0 + count - 1: the first flaw of each hit particle.
+__global__ void __launch_bounds__(BLOCK) k_flaws_first(uint64_t seed, uint64_t p0, uint64_t count, uint32_t nb,
+                                                       uint32_t* __restrict__ first) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint64_t p = p0 + t;
+    atomicMin(&first[flaw_target(seed, p, nb)], uint32_t(p));
+}
+
+// Particles without a flaw and the largest first flaw (acc[0], acc[1]); one atomic per wavefront.
+__global__ void __launch_bounds__(BLOCK) k_flaws_cover(const uint32_t* __restrict__ first, uint32_t nb,
+                                                       uint32_t* __restrict__ acc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t f = i < nb ? first[i] : 0u;
+    const bool open = i < nb && f == 0xffffffffu;
+    uint32_t mx = open ? 0u : f;
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, uint32_t(__shfl_xor(int(mx), o, 64)));
+    const unsigned long long ballot = __ballot(open);
+    if ((threadIdx.x & 63) == 0) {
+        const uint32_t c = uint32_t(__popcll(ballot));
+        if (c) atomicAdd(&acc[0], c);
+        atomicMax(&acc[1], mx);
+    }
+}
+
+// Flaws 1 .. K: count and last flaw of each particle.
+__global__ void __launch_bounds__(BLOCK) k_flaws_count(uint64_t seed, uint32_t K, uint32_t nb,
+                                                       uint32_t* __restrict__ cnt, uint32_t* __restrict__ last) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= K) return;
+    const uint64_t p = t + 1;
+    const uint32_t i = flaw_target(seed, p, nb);
+    atomicAdd(&cnt[i], 1u);
+    atomicMax(&last[i], uint32_t(p));
+}
+
+// ScalarGradyKippModel::setFlaws (core/physics/Damage.cpp:17-126), accumulate mode, per particle of the
+// body: eps_min / eps_max of its first / last flaw, eps(p) = denom p^(1/m); m_zero = ln n / ln(eps_max /
+// eps_min) for n > 1 flaws, else 1; growth = cg / (2 h).
+__global__ void __launch_bounds__(BLOCK) k_flaws_finish(DevState s, const uint32_t* __restrict__ slot_of,
+                                                        size_t body_first, uint32_t nb,
+                                                        const uint32_t* __restrict__ first,
+                                                        const uint32_t* __restrict__ last,
+                                                        const uint32_t* __restrict__ cnt, double denom,
+                                                        double inv_m, double cg) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const uint32_t slot = slot_of[body_first + i];
+    const double emin = denom * pow(double(first[i]), inv_m);
+    const double emax = denom * pow(double(last[i]), inv_m);
+    const uint32_t n = cnt[i];
+    s.d[EPSMIN][slot] = emin;
+    s.d[MZERO][slot] = n > 1 ? log(double(n)) / log(emax / emin) : 1.0;
+    s.d[GROWTH][slot] = cg / (2.0 * s.d[H][slot]);
+    s.u[NFLAWS][slot] = n;
+}
+
+}  // namespace
+
+int flaws_first_pass(const Launch& l, uint64_t seed, uint64_t p0, uint64_t count, uint32_t nb, uint32_t* first) {
+    if (!count) return 0;
+    const uint64_t blocks = (count + BLOCK - 1) / BLOCK;
+    k_flaws_first<<<unsigned(blocks), BLOCK, 0, l.stream>>>(seed, p0, count, nb, first);
+    return 0;
+}
+
+void flaws_cover(const Launch& l, const uint32_t* first, uint32_t nb, uint32_t* acc) {
+    k_fill_u32<<<1, 64, 0, l.stream>>>(acc, 2, 0u);
+    k_flaws_cover<<<grid_for(nb), BLOCK, 0, l.stream>>>(first, nb, acc);
+}
+
+void flaws_count_pass(const Launch& l, uint64_t seed, uint32_t K, uint32_t nb, uint32_t* cnt, uint32_t* last) {
+    k_flaws_count<<<unsigned((uint64_t(K) + BLOCK - 1) / BLOCK), BLOCK, 0, l.stream>>>(seed, K, nb, cnt, last);
+}
+
+void flaws_finish(const Launch& l, DevState s, const uint32_t* slot_of, size_t body_first, uint32_t nb,
+                  const uint32_t* first, const uint32_t* last, const uint32_t* cnt, double denom, double inv_m,
+                  double cg) {
+    k_flaws_finish<<<grid_for(nb), BLOCK, 0, l.stream>>>(s, slot_of, body_first, nb, first, last, cnt, denom, inv_m,
+                                                          cg);
+}
+
+}  // namespace osph

@@ -30,7 +30,7 @@ EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
            "osph_stream_copy", "osph_tune", "osph_integrals", "osph_step_pc_batch",
            "osph_profile_enable", "osph_profile_read", "osph_adjoint_forward", "osph_adjoint_backward",
-           "osph_adjoint_clear", "osph_attach_gravity"]
+           "osph_adjoint_clear", "osph_attach_gravity", "osph_set_flaws"]
 
 
 class Material(C.Structure):
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_uint)]
     L.osph_adjoint_clear.argtypes = [vp]
     L.osph_attach_gravity.argtypes = [vp, vp]
+    L.osph_set_flaws.argtypes = [vp, sz, sz, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.osph_profile_enable.argtypes = [vp, i]
     L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
     _lib = L
@@ -191,6 +192,13 @@ class GpuSolver:
                        theta=self.cfg.gravity_theta, order=self.cfg.gravity_order, leaf_size=self.cfg.gravity_leaf,
                        G=G_SI if brute else self.cfg.gravity_G, precision=self.precision)
         self.attach_gravity(g)
+
+    def set_flaws(self, first: int, count: int, volume: float, growth_speed: float, seed: int,
+                  k_weibull: float = 4.0e35, m_weibull: float = 9.0):
+        """osph_set_flaws: Grady-Kipp flaws of the body [first, first + count) drawn on the device."""
+        self._check(self.L.osph_set_flaws(self.ctx, int(first), int(count), float(volume), float(k_weibull),
+                                          float(m_weibull), float(growth_speed), int(seed) & ((1 << 64) - 1)),
+                    "osph_set_flaws")
 
     def attach_gravity(self, g):
         """osph_attach_gravity: g (a GpuGravity, or None to detach) evaluates the self-gravity of every

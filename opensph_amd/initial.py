@@ -156,6 +156,29 @@ def grady_kipp_flaws(r: np.ndarray, m: np.ndarray, rho: np.ndarray, material: di
     return eps_min, m_zero, growth, n_flaws
 
 
+def bodies(st: ParticleState) -> List[Tuple[int, int]]:
+    """(first, count) of each body: the runs of equal FLAG, as the bodies are laid out in storage."""
+    flag = np.asarray(st["flag"])
+    if flag.size == 0:
+        return []
+    cuts = np.concatenate([[0], np.flatnonzero(np.diff(flag)) + 1, [flag.size]])
+    return [(int(a), int(b - a)) for a, b in zip(cuts[:-1], cuts[1:])]
+
+
+def device_flaws(solver, st: ParticleState, seed: int, k_weibull: float = 4.0e35, m_weibull: float = 9.0,
+                 rayleigh: float = 0.4):
+    """The Grady-Kipp flaws of every body of `st` (resident in `solver`) drawn on the device
+    (osph_set_flaws): each body's volume sum(m / rho) and crack-growth speed as grady_kipp_flaws takes
+    them; body b draws with seed + b."""
+    for b, (first, count) in enumerate(bodies(st)):
+        sl = slice(first, first + count)
+        material = st.materials[int(st["mat"][first])]
+        mu, A, rho0 = material["shear_modulus"], material["A"], material["rho0"]
+        cg = rayleigh * math.sqrt((A + 4.0 / 3.0 * mu) / rho0)
+        V = float(np.cumsum(st["m"][sl] / st["rho"][sl])[-1])
+        solver.set_flaws(first, count, V, cg, seed + b, k_weibull=k_weibull, m_weibull=m_weibull)
+
+
 def kernel_w0() -> float:
     """Central value of the M4 LUT kernel, LutKernel::valueImpl(0) = CubicSpline<3>::valueImpl(0)."""
     q = 0.0
