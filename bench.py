@@ -37,8 +37,8 @@ FLOP_PER_QUADRUPOLE = 61.0
 # HBM bytes per launch of the pair kernel at the default workload and tuning, from two rocprofv3 --pmc
 # passes over this bench (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000: the factors the 1 GiB device
 # copy of --calibrate reads).  None where not measured.
-MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (473438 + 97460) * 1024.0,
-                          "pairs_fp64_finalize_correct": (1168525 + 272585) * 1024.0}
+MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (473765 + 97454) * 1024.0,
+                          "pairs_fp64_finalize_correct": (1170388 + 272513) * 1024.0}
 
 
 def log(*a):

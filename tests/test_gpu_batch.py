@@ -48,3 +48,30 @@ def test_batch_with_an_empty_member_and_errors():
     assert nxt[0] > 0 and nxt[1] == cfg.max_dt
     with pytest.raises(gpu.GpuError):
         gpu.step_batch([a, a])
+
+
+def test_batch_with_self_gravity_equals_single_steps():
+    """Members with an attached self-gravity (Barnes-Hut and the direct sum, FP64 and mixed) step in a
+    batch exactly as alone: each member's gravity is evaluated on its own predicted state."""
+    import dataclasses
+
+    ms = members()[:2] + members()[3:4]
+    grav = [2, 1, 2]
+    print(f"[case] batch with gravity: " + ", ".join(str(st.n) for st, _, _ in ms), flush=True)
+
+    def make():
+        return [gpu.GpuSolver(st, dataclasses.replace(cfg, gravity=gv), precision=p)
+                for (st, cfg, p), gv in zip(ms, grav)]
+
+    batch, alone = make(), make()
+    for step in range(4):
+        nxt, crit = gpu.step_batch(batch)
+        for k, s in enumerate(alone):
+            _, n1, c1 = s.step()
+            assert n1 == nxt[k] and c1 == crit[k], (step, k)
+    for k, ((st, _, _), a, b) in enumerate(zip(ms, batch, alone)):
+        da, db = a.download(st), b.download(st)
+        for f in FIELDS:
+            assert np.array_equal(da[f], db[f]), (k, f)
+    for s in batch + alone:
+        s.close()
