@@ -117,3 +117,21 @@ def test_device_flaws_statistics_match_host_generator():
         assert abs(nf.var() - lam * (1 - 1 / count)) < 0.05 * lam, (name, nf.var(), lam)
         first_flaw = (out["eps_min"][sl] / denom) ** 9.0
         assert abs(np.median(first_flaw) / count - math.log(2.0)) < 0.02, (name, np.median(first_flaw) / count)
+
+
+def test_set_flaws_arguments():
+    st, cfg = _state(400, 40)
+    print(f"[case] device flaws arguments n={st.n}", flush=True)
+    g = gpu.GpuSolver(st, cfg)
+    before = g.download(st)
+    g.set_flaws(0, 0, 1.0, 1.0, seed=1)  # an empty body changes nothing
+    after = g.download(st)
+    for k in ("eps_min", "m_zero", "growth", "n_flaws"):
+        assert np.array_equal(after[k], before[k]), k
+    for args in ((st.n, 1, 1.0, 1.0), (0, st.n + 1, 1.0, 1.0), (0, 10, 0.0, 1.0), (0, 10, 1.0, -1.0)):
+        with pytest.raises(gpu.GpuError):
+            g.set_flaws(*args, seed=1)
+    g.set_flaws(0, 1, 1.0, 1.0, seed=1)  # a one-particle body: one flaw, m_zero 1
+    one = g.download(st)
+    assert one["n_flaws"][0] == 1 and one["m_zero"][0] == 1.0
+    g.close()

@@ -100,3 +100,20 @@ def test_detached_gravity_gives_the_plain_solver():
     a.close()
     b.close()
     grav.close()
+
+
+def test_gravity_on_empty_and_single_particle_states():
+    from opensph_amd import initial
+
+    st, cfg = initial.fluid_sphere(n_requested=200, radius=100.0)
+    cfg = dataclasses.replace(cfg, gravity=2)
+    for idx in ([], [0]):
+        sub = st.subset(np.asarray(idx, dtype=np.int64))
+        print(f"[case] gravity solver n={sub.n}", flush=True)
+        g = gpu.GpuSolver(sub, cfg)
+        g.integrate()
+        dt, nxt, crit = g.step()
+        assert np.isfinite(nxt)
+        if sub.n:
+            assert np.all(g.download(sub)["dv"][:, :3] == 0.0)  # no other particle, no force
+        g.close()

@@ -121,3 +121,25 @@ def test_malformed_files_raise(tmp_path):
     st["mat"][0] = 1
     with pytest.raises(ssf.SsfError):
         ssf.from_state(st)
+
+
+def test_file_without_materials_round_trips(tmp_path):
+    """The NOMAT layout (a storage without materials: one particle block, no parameters or ranges)."""
+    n = 17
+    rng = np.random.default_rng(1)
+    quantities = [(ssf.POSITION, 2, ssf.VECTOR), (ssf.MASS, 0, ssf.SCALAR), (ssf.FLAG, 0, ssf.INDEX)]
+    buffers = {ssf.POSITION: [rng.normal(size=(n, 4)) for _ in range(3)], ssf.MASS: [rng.random(n)],
+               ssf.FLAG: [rng.integers(0, 5, n)]}
+    f = ssf.SsfFile(3.5, 0.125, ssf.LATEST, b"nbody" + bytes(11), bytes(16), 0, quantities, [],
+                    [ssf.Block(0, n, buffers)], [])
+    out = tmp_path / "nomat.ssf"
+    ssf.write(out, f)
+    g = ssf.read(out)
+    assert g.materials == [] and g.particle_count == n and g.run_type_name() == "nbody"
+    for qid, bufs in buffers.items():
+        for a, b in zip(bufs, g.blocks[0].buffers[qid]):
+            assert np.array_equal(a, b)
+    ssf.write(tmp_path / "again.ssf", g)
+    assert (tmp_path / "again.ssf").read_bytes() == out.read_bytes()
+    st = ssf.to_state(g)
+    assert st.n == n and np.array_equal(st["r"], buffers[ssf.POSITION][0])
