@@ -759,11 +759,41 @@ struct EvalStage {
     uint32_t sidx[64];
 };
 
-// One wavefront per target leaf.  Lanes are (particle p, split sp) with P = pow2 >= count particles and
-// S = 64 / P splits.  The node list and the source particles (every listed leaf followed by the target
-// leaf itself) are staged 64 at a time in the wavefront's LDS with coalesced loads; split sp then takes
-// every S-th staged entry, and the splits are summed with cross-lane adds at the end.  Accelerations
-// are added to the state planes.
+// Lanes of a target leaf's wavefront: lane = sp * tc + p, for particle p < tc of the leaf and split
+// sp < S = 64 / tc.  Lanes from S * tc on take no entries.  (Splits of a power-of-two width would leave
+// up to half the lanes idle for leaves of 17 particles; a 25-particle leaf limit makes most leaves
+// 9-25 particles.)
+struct LaneSplit {
+    uint32_t S, p, sp;
+    bool used;
+};
+
+__device__ __forceinline__ LaneSplit lane_split(uint32_t lane, uint32_t tc) {
+    LaneSplit l;
+    l.S = __builtin_amdgcn_readfirstlane(64u / tc);
+    l.sp = lane / tc;
+    l.p = lane - l.sp * tc;
+    l.used = l.sp < l.S;
+    return l;
+}
+
+// Sums the splits' partial accelerations into the lanes of split 0: split sp adds split sp + d for
+// d = 1, 2, 4, ... when sp is a multiple of 2d.
+__device__ __forceinline__ void sum_splits(const LaneSplit& l, uint32_t lane, uint32_t tc, double a[3]) {
+    for (uint32_t d = 1; d < l.S; d <<= 1) {
+        const int src = int(min(lane + d * tc, 63u));
+        const bool take = l.used && (l.sp & (2 * d - 1)) == 0 && l.sp + d < l.S;
+        for (int c = 0; c < 3; ++c) {
+            const double v = __shfl(a[c], src, 64);
+            if (take) a[c] += v;
+        }
+    }
+}
+
+// One wavefront per target leaf, its lanes split as lane_split says.  The node list and the source
+// particles (every listed leaf followed by the target leaf itself) are staged 64 at a time in the
+// wavefront's LDS with coalesced loads; split sp then takes every S-th staged entry, and the splits are
+// summed across lanes at the end.  Accelerations are added to the state planes.
 template <int ORDER>
 __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t* __restrict__ order,
                                                 const uint32_t* __restrict__ targets, uint32_t ntargets,
@@ -779,12 +809,11 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
     const uint32_t T = targets[q];
     uint32_t tf, tc;
     node_range(t, T, tf, tc);
-    uint32_t P = 1;
-    while (P < tc) P <<= 1;
-    const uint32_t S = 64 / P;
-    const uint32_t p = lane & (P - 1), sp = lane / P;
-    const bool active = p < tc;
-    const uint32_t k = tf + (active ? p : 0);
+    const LaneSplit ls = lane_split(lane, tc);
+    const uint32_t S = ls.S, sp = ls.sp;
+    const bool active = ls.used;
+    const uint32_t j0 = active ? sp : 64u;  // unused lanes take no entries
+    const uint32_t k = tf + ls.p;
     const double4 me = t.pos[k];
     double a[3] = { 0.0, 0.0, 0.0 };
     const uint32_t nn = min(ncount[q], cap_n), nx = min(xcount[q], cap_x);
@@ -810,7 +839,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
             }
         }
         wave_sync();
-        for (uint32_t j = sp; j < cnt; j += S) {
+        for (uint32_t j = j0; j < cnt; j += S) {
             NodeMom mm;
             mm.M = S_.M[j];
             if constexpr (ORDER >= 2) {
@@ -861,7 +890,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
                 S_.sidx[lane] = src;
             }
             wave_sync();
-            for (uint32_t j = sp; j < cnt; j += S) {
+            for (uint32_t j = j0; j < cnt; j += S) {
                 if (S_.sidx[j] == k) continue;
                 const double4 o = S_.src[j];
                 add_pair<false>(grads, o.x - me.x, o.y - me.y, o.z - me.z, 0.5 * (o.w + me.w), S_.sm[j], a);
@@ -869,10 +898,8 @@ __global__ void __launch_bounds__(BLOCK) k_eval(Tree t, Planes s, const uint32_t
             wave_sync();
         }
     }
-    for (uint32_t o = P; o < 64; o <<= 1) {
-        for (int c = 0; c < 3; ++c) a[c] += __shfl_xor(a[c], int(o), 64);
-    }
-    if (active && sp == 0) {
+    sum_splits(ls, lane, tc, a);
+    if (sp == 0) {
         const uint32_t i = order[k];
         s.d[GAX][i] += a[0];
         s.d[GAY][i] += a[1];
@@ -1017,12 +1044,11 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const ui
     const uint32_t T = targets[q];
     uint32_t tf, tc;
     node_range(t, T, tf, tc);
-    uint32_t P = 1;
-    while (P < tc) P <<= 1;
-    const uint32_t S = 64 / P;
-    const uint32_t p = lane & (P - 1), sp = lane / P;
-    const bool active = p < tc;
-    const uint32_t k = tf + (active ? p : 0);
+    const LaneSplit ls = lane_split(lane, tc);
+    const uint32_t S = ls.S, sp = ls.sp;
+    const bool active = ls.used;
+    const uint32_t j0 = active ? sp : 64u;  // unused lanes take no entries
+    const uint32_t k = tf + ls.p;
     const double4 R = t.pos[tf];
     float mx, my, mz, mh;
     {
@@ -1059,7 +1085,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const ui
         }
         wave_sync();
         float b[3] = { 0.f, 0.f, 0.f };
-        for (uint32_t j = sp; j < cnt; j += S) {
+        for (uint32_t j = j0; j < cnt; j += S) {
             float q2[5], q3[7];
             if constexpr (ORDER >= 2) {
                 for (int c = 0; c < 5; ++c) q2[c] = S_.q2[c][j];
@@ -1116,7 +1142,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const ui
             }
             const float close = fmaxf(wave_max_f(o2), me2) * (1.f / 64.f);  // (O / 8)^2
             float b[3] = { 0.f, 0.f, 0.f };
-            for (uint32_t j = sp; j < cnt; j += S) {
+            for (uint32_t j = j0; j < cnt; j += S) {
                 const uint32_t src = S_.sidx[j];
                 if (src == k) continue;
                 const float4 o = S_.src[j];
@@ -1133,10 +1159,8 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const ui
             wave_sync();
         }
     }
-    for (uint32_t o = P; o < 64; o <<= 1) {
-        for (int c = 0; c < 3; ++c) a[c] += __shfl_xor(a[c], int(o), 64);
-    }
-    if (active && sp == 0) {
+    sum_splits(ls, lane, tc, a);
+    if (sp == 0) {
         const uint32_t i = order[k];
         s.d[GAX][i] += a[0] * sc.back;
         s.d[GAY][i] += a[1] * sc.back;
