@@ -65,6 +65,23 @@ def test_mixed_with_every_node_opened_matches_direct_sum():
     assert err(a, ref) < TOL, err(a, ref)
 
 
+@pytest.mark.parametrize("leaf", [33, 47, 64])
+def test_large_leaves_with_every_node_opened_match_direct_sum(leaf):
+    """Target leaves of more than 32 particles, up to the 64-particle limit, take one split of the
+    wavefront's lanes (the others idle); theta -> 0 in both precisions is the oracle's direct sum."""
+    r = ball(2000, 1e7, 37)
+    m = np.full(len(r), 1e20)
+    ref = GravityOracle().direct(r, m, G_SI, kernel="point")
+    for prec, tol in (("fp64", 1e-13), ("mixed", TOL)):
+        print(f"[case] gravity theta0 leaf={leaf} precision={prec} n={len(r)}", flush=True)
+        g = GpuGravity(r, m, theta=1e-12, order=2, leaf_size=leaf, kernel="point", precision=prec)
+        g.build()
+        a, st = g.eval_self()
+        g.close()
+        assert st["nodes_approx"] == 0
+        assert err(a, ref) < tol, (prec, err(a, ref))
+
+
 @pytest.mark.parametrize("radius,mass", [(1.0, 1e-3), (1e5, 1e12), (1.5e11, 1e29), (1e17, 1e33)])
 def test_mixed_scales_from_metres_to_astronomical(radius, mass):
     """Lengths and G m are brought into FP32's range by powers of two: the same relative accuracy at
