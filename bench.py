@@ -39,6 +39,9 @@ FLOP_PER_QUADRUPOLE = 61.0
 # copy of --calibrate reads).  None where not measured.
 MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (473765 + 97454) * 1024.0,
                           "pairs_fp64_finalize_correct": (1170388 + 272513) * 1024.0}
+# The same for the Barnes-Hut evaluation of the 8,000,000-particle N-body sphere (FETCH_SIZE x 2 +
+# WRITE_SIZE, KiB per dispatch, averaged over 5 dispatches).
+NBODY_TRAFFIC_BYTES = {"eval_mixed": (2 * 3746826 + 751902) * 1024.0, "eval": (2 * 3631237 + 751959) * 1024.0}
 
 
 def log(*a):
@@ -228,7 +231,8 @@ def nbody_once(args, prec):
            "roofline": {"bound": "mfma", "kernel": kname,
                         "achieved": (FLOP_PER_GRAVITY_PAIR * st["pairs"] + FLOP_PER_QUADRUPOLE * st["expansions"])
                         / (eval_ms * 1e-3) / 1e12,
-                        "peak": peak, "unit": "TFLOP/s", "traffic": None},
+                        "peak": peak, "unit": "TFLOP/s",
+                        "traffic": NBODY_TRAFFIC_BYTES.get(kname) if n == 8_000_000 else None},
            "_dv0": dv0}
     out["roofline"]["frac"] = out["roofline"]["achieved"] / peak
     return out
