@@ -39,6 +39,14 @@ FLOP_PER_QUADRUPOLE = 61.0
 # copy of --calibrate reads).  None where not measured.
 MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (473765 + 97454) * 1024.0,
                           "pairs_fp64_finalize_correct": (1170388 + 272513) * 1024.0}
+# Algorithmic HBM bytes of one pair-kernel launch per particle: each particle's own neighbour record
+# (80 B mixed, 128 B FP64), its order, count (and group) entries and the 92 B it writes (acceleration,
+# du, divergence, velocity gradient: 11 doubles, and the neighbour count); plus 4 B per listed neighbour.
+# The fused FP64 kernel adds the finalize + corrector + criteria pass (about 60 doubles read and 25
+# written per particle), less the 92 B it no longer writes for that pass to read back.
+PAIR_BYTES_PER_PARTICLE = {"pairs_mixed": 180.0, "pairs_fp64": 232.0,
+                           "pairs_fp64_finalize_correct": 232.0 + 680.0 - 92.0}
+PAIR_BYTES_PER_NEIGHBOR = 4.0
 # The same for the Barnes-Hut evaluation of the 8,000,000-particle N-body sphere (FETCH_SIZE x 2 +
 # WRITE_SIZE, KiB per dispatch, averaged over 5 dispatches).
 NBODY_TRAFFIC_BYTES = {"eval_mixed": (2 * 3746826 + 751902) * 1024.0, "eval": (2 * 3631237 + 751959) * 1024.0}
@@ -48,15 +56,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(st, cfg, max_seconds=25.0):
-    """Times the reference core (ThreadPool, all host cores) on PredictorCorrector steps of the same
-    workload until about max_seconds of CPU wall time; falls back to the oracle port on one core."""
+def cpu_baseline(st, cfg, dt0, window, max_seconds=25.0):
+    """Times the reference core (ThreadPool, all host cores) on PredictorCorrector steps of the same window
+    of the workload: it starts from the FP64 GPU run's state at the start of its timed window (in contact)
+    with that run's time step, and runs until about max_seconds of CPU wall time; falls back to the oracle
+    port on one core."""
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    where = (f"from t = {window['t_start']:.4f} s (after {window['steps_before']} steps from t = 0; the bodies "
+             f"touch at t = {window['t_contact']:.4f} s)")
     try:
         from oracle.bridge import RefRun
 
         if RefRun.available():
-            ref = RefRun(st, cfg, nthreads=threads)
+            ref = RefRun(st, cfg, nthreads=threads, dt0=dt0)
             ref.step()  # first step (tree and buffers allocated) not timed
             steps, t0 = 0, time.perf_counter()
             while True:
@@ -68,16 +80,17 @@ def cpu_baseline(st, cfg, max_seconds=25.0):
             ref.close()
             return {"value": st.n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
                     "sample": f"{steps} PredictorCorrector steps (after one untimed) of the same {st.n}-particle "
-                              f"impact, OpenSPH core built from source (-O2, strict FP), ThreadPool x{threads}"}
+                              f"impact {where}, OpenSPH core built from source (-O2, strict FP), "
+                              f"ThreadPool x{threads}"}
         from oracle.bridge import Oracle
 
         o = Oracle()
-        sub = st
         t0 = time.perf_counter()
-        o.run_pc(sub, cfg, 1)
+        o.run_pc(st, cfg, 1, dt0=dt0)
         el = time.perf_counter() - t0
-        return {"value": sub.n / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-                "sample": f"1 PredictorCorrector step of the same {sub.n}-particle impact, oracle C port, 1 core"}
+        return {"value": st.n / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                "sample": f"1 PredictorCorrector step of the same {st.n}-particle impact {where}, oracle C port, "
+                          f"1 core"}
     except Exception as e:  # the baseline is reported, never required
         log("cpu baseline failed:", e)
         return None
@@ -88,10 +101,20 @@ def total_energy(st):
     return float(np.sum(st["m"] * (0.5 * v2 + st["u"])))
 
 
-def run_precision(gpu, st, cfg, prec, warmup, steps, tune=None, tuner=False):
+def run_precision(gpu, st, cfg, prec, warmup, steps, t_contact, total_steps=100, tune=None, tuner=False):
+    """One precision of the impact from t = 0: untimed PC steps into contact (at least two steps past the
+    contact time), the warmup, the timed window, a per-kernel profile over the next steps, then untimed
+    steps up to total_steps for the energy drift of the whole run."""
     solver = gpu.GpuSolver(st, cfg, precision=prec)
     if tune:
         solver.tune(**tune)
+    t, done, past = 0.0, 0, 0
+    while t < t_contact or past < 2:
+        dt, _, _ = solver.step()
+        t += dt
+        done += 1
+        past += t >= t_contact
+    prelude = done
     tuned = None
     if tuner:
         # launch shapes picked by the autotuner on this state (it leaves the state as it found it)
@@ -99,35 +122,53 @@ def run_precision(gpu, st, cfg, prec, warmup, steps, tune=None, tuner=False):
         tuned = autotune.autotune(solver, steps=10, log=log)
         log(f"{prec}: autotuned {tuned['settings']} ({tuned['ms_per_step']:.3f} ms/step)")
     for _ in range(warmup):
-        solver.step()
+        dt, _, _ = solver.step()
+        t += dt
+    done += warmup
     solver.sync()
+    start = solver.download(st)
+    t_start, dt_start = t, solver.dt
     # the timed steps carry HIP events around the pair-sum launches only (for the roofline); events
     # around every launch would add a few microseconds between kernels
     solver.profile(2)
+    dts = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        solver.step()
+        dts.append(solver.step()[0])
     solver.sync()
     elapsed = time.perf_counter() - t0
     prof_pairs = solver.profile_read()
-    final = solver.download(st)
-    # per-kernel breakdown from a few more steps, outside the timed region
+    done += steps
+    t += sum(dts)
+    end = solver.download(st)
+    t_end = t
+    # per-kernel breakdown: every launch timed over the next prof_steps steps (one window)
+    prof_steps = min(max(steps, 1), 10)
     solver.profile(1)
-    for _ in range(min(steps, 10)):
-        solver.step()
+    for _ in range(prof_steps):
+        t += solver.step()[0]
     prof = solver.profile_read()
-    prof.update(prof_pairs)
     solver.profile(False)
-    stats = solver.integrate()  # neighbour statistics (outside the timed region)
+    done += prof_steps
+    while done < total_steps:
+        t += solver.step()[0]
+        done += 1
+    final = solver.download(st)
     solver.close()
     return {"value": st.n * steps / elapsed, "elapsed": elapsed, "ms_per_step": 1e3 * elapsed / steps,
-            "prof": prof, "stats": stats, "final": final, "tuned": tuned["settings"] if tuned else None}
+            "prof": prof, "prof_steps": prof_steps, "prof_pairs": prof_pairs, "start": start, "end": end,
+            "final": final, "dt_start": dt_start,
+            "window": {"t_contact": t_contact, "steps_before": prelude + warmup, "prelude_steps": prelude,
+                       "t_start": t_start, "t_end": t_end, "run_steps": done, "t_run_end": t},
+            "tuned": tuned["settings"] if tuned else None}
 
 
-def compare_precisions(gpu, st, cfg, runs):
-    """Acceleration error of the mixed pair sums against FP64 on the same (final FP64) state, and the
-    energy of both runs after warmup + steps (the 100-step workload)."""
-    ref_state = runs["fp64"]["final"]
+def compare_precisions(gpu, st, cfg, runs, max_accel_err=1e-5):
+    """Acceleration and density-rate error of the mixed pair sums against FP64 on the same state (the FP64
+    run at the end of its timed window, in contact), and the energy drift of both runs over the whole run
+    from t = 0.  Raises when the state has no pair sums (the bodies are not in contact) or when the mixed
+    error exceeds max_accel_err: the headline is only valid at that accuracy."""
+    ref_state = runs["fp64"]["end"]
     out = {}
     for prec in ("fp64", "mixed"):
         g = gpu.GpuSolver(ref_state, cfg, precision=prec)
@@ -135,25 +176,32 @@ def compare_precisions(gpu, st, cfg, runs):
         out[prec] = g.download(ref_state)
         g.close()
     a64, amx = out["fp64"]["dv"][:, :3], out["mixed"]["dv"][:, :3]
+    n64 = float(np.linalg.norm(a64))
+    if not (n64 > 0.0) or not np.isfinite(n64):
+        raise RuntimeError("the FP64 accelerations of the window's state are zero or not finite: the bodies are "
+                           "not in contact, so the window does not exercise the pair sums")
     e0 = total_energy(st)
     e64, emx = total_energy(runs["fp64"]["final"]), total_energy(runs["mixed"]["final"])
-    return {"accel_rel_l2": float(np.linalg.norm(amx - a64) / np.linalg.norm(a64)),
-            "drho_rel_l2": float(np.linalg.norm(out["mixed"]["drho"] - out["fp64"]["drho"]) /
-                                 np.linalg.norm(out["fp64"]["drho"])),
-            "neighbor_counts_equal": bool(np.array_equal(out["mixed"]["ncnt"], out["fp64"]["ncnt"])),
-            "energy_drift_fp64": (e64 - e0) / e0, "energy_drift_mixed": (emx - e0) / e0,
-            "energy_mixed_vs_fp64": (emx - e64) / e0}
-
-
-def nbody_state(n, seed=7):
-    """Uniform sphere of R = 100 km, equal masses 1.25e12 kg, h = 10 m (M4 softening support 20 m), at rest."""
-    rng = np.random.default_rng(seed)
-    out = np.empty((0, 3))
-    while len(out) < n:
-        x = rng.uniform(-1.0, 1.0, size=(int(1.3 * (n - len(out))) + 1024, 3))
-        out = np.concatenate([out, x[np.einsum("ij,ij->i", x, x) < 1.0]])
-    r = np.c_[out[:n] * 1.0e5, np.full(n, 10.0)]
-    return r, np.full(n, 1.25e12)
+    acc = {"state": "FP64 run at t_end of its timed window",
+           "accel_rel_l2": float(np.linalg.norm(amx - a64) / n64),
+           "accel_err_over_max": float(np.max(np.linalg.norm(amx - a64, axis=1)) /
+                                       np.max(np.linalg.norm(a64, axis=1))),
+           "drho_rel_l2": float(np.linalg.norm(out["mixed"]["drho"] - out["fp64"]["drho"]) /
+                                np.linalg.norm(out["fp64"]["drho"])),
+           "du_rel_l2": float(np.linalg.norm(out["mixed"]["du"] - out["fp64"]["du"]) /
+                              max(np.linalg.norm(out["fp64"]["du"]), 1e-300)),
+           "particles_accelerated": int(np.count_nonzero(np.any(a64 != 0.0, axis=1))),
+           "neighbor_counts_equal": bool(np.array_equal(out["mixed"]["ncnt"], out["fp64"]["ncnt"])),
+           "energy_steps": {p: runs[p]["window"]["run_steps"] for p in ("fp64", "mixed")},
+           "energy_drift_fp64": (e64 - e0) / e0, "energy_drift_mixed": (emx - e0) / e0,
+           "energy_mixed_vs_fp64": (emx - e64) / e0,
+           "gate": {"accel_rel_l2_max": max_accel_err, "energy_drift_max": 1e-6}}
+    if not acc["accel_rel_l2"] <= max_accel_err:
+        raise RuntimeError(f"mixed-precision acceleration error {acc['accel_rel_l2']:.3g} exceeds {max_accel_err:g}")
+    if not (abs(acc["energy_drift_mixed"]) <= 1e-6 and abs(acc["energy_drift_fp64"]) <= 1e-6):
+        raise RuntimeError(f"energy drift over {acc['energy_steps']} steps exceeds 1e-6: fp64 "
+                           f"{acc['energy_drift_fp64']:.3g}, mixed {acc['energy_drift_mixed']:.3g}")
+    return acc
 
 
 def run_nbody(args):
@@ -179,11 +227,11 @@ def run_nbody(args):
 
 
 def nbody_once(args, prec):
-    from opensph_amd import gpu
+    from opensph_amd import gpu, initial
     from opensph_amd.gravity import GpuGravity
 
     n = args.nbody_n
-    r, m = nbody_state(n)
+    r, m = initial.nbody_sphere(n)
     g = GpuGravity(r, m, solver="barnes_hut", kernel="m4", theta=0.5, order=2, leaf_size=25, precision=prec)
     # the accelerations of the first evaluation, for the FP32 / FP64 comparison (untimed)
     g.build()
@@ -228,7 +276,7 @@ def nbody_once(args, prec):
                         "err_over_rms_max": float(np.max(np.linalg.norm(bh - direct, axis=1)) / rms)},
            "interactions": {"nodes_approx": int(st["nodes_approx"]), "leaves_exact": int(st["nodes_exact"]),
                             "expansions": int(st["expansions"]), "pairs": int(st["pairs"]), "eval_ms": eval_ms},
-           "roofline": {"bound": "mfma", "kernel": kname,
+           "roofline": {"bound": "valu", "kernel": kname,
                         "achieved": (FLOP_PER_GRAVITY_PAIR * st["pairs"] + FLOP_PER_QUADRUPOLE * st["expansions"])
                         / (eval_ms * 1e-3) / 1e12,
                         "peak": peak, "unit": "TFLOP/s",
@@ -238,35 +286,13 @@ def nbody_once(args, prec):
     return out
 
 
-def ensemble_members(count, seed=11, flaws=True):
-    """Impact ensemble: N ~ exp(U(ln 5e3, ln 2e5)), R_p/R_t ~ U(0.05, 0.2) at equal particle spacing,
-    angle ~ U(0, 90) deg, speed ~ U(3, 7) km/s, R_t = 50 km, basalt; one seed per member.  flaws=False
-    leaves the Grady-Kipp flaws to the device (initial.device_flaws after the upload)."""
-    from opensph_amd import initial
-
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(count):
-        n = float(np.exp(rng.uniform(np.log(5e3), np.log(2e5))))
-        ratio = rng.uniform(0.05, 0.2)
-        n_p = max(100, int(n * ratio ** 3 / (1 + ratio ** 3)))
-        n_t = max(100, int(n) - n_p)
-        st, cfg = initial.impact(n_target=n_t, n_projectile=n_p, r_target=50e3, r_projectile=50e3 * ratio,
-                                 speed=rng.uniform(3e3, 7e3), angle_deg=rng.uniform(0.0, 90.0),
-                                 seed=int(rng.integers(1 << 30)), flaws=flaws)
-        out.append((st, cfg))
-    return out
-
-
 def run_ensemble(args):
     """Impact ensemble: all members stepped together (osph_step_pc_batch) against one after another."""
-    from opensph_amd import gpu
-
-    from opensph_amd import initial
+    from opensph_amd import gpu, initial
 
     # set-up: lattices and placement on the host, the Grady-Kipp flaws drawn on the device
     t0 = time.perf_counter()
-    ms = ensemble_members(args.ensemble_n, flaws=False)
+    ms = initial.impact_ensemble(args.ensemble_n, flaws=False)
     t_host = time.perf_counter() - t0
     solvers = [gpu.GpuSolver(st, cfg, precision="mixed") for st, cfg in ms]
     t_upload = time.perf_counter() - t0 - t_host
@@ -371,6 +397,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
                     help="both: time the FP64 drop-in and the mixed-precision form; value = mixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--total-steps", type=int, default=100,
+                    help="impact: steps from t = 0 over which the energy drift is reported (untimed steps after "
+                         "the timed window and its profile fill up to this count)")
     ap.add_argument("--tune", action="append", default=[],
                     help="launch-shape setting key=value passed to osph_tune (e.g. pairs_lds=40960)")
     ap.add_argument("--gravity", type=int, default=0, choices=[0, 1, 2],
@@ -415,44 +444,66 @@ def main():
     if args.gravity:
         import dataclasses
         cfg = dataclasses.replace(cfg, gravity=args.gravity)
-    log(f"setup {st.n} particles in {time.perf_counter() - t0:.1f}s")
+    t_contact = initial.contact_time(st)
+    log(f"setup {st.n} particles in {time.perf_counter() - t0:.1f}s; the bodies touch at t = {t_contact:.4f} s")
     name, cus = gpu.device_info()
     precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     runs = {}
     for prec in precisions:
         tune = dict((kv.split("=")[0], int(kv.split("=")[1])) for kv in args.tune)
-        runs[prec] = run_precision(gpu, st, cfg, prec, args.warmup, args.steps, tune, args.autotune)
+        runs[prec] = run_precision(gpu, st, cfg, prec, args.warmup, args.steps, t_contact, args.total_steps, tune,
+                                   args.autotune)
         log(f"{prec}: {runs[prec]['value']:.4g} particle-steps/s, {runs[prec]['ms_per_step']:.3f} ms/step")
     main_prec = precisions[-1]
     r = runs[main_prec]
     value = r["value"]
     elapsed = r["elapsed"]
-    stats = r["stats"]
-    prof = r["prof"]
 
-    pairs = stats["neighbors_mean"] * st.n / 2.0
+    # unordered pairs in the timed window: the neighbour counts of its first and last evaluation
+    nb_start = float(np.mean(r["start"]["ncnt"]))
+    nb_end = float(np.mean(r["end"]["ncnt"]))
+    nb_mean = 0.5 * (nb_start + nb_end)
+    pairs = nb_mean * st.n / 2.0
     kernel = "pairs_mixed" if main_prec == "mixed" else "pairs_fp64"
-    if kernel not in prof and kernel + "_finalize_correct" in prof:
+    prof_pairs = r["prof_pairs"]
+    if kernel not in prof_pairs and kernel + "_finalize_correct" in prof_pairs:
         # the PC step fuses the finalize, corrector and criteria into the pair kernel: the roofline then
         # holds the pair flops against the fused kernel's time (a lower bound of the pair part's rate)
         kernel += "_finalize_correct"
-    ms, launches = prof.get(kernel, (float("nan"), 1))
+    ms, launches = prof_pairs.get(kernel, (float("nan"), 1))
     avg_ms = ms / max(launches, 1)
     achieved = FLOP_PER_PAIR * pairs / (avg_ms * 1e-3) / 1e12
     peak = FP32_PEAK_TFLOPS if main_prec == "mixed" else FP64_PEAK_TFLOPS
-    dominant = max(prof.items(), key=lambda kv: kv[1][0])[0] if prof else kernel
+    alg_bytes = st.n * (PAIR_BYTES_PER_PARTICLE[kernel] + PAIR_BYTES_PER_NEIGHBOR * nb_mean)
     traffic = MEASURED_TRAFFIC_BYTES.get(kernel) if st.n == 1062568 else None
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+
+    def per_step(run):
+        # every launch of the profiled window, divided by its step count: true per-step times
+        return {k: v[0] / run["prof_steps"] for k, v in sorted(run["prof"].items())}
+
+    steps_ms = per_step(r)
+    roofline = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": traffic, "kernel": kernel,
-                "kernel_ms": avg_ms, "dominant_kernel": dominant,
-                "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in sorted(prof.items())}}
+                "kernel_ms_per_launch": avg_ms, "launches_timed": launches,
+                "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs,
+                "hbm": {"algorithmic_bytes": alg_bytes, "achieved_GBs": alg_bytes / (avg_ms * 1e-3) / 1e9,
+                        "peak_GBs": HBM_PEAK_GBS, "frac": alg_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                "dominant_kernel": max(steps_ms.items(), key=lambda kv: kv[1])[0] if steps_ms else kernel,
+                "kernel_ms_per_step": steps_ms,
+                "kernel_launches_per_step": {k: v[1] / r["prof_steps"] for k, v in sorted(r["prof"].items())},
+                "profiled_steps": r["prof_steps"]}
+    if main_prec == "mixed":
+        # FP32 without packed instructions (disabled in this build) issues at the FP64 rate
+        roofline["frac_of_unpacked_fp32"] = achieved / FP64_PEAK_TFLOPS
     accuracy = None
     if "fp64" in runs and "mixed" in runs:
         accuracy = compare_precisions(gpu, st, cfg, runs)
 
     # the reference driver's self-gravity is the direct sum: O(N^2) per step at 1M particles, not a
     # bounded sample, so the gravity variants report no CPU baseline
-    baseline = None if (args.no_cpu or args.gravity) else cpu_baseline(st, cfg)
+    base_run = runs["fp64"] if "fp64" in runs else r
+    baseline = None if (args.no_cpu or args.gravity) else cpu_baseline(base_run["start"], cfg, base_run["dt_start"],
+                                                                        base_run["window"])
     out = {
         "metric": "particle-steps/s, 1M impact: mixed vs FP64 GPU at <=1e-5 accel err",
         "value": value,
@@ -470,10 +521,12 @@ def main():
                                "continuity + adaptive h, von Mises + Grady-Kipp damage, predictor-corrector",
                    "particles": st.n, "precision": main_prec, "device": name, "compute_units": cus,
                    "self_gravity": {0: None, 1: "direct sum", 2: "Barnes-Hut theta 0.5 octupole leaf 25"}[args.gravity],
-                   "neighbors_mean": stats["neighbors_mean"]},
+                   "neighbors_mean": nb_mean},
+        "window": r["window"],
         "fp64_value": runs["fp64"]["value"] if "fp64" in runs else None,
-        "fp64_kernel_ms_per_step": ({k: v[0] / max(v[1], 1) for k, v in sorted(runs["fp64"]["prof"].items())}
-                                    if "fp64" in runs else None),
+        "fp64_ms_per_step": runs["fp64"]["ms_per_step"] if "fp64" in runs else None,
+        "fp64_window": runs["fp64"]["window"] if "fp64" in runs else None,
+        "fp64_kernel_ms_per_step": per_step(runs["fp64"]) if "fp64" in runs else None,
         "mixed_over_fp64": (runs["mixed"]["value"] / runs["fp64"]["value"]) if len(runs) == 2 else None,
         "accuracy": accuracy,
         "autotuned": {p: runs[p]["tuned"] for p in runs} if args.autotune else None,

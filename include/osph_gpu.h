@@ -158,7 +158,9 @@ int osph_set_flaws(osph_ctx* ctx, size_t first, size_t count, double volume, dou
  * each evaluation (the gravity context's own particles are overwritten); set the gravity context up as
  * Factory::getGravity would (core/system/Factory.cpp:361-413; BruteForceGravity takes Constants::gravity,
  * BarnesHut GRAVITY_CONSTANT).  g = NULL detaches.  The gravity context must outlive the attachment;
- * the adjoint is not available while one is attached. */
+ * the adjoint is not available while one is attached.  A gravity context serves one SPH context at a
+ * time (its planes hold that context's particles and accelerations between the evaluation and the pair
+ * sums): attaching it to a second one fails with OSPH_ERR_ARG until the first detaches or is destroyed. */
 struct osph_gravity;
 int osph_attach_gravity(osph_ctx* ctx, struct osph_gravity* g);
 int osph_sync(osph_ctx* ctx);

@@ -75,6 +75,7 @@ struct osph_ctx {
     struct {
         bool ready = false;
         int steps = 0, every = 0, saved_reorder = 0;
+        bool reorder_saved = false;  // saved_reorder holds the setting to restore (a forward run began)
         std::vector<double> dt;
         double* ckpt = nullptr;  // ceil(steps / every) blocks of 13 planes of cap doubles
         size_t ckpt_blocks = 0, ckpt_cap = 0;
@@ -837,6 +838,7 @@ int osph_create(osph_ctx** out, const osph_config* cfg, const osph_material* mat
 void osph_destroy(osph_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->grav && gravity_owner(c->grav) == c) gravity_set_owner(c->grav, nullptr);
     resolve_events(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     if (c->grav_ready) (void)hipEventDestroy(c->grav_ready);
@@ -1177,7 +1179,12 @@ int osph_set_flaws(osph_ctx* c, size_t first, size_t count, double volume, doubl
 int osph_attach_gravity(osph_ctx* c, osph_gravity* g) {
     if (!c) return OSPH_ERR_ARG;
     if (g && c->adj.ready) return c->fail(OSPH_ERR_UNSUPPORTED, "the adjoint has no self-gravity");
+    if (g && gravity_owner(g) && gravity_owner(g) != c) {
+        return c->fail(OSPH_ERR_ARG, "the gravity context is attached to another SPH context");
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->grav && c->grav != g && gravity_owner(c->grav) == c) gravity_set_owner(c->grav, nullptr);
+    if (g) gravity_set_owner(g, c);
     if (g && !c->grav_ready) {
         HIPCHK(c, hipEventCreateWithFlags(&c->grav_ready, hipEventDisableTiming));
         HIPCHK(c, hipEventCreateWithFlags(&c->grav_done, hipEventDisableTiming));
@@ -1282,7 +1289,10 @@ int osph_adjoint_forward(osph_ctx* c, int steps, const double* dt, int checkpoin
         if (!(dt[k] > 0.0) || !std::isfinite(dt[k])) return c->fail(OSPH_ERR_ARG, "time steps must be positive");
     }
     const size_t n = c->n;
-    if (!c->adj.ready) c->adj.saved_reorder = c->tune.reorder_period;
+    if (!c->adj.reorder_saved) {
+        c->adj.saved_reorder = c->tune.reorder_period;
+        c->adj.reorder_saved = true;
+    }
     c->tune.reorder_period = 0;  // the slot order stays fixed between the checkpoints and the replays
     c->adj.ready = false;
     const size_t blocks = size_t((steps + checkpoint_every - 1) / checkpoint_every);
@@ -1402,7 +1412,8 @@ int osph_adjoint_backward(osph_ctx* c, const double* seed_r, const double* seed_
 
 int osph_adjoint_clear(osph_ctx* c) {
     if (!c) return OSPH_ERR_ARG;
-    if (c->adj.ready) c->tune.reorder_period = c->adj.saved_reorder;
+    if (c->adj.reorder_saved) c->tune.reorder_period = c->adj.saved_reorder;
+    c->adj.reorder_saved = false;
     c->adj.ready = false;
     (void)hipFree(c->adj.ckpt);
     c->adj.ckpt = nullptr;

@@ -1436,6 +1436,7 @@ using namespace osph;
 
 struct osph_gravity {
     osph_gravity_config cfg{};
+    const void* owner = nullptr;  // the SPH context it is attached to (osph_attach_gravity), if any
     int precision = OSPH_PRECISION_FP64;  // the tree walk's arithmetic (osph_gravity_set_precision)
     hipStream_t stream = nullptr;
     size_t n = 0, cap = 0;
@@ -1959,6 +1960,9 @@ __global__ void __launch_bounds__(BLOCK) k_load_planes(Planes s, const double* _
 // another context's device planes: once `ready` has fired, the planes are copied in on this context's
 // stream, the tree is built and the self-gravity evaluated; `done` is recorded after the last kernel and
 // accel[0..2] then point to the accelerations (n doubles each, in the order of the source planes).
+const void* gravity_owner(const osph_gravity* g) { return g->owner; }
+void gravity_set_owner(osph_gravity* g, const void* owner) { g->owner = owner; }
+
 int gravity_self_from_device(osph_gravity* g, size_t n, const double* const src[5], hipEvent_t ready,
                              hipEvent_t done, const double* accel[3]) {
     if (!g || (n && !src)) return ERR_ARG;

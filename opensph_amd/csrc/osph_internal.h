@@ -298,5 +298,10 @@ void flaws_finish(const Launch& l, DevState s, const uint32_t* slot_of, size_t b
 // `done` is recorded on the gravity context's stream after the evaluation, and accel[0..2] then point
 // to the accelerations in the order of the source planes.  Returns a status of osph_gravity.h.
 struct osph_gravity;
+// The SPH context a gravity context is attached to (null: none).  One gravity context serves at most one
+// SPH context: its planes hold that context's particles and accelerations between the evaluation and the
+// pair sums.
+const void* gravity_owner(const osph_gravity* g);
+void gravity_set_owner(osph_gravity* g, const void* owner);
 int gravity_self_from_device(osph_gravity* g, size_t n, const double* const src[5], hipEvent_t ready,
                              hipEvent_t done, const double* accel[3]);

@@ -179,6 +179,67 @@ def device_flaws(solver, st: ParticleState, seed: int, k_weibull: float = 4.0e35
         solver.set_flaws(first, count, V, cg, seed + b, k_weibull=k_weibull, m_weibull=m_weibull)
 
 
+def contact_time(st: ParticleState, kernel_radius: float = 2.0) -> float:
+    """Earliest time at which a particle of one body comes within the kernel support of a particle of
+    another, with every body moving rigidly at its particles' mean velocity (the state of a collision set
+    up by CollisionGeometrySetupJob, core/run/jobs/ParticleJobs.cpp:532-577, before the bodies touch).
+    The support of a pair is kernel_radius * hbar = (h_i + h_j) for the M4 kernel; the smallest h of each
+    body is used, so at the returned time at least one pair is inside its support.  Until then every pair
+    of the run joins particles of one body, and a body at rest in its reference state (rho = rho0, u = 0,
+    S = 0) has zero pair sums.  Returns 0 for fewer than two bodies, inf when the bodies do not meet."""
+    from scipy.spatial import cKDTree
+
+    parts = bodies(st)
+    if len(parts) < 2:
+        return 0.0
+    r = np.asarray(st["r"])
+    v = np.asarray(st["v"])
+    best = math.inf
+    for a in range(len(parts)):
+        for b in range(a + 1, len(parts)):
+            fa, ca = parts[a]
+            fb, cb = parts[b]
+            pa, pb = r[fa:fa + ca, :3], r[fb:fb + cb, :3]
+            w = v[fb:fb + cb, :3].mean(axis=0) - v[fa:fa + ca, :3].mean(axis=0)
+            cut = 0.5 * kernel_radius * (float(r[fa:fa + ca, 3].min()) + float(r[fb:fb + cb, 3].min()))
+            tree = cKDTree(pa)
+
+            def gap(t):
+                d, _ = tree.query(pb + w * t, k=1)
+                return float(d.min()) - cut
+
+            g0 = gap(0.0)
+            if g0 < 0.0:
+                return 0.0
+            speed = float(np.linalg.norm(w))
+            if speed == 0.0:
+                continue
+            # the gap closes at most at the relative speed, so a step of gap / speed never passes the
+            # contact (sphere tracing); the floor of 1 % of the support bounds the step count, and the
+            # contact is then bisected inside the last step
+            reach = g0 + float(np.ptp(pa, axis=0).max()) + float(np.ptp(pb, axis=0).max())
+            lo, hi, g = 0.0, math.inf, g0
+            for _ in range(10000):
+                t = lo + max(g, 0.01 * cut) / speed
+                g = gap(t)
+                if g < 0.0:
+                    hi = t
+                    break
+                if g > reach:
+                    break
+                lo = t
+            if not math.isfinite(hi):
+                continue
+            for _ in range(50):
+                mid = 0.5 * (lo + hi)
+                if gap(mid) < 0.0:
+                    hi = mid
+                else:
+                    lo = mid
+            best = min(best, hi)
+    return best
+
+
 def kernel_w0() -> float:
     """Central value of the M4 LUT kernel, LutKernel::valueImpl(0) = CubicSpline<3>::valueImpl(0)."""
     q = 0.0
@@ -300,3 +361,35 @@ def impact(n_target: int = 950_000, n_projectile: int = 50_000, r_target: float 
         st["n_flaws"][sl] = nf
     cfg = RunConfig(solid_stress=1, adaptive_h=1, has_reduce=1, has_damage=1, criteria=15)
     return st, cfg
+
+
+def nbody_sphere(n: int, radius: float = 1.0e5, mass: float = 1.25e12, h: float = 10.0, seed: int = 7):
+    """Self-gravity N-body input: n equal masses uniform in a sphere (rejection sampling of numpy's PCG64
+    with an explicit seed), at rest, smoothing length h (the M4 softening support is 2 h).  Returns
+    (r (n, 4) with h in lane 3, m (n,))."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((0, 3))
+    while len(out) < n:
+        x = rng.uniform(-1.0, 1.0, size=(int(1.3 * (n - len(out))) + 1024, 3))
+        out = np.concatenate([out, x[np.einsum("ij,ij->i", x, x) < 1.0]])
+    r = np.c_[out[:n] * radius, np.full(n, h)]
+    return r, np.full(n, mass)
+
+
+def impact_ensemble(count: int, seed: int = 11, n_range=(5e3, 2e5), flaws: bool = True, offset_h: float = 4.0):
+    """Impact ensemble: N ~ exp(U(ln n_lo, ln n_hi)), R_p / R_t ~ U(0.05, 0.2) at equal particle spacing
+    (N_p / N_t = (R_p / R_t)^3, N_p >= 100), angle ~ U(0, 90) deg, speed ~ U(3, 7) km/s, R_t = 50 km,
+    basalt; one seed per member.  flaws=False leaves the Grady-Kipp flaws to the device
+    (device_flaws after the upload).  Returns a list of (state, config)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        ratio = rng.uniform(0.05, 0.2)
+        n_p = max(100, int(n * ratio ** 3 / (1 + ratio ** 3)))
+        n_t = max(100, int(n) - n_p)
+        st, cfg = impact(n_target=n_t, n_projectile=n_p, r_target=50e3, r_projectile=50e3 * ratio,
+                         speed=rng.uniform(3e3, 7e3), angle_deg=rng.uniform(0.0, 90.0),
+                         seed=int(rng.integers(1 << 30)), flaws=flaws, offset_h=offset_h)
+        out.append((st, cfg))
+    return out

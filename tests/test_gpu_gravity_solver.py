@@ -117,3 +117,32 @@ def test_gravity_on_empty_and_single_particle_states():
         if sub.n:
             assert np.all(g.download(sub)["dv"][:, :3] == 0.0)  # no other particle, no force
         g.close()
+
+
+def test_gravity_context_serves_one_solver():
+    """A gravity context holds one SPH context's particles and accelerations between its evaluation and
+    that context's pair sums, so attaching it to a second context fails until the first lets it go."""
+    from opensph_amd.gravity import GpuGravity
+
+    st, cfg = _cases()["fluid_exercised"]()
+    print(f"[case] gravity ownership n={st.n}", flush=True)
+    a = gpu.GpuSolver(st, cfg)
+    b = gpu.GpuSolver(st, cfg)
+    g = GpuGravity(st["r"], st["m"], solver="brute_force")
+    a.attach_gravity(g)
+    with pytest.raises(gpu.GpuError, match="another SPH context"):
+        b.attach_gravity(g)
+    a.attach_gravity(None)
+    b.attach_gravity(g)  # free again after the detach
+    b.integrate()
+    b.attach_gravity(None)
+    a.attach_gravity(g)
+    a.gravity = None  # the Python side must not close g with its owner
+    a.close()  # destroying the owner releases it too
+    c = gpu.GpuSolver(st, cfg)
+    c.attach_gravity(g)
+    c.attach_gravity(None)
+    for s in (b, c):
+        s.gravity = None
+        s.close()
+    g.close()
