@@ -1956,13 +1956,13 @@ __global__ void __launch_bounds__(BLOCK) k_load_planes(Planes s, const double* _
 
 }  // namespace
 
+const void* gravity_owner(const osph_gravity* g) { return g->owner; }
+void gravity_set_owner(osph_gravity* g, const void* owner) { g->owner = owner; }
+
 // GravitySolver::loop's gravity half (core/sph/solvers/GravitySolver.cpp:64-80) for particles held in
 // another context's device planes: once `ready` has fired, the planes are copied in on this context's
 // stream, the tree is built and the self-gravity evaluated; `done` is recorded after the last kernel and
 // accel[0..2] then point to the accelerations (n doubles each, in the order of the source planes).
-const void* gravity_owner(const osph_gravity* g) { return g->owner; }
-void gravity_set_owner(osph_gravity* g, const void* owner) { g->owner = owner; }
-
 int gravity_self_from_device(osph_gravity* g, size_t n, const double* const src[5], hipEvent_t ready,
                              hipEvent_t done, const double* accel[3]) {
     if (!g || (n && !src)) return ERR_ARG;
