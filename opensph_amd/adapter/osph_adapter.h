@@ -1,0 +1,126 @@
+// OpenSPH-side adapter of the MI355X hot path: the classes a maintainer adds to OpenSPH so that its own
+// run loop drives the C ABI of include/osph_gpu.h and include/osph_gravity.h.  Compiled inside OpenSPH
+// (against its headers, -I <OpenSPH>/core), linked with libosph_gpu.so.
+//
+//   GpuSymmetricSolver     ISolver (core/timestepping/ISolver.h:20-67) in place of SymmetricSolver<3>
+//                          (core/sph/solvers/SymmetricSolver.cpp:15-196): create() sets up the quantities as
+//                          the CPU solver does; integrate() evaluates the derivatives on the GPU.  It owns the
+//                          device context and the device copy of the state.
+//   GpuPredictorCorrector  ITimeStepping (core/timestepping/TimeStepping.h:46-87) in place of
+//                          PredictorCorrector (TimeStepping.cpp:272-346): the whole step, the time-step
+//                          criterion included, runs on the device with the state resident between steps.
+//   GpuGravity             IGravity (core/gravity/IGravity.h:16-64) in place of BarnesHut /
+//                          BruteForceGravity, as Factory::getGravity (core/system/Factory.cpp:361-413) builds them.
+#pragma once
+
+#include "osph_gpu.h"
+#include "osph_gravity.h"
+
+#include "gravity/IGravity.h"
+#include "quantities/Storage.h"
+#include "sph/solvers/SymmetricSolver.h"
+#include "system/Settings.h"
+#include "system/Statistics.h"
+#include "timestepping/ISolver.h"
+#include "timestepping/TimeStepCriterion.h"
+#include "timestepping/TimeStepping.h"
+
+NAMESPACE_SPH_BEGIN
+
+/// The library's run settings from OpenSPH's RunSettings (the entries named in osph_config's comments) and
+/// the quantities the storage holds (STRESS_REDUCING, DAMAGE).
+osph_config gpuRunConfig(const RunSettings& settings, const Storage& storage, int precision);
+
+/// The library's material parameters from one OpenSPH material: its BodySettings (Tillotson, rheology,
+/// damage, Drucker-Prager) and the ranges and minimal values the solver set in create().
+osph_material gpuMaterial(const IMaterial& material);
+
+/// Every quantity of osph_gpu.h the storage holds, host -> device, in the reference's element layouts
+/// (Vector 32 B, TracelessTensor and SymmetricTensor 64 B, Float, Size); MATERIAL_ID from the storage's
+/// material sequences.  Resizes the device state to the storage's particle count.
+void uploadStorage(osph_ctx* ctx, const Storage& storage);
+
+/// The mirror image of uploadStorage: every quantity of osph_gpu.h the storage holds, device -> host.
+void downloadStorage(osph_ctx* ctx, Storage& storage);
+
+/// SymmetricSolver<3> on the GPU.  The device copy of the state is either current (after upload() or a
+/// resident step) or stale; integrate() called by a host-side integrator uploads the state, evaluates and
+/// downloads the derivatives, as SymmetricSolver<3>::integrate leaves them in the storage.
+class GpuSymmetricSolver : public ISolver {
+    osph_ctx* ctx = nullptr;
+    SymmetricSolver<3> cpuTerms;  // create(): the same quantities, ranges and minimal values as the CPU solver
+    RunSettings settings;
+    int precision;
+    bool resident = false;
+
+public:
+    GpuSymmetricSolver(IScheduler& scheduler, const RunSettings& settings, const EquationHolder& eqs,
+        int precision = OSPH_PRECISION_FP64);
+    ~GpuSymmetricSolver() override;
+
+    void integrate(Storage& storage, Statistics& stats) override;
+    void create(Storage& storage, IMaterial& material) const override;
+
+    /// Creates the device context for the storage's materials (after create() ran for every body) and
+    /// uploads the state; the device state is then current.
+    void upload(const Storage& storage);
+    /// Device state -> storage (before output, or before handing the storage to host code).
+    void sync(Storage& storage) const;
+    /// Whether the device holds the current state (resident stepping).
+    bool isResident() const {
+        return resident;
+    }
+    /// The host changed the storage: the next resident step uploads it again.
+    void invalidate() {
+        resident = false;
+    }
+    osph_ctx* context() const {
+        return ctx;
+    }
+    void check(int rc) const;
+
+private:
+    void ensureContext(const Storage& storage);
+};
+
+/// PredictorCorrector with the state resident in HBM: each step is one osph_step_pc (predictions, the
+/// derivatives, corrections and the MultiCriterion time step) on the solver's device state.  The solver
+/// passed to step() must be a GpuSymmetricSolver; the storage is uploaded on the first step (or after
+/// invalidate()) and brought back by GpuSymmetricSolver::sync.  The time step comes from the device, so
+/// the base class runs without a host criterion; the criterion of the last step is criterion().
+class GpuPredictorCorrector : public ITimeStepping {
+    CriterionId lastCriterion = CriterionId::INITIAL_VALUE;
+
+public:
+    GpuPredictorCorrector(const SharedPtr<Storage>& storage, const RunSettings& settings);
+
+    CriterionId criterion() const {
+        return lastCriterion;
+    }
+
+protected:
+    void stepParticles(IScheduler& scheduler, ISolver& solver, Statistics& stats) override;
+};
+
+/// BarnesHut / BruteForceGravity on the GPU with the settings Factory::getGravity reads.
+class GpuGravity : public IGravity {
+    osph_gravity* g = nullptr;
+
+public:
+    explicit GpuGravity(const RunSettings& settings);
+    ~GpuGravity() override;
+
+    void build(IScheduler& scheduler, const Storage& storage) override;
+    void evalSelfGravity(IScheduler& scheduler, ArrayView<Vector> dv, Statistics& stats) const override;
+    void evalAttractors(IScheduler& scheduler,
+        ArrayView<Attractor> attractors,
+        ArrayView<Vector> dv) const override;
+    Vector evalAcceleration(const Vector& r0) const override;
+    Float evalEnergy(IScheduler& scheduler, Statistics& stats) const override;
+    RawPtr<const IBasicFinder> getFinder() const override;
+
+private:
+    void check(int rc) const;
+};
+
+NAMESPACE_SPH_END

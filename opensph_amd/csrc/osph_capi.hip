@@ -1,5 +1,6 @@
 // C ABI of the MI355X OpenSPH hot path (declared in include/osph_gpu.h): context, HBM-resident state,
 // neighbour build orchestration, SymmetricSolver<3>::integrate and PredictorCorrector::step.
+#include "osph_device.h"
 #include "osph_internal.h"
 #include "osph_gravity.h"
 
@@ -20,6 +21,47 @@ constexpr double PI_H = 3.14159265358979323846264338327950288;
 struct ProfEvent {
     const char* name;
     hipEvent_t start, stop;
+};
+
+// Device and pinned host buffers of a batched step (osph_step_pc_batch over contexts with particles):
+// the block tables of the batched launches, the member table, the members' accumulators, the batch's
+// sort arrays and the captured HIP graph of the stages after the sort.  Owned by the batch's first
+// member; rebuilt when the members, their sizes or the launch tuning change.
+struct BatchPlan {
+    std::vector<osph_ctx*> members;
+    std::vector<size_t> sizes;
+    Tuning tune{};
+    bool mixed = false;
+    size_t total = 0;
+    std::vector<uint32_t> pfirst;  // host copy of the particle table's first[]
+    uint32_t* tables = nullptr;
+    BatchBlocks pb{}, lb{}, qb{};
+    BatchMember* dev = nullptr;
+    BatchMember* host = nullptr;
+    unsigned long long* small = nullptr;
+    unsigned long long* small_host = nullptr;
+    uint64_t *keys = nullptr, *keys_sorted = nullptr;
+    uint32_t *vals = nullptr, *vals_sorted = nullptr;
+    void* sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_bits = -1, graph_screened = -1, graph_plain = -1;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_bits = graph_screened = graph_plain = -1;
+    }
+    ~BatchPlan() {
+        drop_graph();
+        void* dp[] = { tables, dev, small, keys, keys_sorted, vals, vals_sorted, sort_temp };
+        for (void* p : dp) (void)hipFree(p);
+        if (host) (void)hipHostFree(host);
+        if (small_host) (void)hipHostFree(small_host);
+    }
 };
 
 }  // namespace
@@ -100,6 +142,7 @@ struct osph_ctx {
 
     Tuning tune{};
     std::string err;
+    BatchPlan* batch = nullptr;  // batched steps led by this context (osph_step_pc_batch)
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -404,6 +447,21 @@ Screen make_screen(const osph_ctx* c, const double lo[3], const double hi[3], do
     return sc;
 }
 
+// The class reference h of the first search after an upload: the smallest h (one host synchronisation).
+int ensure_href(osph_ctx* c) {
+    if (c->href > 0.0) return OSPH_OK;
+    Launch l = L(c);
+    timed(c, "init_small", [&] { launch_init_small(l, c->small); });
+    timed(c, "bounds", [&] { launch_bounds_reduce(l, c->s, c->small, c->n); });
+    HIPCHK(c, hipMemcpyAsync(c->small_host, c->small, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    if (int e = sync_check(c)) return e;
+    const double h0 = decode_ord(c->small_host[7]);
+    if (!(h0 > 0.0) || !std::isfinite(h0)) return c->fail(OSPH_ERR_STATE, "smoothing lengths must be positive");
+    c->href = h0;
+    return OSPH_OK;
+}
+
 // The pass before the neighbour search (k_advance: predictions for mode 2, the material initialize for
 // modes 1 and 2) with the per-class grid bounds, then the grid, the sort by cell, the cell ranges and
 // the packed neighbour records (ISymmetricFinder::build).  One host synchronisation reads the class
@@ -412,16 +470,8 @@ Screen make_screen(const osph_ctx* c, const double lo[3], const double hi[3], do
 int search_begin(osph_ctx* c, int mode, double dt) {
     const size_t n = c->n;
     Launch l = L(c);
+    if (int e = ensure_href(c)) return e;
     timed(c, "init_small", [&] { launch_init_small(l, c->small); });
-    if (!(c->href > 0.0)) {
-        timed(c, "bounds", [&] { launch_bounds_reduce(l, c->s, c->small, n); });
-        HIPCHK(c, hipMemcpyAsync(c->small_host, c->small, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                 c->stream));
-        if (int e = sync_check(c)) return e;
-        const double h0 = decode_ord(c->small_host[7]);
-        if (!(h0 > 0.0) || !std::isfinite(h0)) return c->fail(OSPH_ERR_STATE, "smoothing lengths must be positive");
-        c->href = h0;
-    }
     const double href = c->href;
     unsigned long long* cacc = c->small + SMALL_CLASSES;
     unsigned long long* vacc = c->cfg.precision == OSPH_PRECISION_MIXED ? c->small + SMALL_VEL : nullptr;
@@ -440,20 +490,18 @@ int search_begin(osph_ctx* c, int mode, double dt) {
     return OSPH_OK;
 }
 
-// Second half: waits for the class bounds, then the grid, the sort by cell, the ranges and the records.
-int search_finish(osph_ctx* c) {
-    const size_t n = c->n;
-    Launch l = L(c);
+// The plan of a neighbour search from the class (and velocity) bounds in sh (a host copy of the SMALL_*
+// accumulators): the grid, the FP32 screen and, in mixed precision, the lattices; the class reference h
+// of the next search; the cell and row tables sized.  *rows_out = the rows of the grid.  Host only.
+int plan_search(osph_ctx* c, const unsigned long long* sh, size_t* rows_out) {
     const double href = c->href;
-    if (int e = sync_check(c)) return e;
-
     double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
     double hmin = INFINITY, hmax = 0.0;
     double clo[MAX_CLASSES][3], chi[MAX_CLASSES][3], chmax[MAX_CLASSES];
     unsigned long long ccount[MAX_CLASSES];
     int last = 0;
     for (int cl = 0; cl < MAX_CLASSES; ++cl) {
-        const unsigned long long* a = c->small_host + SMALL_CLASSES + CLASS_STRIDE * cl;
+        const unsigned long long* a = sh + SMALL_CLASSES + CLASS_STRIDE * cl;
         ccount[cl] = a[8];
         if (!ccount[cl]) continue;
         last = cl;
@@ -476,8 +524,7 @@ int search_finish(osph_ctx* c) {
     }
     c->href = hmin;  // class reference of the next search
 
-    const Screen sc = make_screen(c, lo, hi, hmin);
-    c->screen = sc;
+    c->screen = make_screen(c, lo, hi, hmin);
     Grid g{};
     g.hmin = href;
     g.nclass = last + 1;
@@ -517,8 +564,77 @@ int search_finish(osph_ctx* c) {
     g.key_bits = 1;
     while (g.key_bits < 32 && (uint64_t(1) << g.key_bits) < uint64_t(base)) ++g.key_bits;
     c->grid = g;
-    const size_t cells = base;
-    if (int e = ensure_cells(c, cells)) return e;
+    if (int e = ensure_cells(c, size_t(base))) return e;
+    size_t rows = 0;
+    for (int cl = 0; cl < g.nclass; ++cl) {
+        if (!g.c[cl].empty) rows += size_t(g.c[cl].dim[1]) * size_t(g.c[cl].dim[2]);
+    }
+    if (int e = ensure_rows(c, rows)) return e;
+    *rows_out = rows;
+    if (c->cfg.precision == OSPH_PRECISION_MIXED) {
+        // lattice of power-of-two edge >= 2 h_min (10 bits per axis) for the FP32 relative positions
+        Lattice lat{};
+        double S = std::exp2(std::ceil(std::log2(2.0 * hmin)));
+        double ext = 0.0;
+        for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
+        while (ext / S > 1000.0) S *= 2.0;
+        for (int k = 0; k < 3; ++k) lat.lo[k] = lo[k];
+        lat.S = S;
+        lat.Sf = float(S);
+        // velocity lattice: power-of-two edge with the velocity range within 1000 cells
+        double vrange = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            lat.vlo[k] = decode_ord(sh[SMALL_VEL + k]);
+            const double vhi = decode_ord(sh[SMALL_VEL + 3 + k]);
+            if (!std::isfinite(lat.vlo[k]) || !std::isfinite(vhi)) {
+                return c->fail(OSPH_ERR_STATE, "velocities must be finite");
+            }
+            vrange = std::max(vrange, vhi - lat.vlo[k]);
+        }
+        double Sv = 1.0;
+        if (vrange > 0.0) Sv = std::exp2(std::ceil(std::log2(vrange / 1000.0)));
+        while (vrange / Sv > 1000.0) Sv *= 2.0;
+        Sv = std::max(Sv, std::exp2(-100.0));  // keeps the FP32 copy of the edge normal
+        lat.Sv = Sv;
+        lat.Svf = float(Sv);
+        c->lattice = lat;
+    }
+    return OSPH_OK;
+}
+
+// Whether this search moves the state into cell order (every tune.reorder_period searches).
+bool reorder_due(osph_ctx* c) {
+    if (c->tune.reorder_period > 0 && ++c->since_reorder >= c->tune.reorder_period) {
+        c->since_reorder = 0;
+        return true;
+    }
+    return false;
+}
+
+// The second block of state planes, which a reorder fills.
+DevState other_state(const osph_ctx* c) {
+    DevState dst = c->s;
+    for (int k = 0; k < NDPLANES; ++k) dst.d[k] = c->dblock2 + size_t(k) * c->cap;
+    for (int k = 0; k < NIPLANES; ++k) dst.u[k] = c->ublock2 + size_t(k) * c->cap;
+    return dst;
+}
+
+// After a reorder into the second block: it becomes the state.
+void swap_state(osph_ctx* c, const DevState& dst) {
+    c->s = dst;
+    std::swap(c->dblock, c->dblock2);
+    std::swap(c->ublock, c->ublock2);
+    std::swap(c->perm, c->perm2);
+}
+
+// Second half: waits for the class bounds, then the grid, the sort by cell, the ranges and the records.
+int search_finish(osph_ctx* c) {
+    const size_t n = c->n;
+    Launch l = L(c);
+    if (int e = sync_check(c)) return e;
+    size_t rows = 0;
+    if (int e = plan_search(c, c->small_host, &rows)) return e;
+    const Grid g = c->grid;
 
     timed(c, "cell_keys", [&] { launch_cell_keys(l, c->s, g, c->keys, c->iota, n); });
     const size_t sort_need = sort_pairs_temp_bytes(n, g.key_bits);
@@ -536,62 +652,25 @@ int search_finish(osph_ctx* c) {
                               g.key_bits);
     });
     HIPCHK(c, sort_err);
-    size_t rows = 0;
-    for (int cl = 0; cl < g.nclass; ++cl) {
-        if (!g.c[cl].empty) rows += size_t(g.c[cl].dim[1]) * size_t(g.c[cl].dim[2]);
-    }
-    if (int e = ensure_rows(c, rows)) return e;
     timed(c, "cell_ranges", [&] {
         launch_cell_first(l, c->keys_sorted, g, c->row_range, c->cell_first, rows, n);
     });
-    if (c->tune.reorder_period > 0 && ++c->since_reorder >= c->tune.reorder_period) {
+    if (reorder_due(c)) {
         // move the state into this search's cell order: the record gather and the pair kernels' writes
         // then run over (nearly) consecutive slots until the next reorder
+        const DevState dst = other_state(c);
         timed(c, "reorder", [&] {
-            DevState dst = c->s;
-            for (int k = 0; k < NDPLANES; ++k) dst.d[k] = c->dblock2 + size_t(k) * c->cap;
-            for (int k = 0; k < NIPLANES; ++k) dst.u[k] = c->ublock2 + size_t(k) * c->cap;
             launch_permute_state(l, c->s, dst, c->order, c->perm, c->perm2, c->slot_of, n);
             launch_iota(l, c->order, n);
-            c->s = dst;
         });
-        std::swap(c->dblock, c->dblock2);
-        std::swap(c->ublock, c->ublock2);
-        std::swap(c->perm, c->perm2);
-        c->since_reorder = 0;
+        swap_state(c, dst);
     }
     if (c->cfg.precision == OSPH_PRECISION_MIXED) {
-        // lattice of power-of-two edge >= 2 h_min (10 bits per axis) for the FP32 relative positions
-        Lattice lat{};
-        double S = std::exp2(std::ceil(std::log2(2.0 * hmin)));
-        double ext = 0.0;
-        for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
-        while (ext / S > 1000.0) S *= 2.0;
-        for (int k = 0; k < 3; ++k) lat.lo[k] = lo[k];
-        lat.S = S;
-        lat.Sf = float(S);
-        // velocity lattice: power-of-two edge with the velocity range within 1000 cells
-        double vrange = 0.0;
-        for (int k = 0; k < 3; ++k) {
-            lat.vlo[k] = decode_ord(c->small_host[SMALL_VEL + k]);
-            const double vhi = decode_ord(c->small_host[SMALL_VEL + 3 + k]);
-            if (!std::isfinite(lat.vlo[k]) || !std::isfinite(vhi)) {
-                return c->fail(OSPH_ERR_STATE, "velocities must be finite");
-            }
-            vrange = std::max(vrange, vhi - lat.vlo[k]);
-        }
-        double Sv = 1.0;
-        if (vrange > 0.0) Sv = std::exp2(std::ceil(std::log2(vrange / 1000.0)));
-        while (vrange / Sv > 1000.0) Sv *= 2.0;
-        Sv = std::max(Sv, std::exp2(-100.0));  // keeps the FP32 copy of the edge normal
-        lat.Sv = Sv;
-        lat.Svf = float(Sv);
-        c->lattice = lat;
         timed(c, "pack", [&] {
-            launch_pack_mixed(l, c->s, c->prm, lat, c->order, c->recf, c->pos, c->posf, sc, c->group, n);
+            launch_pack_mixed(l, c->s, c->prm, c->lattice, c->order, c->recf, c->pos, c->posf, c->screen, c->group, n);
         });
     } else {
-        timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->pos, c->posf, sc, c->group, n); });
+        timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->pos, c->posf, c->screen, c->group, n); });
     }
     return OSPH_OK;
 }
@@ -839,6 +918,8 @@ void osph_destroy(osph_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->grav && gravity_owner(c->grav) == c) gravity_set_owner(c->grav, nullptr);
+    delete c->batch;
+    c->batch = nullptr;
     resolve_events(c);
     for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     if (c->grav_ready) (void)hipEventDestroy(c->grav_ready);
@@ -1114,6 +1195,238 @@ int osph_step_pc(osph_ctx* c, double dt, double* next_dt, int* crit) {
     return step_end(c, dt, next_dt, crit);
 }
 
+}  // extern "C"
+
+namespace {
+
+bool same_launch_tuning(const Tuning& a, const Tuning& b) {
+    return a.pairs_block == b.pairs_block && a.pairs_lds == b.pairs_lds && a.pairs_lanes == b.pairs_lanes &&
+           a.lists_block == b.lists_block && a.lists_lds == b.lists_lds && a.fuse_finalize == b.fuse_finalize &&
+           a.batch_graph == b.batch_graph;
+}
+
+// The plan of a batch of members (each with particles), led by ms[0].
+int make_plan(const std::vector<osph_ctx*>& ms, BatchPlan*& out) {
+    osph_ctx* lead = ms[0];
+    BatchPlan* old = lead->batch;
+    if (old && old->members == ms && same_launch_tuning(old->tune, lead->tune)) {
+        bool same = true;
+        for (size_t k = 0; k < ms.size(); ++k) same = same && old->sizes[k] == ms[k]->n;
+        if (same) {
+            out = old;
+            return OSPH_OK;
+        }
+    }
+    HIPCHK(lead, hipStreamSynchronize(lead->stream));
+    delete old;
+    lead->batch = nullptr;
+    BatchPlan* P = new BatchPlan();
+    lead->batch = P;
+    const size_t count = ms.size();
+    P->members = ms;
+    P->tune = lead->tune;
+    P->mixed = lead->cfg.precision == OSPH_PRECISION_MIXED;
+    // block tables: particles (BLOCK threads), lists (lists_block), pairs (pairs_block, lanes per particle)
+    std::vector<uint32_t> host;
+    auto table = [&](int block, int per, BatchBlocks& t, std::vector<uint32_t>& first) {
+        first.assign(count + 1, 0u);
+        for (size_t k = 0; k < count; ++k) first[k + 1] = first[k] + batch_blocks_for(ms[k]->n * size_t(per), block);
+        t.total = first[count];
+    };
+    std::vector<uint32_t> pf, lf, qf;
+    table(BLOCK, 1, P->pb, pf);
+    table(P->tune.lists_block, 1, P->lb, lf);
+    table(P->tune.pairs_block, P->tune.pairs_lanes, P->qb, qf);
+    P->pfirst = pf;
+    size_t words = 0;
+    for (const auto* f : { &pf, &lf, &qf }) words += f->size() + f->back();
+    std::vector<uint32_t> all;
+    all.reserve(words);
+    size_t offs[3][2];
+    int ti = 0;
+    for (const auto* f : { &pf, &lf, &qf }) {
+        offs[ti][0] = all.size();
+        for (size_t k = 0; k < count; ++k) {
+            for (uint32_t b = (*f)[k]; b < (*f)[k + 1]; ++b) all.push_back(uint32_t(k));
+        }
+        offs[ti][1] = all.size();
+        all.insert(all.end(), f->begin(), f->end());
+        ++ti;
+    }
+    HIPCHK(lead, hipMalloc(&P->tables, all.size() * sizeof(uint32_t)));
+    HIPCHK(lead, hipMemcpy(P->tables, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    BatchBlocks* tabs[3] = { &P->pb, &P->lb, &P->qb };
+    for (int t = 0; t < 3; ++t) {
+        tabs[t]->member = P->tables + offs[t][0];
+        tabs[t]->first = P->tables + offs[t][1];
+    }
+    HIPCHK(lead, hipMalloc(&P->dev, count * sizeof(BatchMember)));
+    HIPCHK(lead, hipHostMalloc(&P->host, count * sizeof(BatchMember)));
+    std::memset(static_cast<void*>(P->host), 0, count * sizeof(BatchMember));
+    HIPCHK(lead, hipMalloc(&P->small, count * SMALL_WORDS * sizeof(unsigned long long)));
+    HIPCHK(lead, hipHostMalloc(&P->small_host, count * SMALL_WORDS * sizeof(unsigned long long)));
+    for (osph_ctx* c : ms) {
+        P->sizes.push_back(c->n);
+        P->total += c->n;
+    }
+    if (P->total >= (size_t(1) << 32)) return lead->fail(OSPH_ERR_ARG, "a batch must hold fewer than 2^32 particles");
+    HIPCHK(lead, hipMalloc(&P->keys, P->total * sizeof(uint64_t)));
+    HIPCHK(lead, hipMalloc(&P->keys_sorted, P->total * sizeof(uint64_t)));
+    HIPCHK(lead, hipMalloc(&P->vals, P->total * sizeof(uint32_t)));
+    HIPCHK(lead, hipMalloc(&P->vals_sorted, P->total * sizeof(uint32_t)));
+    // sized once for the widest key, so that the captured graph's arguments stay valid
+    P->sort_temp_bytes = std::max<size_t>(sort_pairs64_temp_bytes(P->total, 64), 256);
+    HIPCHK(lead, hipMalloc(&P->sort_temp, P->sort_temp_bytes));
+    out = P;
+    return OSPH_OK;
+}
+
+// One predictor-corrector step of every member with one launch per stage (ms: the members with particles,
+// dts: their time steps; next_dt / crit indexed like ms).  The same per-particle code and buffers as
+// osph_step_pc on each member alone, so each member's result is bit-identical to stepping it alone.
+int batch_step(const std::vector<osph_ctx*>& ms, const std::vector<double>& dts, double* next_dt, int* crit) {
+    osph_ctx* lead = ms[0];
+    BatchPlan* P = nullptr;
+    if (int e = make_plan(ms, P)) return e;
+    const size_t count = ms.size();
+    // the members' own streams are idle (their last calls may have left work queued), and each has the
+    // class reference h of its first search
+    for (osph_ctx* c : ms) {
+        if (hipStreamQuery(c->stream) != hipSuccess) HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->neighbors_valid = false;
+        if (int e = ensure_href(c)) return e;
+    }
+    for (size_t k = 0; k < count; ++k) {
+        osph_ctx* c = ms[k];
+        const size_t need = size_t(P->pfirst[k + 1] - P->pfirst[k]) * bounds_values();
+        if (need > c->bounds_part_cap) {
+            (void)hipFree(c->bounds_part);
+            c->bounds_part = nullptr;
+            c->bounds_part_cap = 0;
+            HIPCHK(c, hipMalloc(&c->bounds_part, need * sizeof(unsigned long long)));
+            c->bounds_part_cap = need;
+        }
+        BatchMember& b = P->host[k];
+        b.s = c->s;
+        b.mats = c->mats_dev;
+        b.prm = c->prm;
+        b.dt = dts[k];
+        b.href = c->href;
+        b.part = c->bounds_part;
+        b.n = uint32_t(c->n);
+        b.small = P->small + SMALL_WORDS * k;
+    }
+    const Launch l{ lead->stream, P->tune };
+    hipStream_t st = lead->stream;
+    // predictions, material initialize and grid bounds of every member; one readback of all bounds
+    HIPCHK(lead, hipMemcpyAsync(P->dev, P->host, count * sizeof(BatchMember), hipMemcpyHostToDevice, st));
+    batch_advance(l, P->dev, int(count), P->pb, P->mixed);
+    HIPCHK(lead, hipMemcpyAsync(P->small_host, P->small, count * SMALL_WORDS * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, st));
+    if (int e = sync_check(lead)) return e;
+    // each member's grid, screen and lattices, planned as its own search plans them
+    int key_bits = 1;
+    bool screened = false, plain = false;
+    uint32_t off = 0;
+    for (size_t k = 0; k < count; ++k) {
+        osph_ctx* c = ms[k];
+        BatchMember& b = P->host[k];
+        size_t rows = 0;
+        if (int e = plan_search(c, P->small_host + SMALL_WORDS * k, &rows)) return e;
+        if (int e = ensure_list(c, list_entries(c->n, c->kmax))) return e;
+        int xslots = 1;
+        for (int cl = 0; cl < c->grid.nclass; ++cl) {
+            if (!c->grid.c[cl].empty) xslots = std::max(xslots, c->grid.c[cl].dim[0] + 1);
+        }
+        b.grid = c->grid;
+        b.screen = c->screen;
+        b.lat = c->lattice;
+        b.kmax = c->kmax;
+        b.rows = uint32_t(rows);
+        b.xslots = xslots;
+        b.reorder = reorder_due(c) ? 1 : 0;
+        b.s2 = b.reorder ? other_state(c) : c->s;
+        b.key_off = off;
+        off += uint32_t(c->n);
+        b.keys_sorted = c->keys_sorted;
+        b.order = c->order;
+        b.row_range = c->row_range;
+        b.cell_first = c->cell_first;
+        b.perm = c->perm;
+        b.perm2 = c->perm2;
+        b.slot_of = c->slot_of;
+        b.rec = P->mixed ? static_cast<void*>(c->recf) : static_cast<void*>(c->rec);
+        b.pos = c->pos;
+        b.posf = c->posf;
+        b.group = c->group;
+        b.list = c->list;
+        b.count = c->count;
+        key_bits = std::max(key_bits, c->grid.key_bits);
+        (b.screen.on ? screened : plain) = true;
+    }
+    int member_bits = 0;
+    while ((size_t(1) << member_bits) < count) ++member_bits;
+    HIPCHK(lead, hipMemcpyAsync(P->dev, P->host, count * sizeof(BatchMember), hipMemcpyHostToDevice, st));
+    batch_cell_keys(l, P->dev, P->pb, key_bits, P->keys, P->vals);
+    HIPCHK(lead, sort_pairs64(l, P->sort_temp, P->sort_temp_bytes, P->keys, P->keys_sorted, P->vals, P->vals_sorted,
+                              P->total, key_bits + member_bits));
+    const bool fuse = P->tune.fuse_finalize == 1 || (P->tune.fuse_finalize == -1 && !P->mixed);
+    // the stages after the sort: their launch shapes and arguments are fixed for the plan (the per-step
+    // values are in the member table), so they are captured once and replayed
+    auto rest = [&]() {
+        batch_split(l, P->dev, P->pb, key_bits, P->keys_sorted, P->vals_sorted);
+        batch_cells(l, P->dev, P->pb);
+        batch_permute(l, P->dev, P->pb);
+        batch_pack(l, P->dev, P->pb, P->mixed);
+        batch_lists(l, P->dev, P->lb, screened, plain);
+        batch_pairs(l, P->dev, P->qb, P->mixed, fuse, lead->lut);
+        if (!fuse) batch_finalize_correct(l, P->dev, P->pb);
+    };
+    if (P->tune.batch_graph) {
+        if (!P->exec || P->graph_bits != key_bits || P->graph_screened != int(screened) ||
+            P->graph_plain != int(plain)) {
+            P->drop_graph();
+            HIPCHK(lead, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rest();
+            HIPCHK(lead, hipStreamEndCapture(st, &P->graph));
+            HIPCHK(lead, hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+            P->graph_bits = key_bits;
+            P->graph_screened = int(screened);
+            P->graph_plain = int(plain);
+        }
+        HIPCHK(lead, hipGraphLaunch(P->exec, st));
+    } else {
+        rest();
+    }
+    HIPCHK(lead, hipMemcpyAsync(P->small_host, P->small, count * SMALL_WORDS * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, st));
+    if (int e = sync_check(lead)) return e;
+    for (size_t k = 0; k < count; ++k) {
+        osph_ctx* c = ms[k];
+        if (P->host[k].reorder) swap_state(c, P->host[k].s2);
+        const unsigned long long* sh = P->small_host + SMALL_WORDS * k;
+        uint32_t flag;
+        std::memcpy(&flag, &sh[SMALL_OVERFLOW], sizeof(uint32_t));
+        if (!flag) {
+            select_timestep(c, sh + SMALL_CRITERIA, next_dt ? next_dt + k : nullptr, crit ? crit + k : nullptr);
+            c->neighbors_valid = true;
+            continue;
+        }
+        // a list overflowed: the member's finalize left its predicted state, which its own stream now
+        // finishes with larger lists, as osph_step_pc does
+        launch_init_small(L(c), c->small);
+        if (int e = grow_lists(c)) return e;
+        if (int e = launch_lists(c)) return e;
+        launch_second_half(c, dts[k]);
+        if (int e = step_end(c, dts[k], next_dt ? next_dt + k : nullptr, crit ? crit + k : nullptr)) return e;
+    }
+    return OSPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* next_dt, int* crit) {
     if (count < 0 || (count > 0 && (!ctxs || !dt))) return OSPH_ERR_ARG;
     for (int k = 0; k < count; ++k) {
@@ -1121,6 +1434,38 @@ int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* nex
         for (int j = 0; j < k; ++j) {
             if (ctxs[j] == ctxs[k]) return ctxs[k]->fail(OSPH_ERR_ARG, "a context appears twice in the batch");
         }
+    }
+    // one launch per stage when the members with particles share the precision and the lanes per particle
+    // (which fix the order of the pair sums) and none has self-gravity attached
+    std::vector<osph_ctx*> ms;
+    std::vector<double> dts;
+    std::vector<int> at;
+    bool batched = true;
+    for (int k = 0; k < count; ++k) {
+        osph_ctx* c = ctxs[k];
+        if (!(dt[k] > 0.0) || !std::isfinite(dt[k])) return c->fail(OSPH_ERR_ARG, "time step must be positive");
+        if (!c->n) continue;
+        if (!ms.empty() && (c->cfg.precision != ms[0]->cfg.precision || c->tune.pairs_lanes != ms[0]->tune.pairs_lanes)) {
+            batched = false;
+        }
+        if (c->grav) batched = false;
+        ms.push_back(c);
+        dts.push_back(dt[k]);
+        at.push_back(k);
+    }
+    if (batched && ms.size() > 1) {
+        std::vector<double> nd(ms.size());
+        std::vector<int> nc(ms.size());
+        if (int e = batch_step(ms, dts, nd.data(), nc.data())) return e;
+        for (int k = 0; k < count; ++k) {
+            if (ctxs[k]->n) continue;
+            if (int e = step_end(ctxs[k], dt[k], next_dt ? next_dt + k : nullptr, crit ? crit + k : nullptr)) return e;
+        }
+        for (size_t j = 0; j < ms.size(); ++j) {
+            if (next_dt) next_dt[at[j]] = nd[j];
+            if (crit) crit[at[j]] = nc[j];
+        }
+        return OSPH_OK;
     }
     for (int k = 0; k < count; ++k) {
         if (int e = step_begin(ctxs[k], dt[k])) return e;
@@ -1218,6 +1563,8 @@ int osph_tune(osph_ctx* c, const char* key, long long value) {
         c->tune.reorder_period = int(value);
     } else if (k == "fuse_finalize" && (value == -1 || value == 0 || value == 1)) {
         c->tune.fuse_finalize = int(value);
+    } else if (k == "batch_graph" && (value == 0 || value == 1)) {
+        c->tune.batch_graph = int(value);
     } else {
         return c->fail(OSPH_ERR_ARG, "unknown tuning key or value out of range: " + k);
     }

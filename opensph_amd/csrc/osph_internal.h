@@ -165,6 +165,7 @@ struct Tuning {
     int reorder_period = 10;  // move the state into cell order every this many searches (0: never)
     int fuse_finalize = -1;   // PC step: finalize, corrector and criteria in the pair kernel: 1 always, 0 never,
                               // -1 for FP64 only (measured faster there, slower in mixed precision)
+    int batch_graph = 1;      // batched steps replay their stages after the sort as a captured HIP graph
 };
 
 struct Launch {
@@ -198,6 +199,8 @@ void launch_advance(const Launch& l, DevState s, const osph_material* mats, Para
                     size_t n);
 // Entries of the per-block partial bounds that launch_advance needs for n particles.
 size_t bounds_partials(size_t n);
+// Values of the per-block partial bounds (partials per block).
+size_t bounds_values();
 
 // ---- adjoint (osph_adjoint.hip) -----------------------------------------------------------------------
 // Materials whose Tillotson parameters (a, b, A) get a gradient.
@@ -279,6 +282,65 @@ void launch_integrals(const Launch& l, DevState s, double* scratch, size_t n);
 size_t sort_pairs_temp_bytes(size_t n, int key_bits);
 hipError_t sort_pairs(const Launch& l, void* temp, size_t temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
                       const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits);
+
+// ---- batched stepping: one launch per stage for every member of an ensemble -----------------------------
+// What the kernels of one predictor-corrector step read for one member of a batch (in device memory, one
+// entry per member).  Every batched kernel runs the same per-particle code as the single-run kernel,
+// with the member's own buffers and parameters, so each member's results are those of stepping it alone.
+struct BatchMember {
+    DevState s;    // the state before this step's reorder
+    DevState s2;   // the state from this step's reorder on (= s when the step does not reorder)
+    const osph_material* mats;
+    Params prm;
+    double dt, href;
+    Grid grid;
+    Screen screen;
+    Lattice lat;
+    uint32_t n, kmax;
+    uint32_t rows;    // rows of the member's grid (all classes)
+    int xslots;       // largest dim_x + 1 over its classes
+    int reorder;      // this step moves the state into cell order (s -> s2)
+    uint32_t key_off; // first position of the member in the batch's sort arrays
+    unsigned long long* small;  // accumulators (SMALL_* layout)
+    unsigned long long* part;   // per-block partial bounds
+    uint32_t *keys_sorted, *order, *row_range, *cell_first;
+    uint32_t *perm, *perm2, *slot_of;
+    void* rec;        // PairRec (FP64) or PairRecF (mixed precision)
+    double4* pos;
+    float4* posf;
+    uint32_t* group;
+    uint32_t *list, *count;
+};
+
+// Block b of a batched launch belongs to member member[b]; the member's blocks are first[m] ..
+// first[m + 1] - 1, a multiple of 8 of them, so that each member's blocks are dealt to the 8 XCDs as in
+// a launch of its own.
+struct BatchBlocks {
+    const uint32_t* member;
+    const uint32_t* first;
+    uint32_t total;
+};
+// Blocks of `block` threads for n items, rounded up to a multiple of 8.
+inline uint32_t batch_blocks_for(size_t n, int block) {
+    const size_t g = (n + size_t(block) - 1) / size_t(block);
+    return uint32_t(((g > 0 ? g : 1) + 7) / 8 * 8);
+}
+// Stage launchers over all `count` members (M in device memory).  `pb`: blocks of BLOCK threads per
+// particle; `lb`: blocks of the list kernel's size; `qb`: blocks of the pair kernel's size and lanes.
+void batch_advance(const Launch& l, const BatchMember* M, int count, BatchBlocks pb, bool vel);
+void batch_cell_keys(const Launch& l, const BatchMember* M, BatchBlocks pb, int key_bits, uint64_t* keys,
+                     uint32_t* vals);
+void batch_split(const Launch& l, const BatchMember* M, BatchBlocks pb, int key_bits, const uint64_t* keys,
+                 const uint32_t* vals);
+void batch_cells(const Launch& l, const BatchMember* M, BatchBlocks pb);
+void batch_permute(const Launch& l, const BatchMember* M, BatchBlocks pb);
+void batch_pack(const Launch& l, const BatchMember* M, BatchBlocks pb, bool mixed);
+void batch_lists(const Launch& l, const BatchMember* M, BatchBlocks lb, bool screened, bool plain);
+void batch_pairs(const Launch& l, const BatchMember* M, BatchBlocks qb, bool mixed, bool fuse, const double* lut);
+void batch_finalize_correct(const Launch& l, const BatchMember* M, BatchBlocks pb);
+size_t sort_pairs64_temp_bytes(size_t n, int key_bits);
+hipError_t sort_pairs64(const Launch& l, void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+                        const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits);
 
 // Grady-Kipp flaws of one body on the device (osph_set_flaws): flaw p (1-based) hits particle
 // flaw_target(seed, p, nb); the first pass records first[i] = min p (first initialised to ~0), cover

@@ -196,8 +196,7 @@ __device__ void block_reduce_ord(unsigned long long (&v)[NV], const bool (&is_mi
     }
 }
 
-__global__ void __launch_bounds__(SMALL_WORDS) k_init_small(unsigned long long* a) {
-    const int t = threadIdx.x;
+__device__ __forceinline__ void init_small_at(unsigned long long* a, int t) {
     bool is_min;
     if (t >= SMALL_CLASSES && t < SMALL_CLASSES + CLASS_STRIDE * MAX_CLASSES) {
         const int k = (t - SMALL_CLASSES) % CLASS_STRIDE;
@@ -211,6 +210,8 @@ __global__ void __launch_bounds__(SMALL_WORDS) k_init_small(unsigned long long* 
     }
     a[t] = is_min ? ~0ull : 0ull;
 }
+
+__global__ void __launch_bounds__(SMALL_WORDS) k_init_small(unsigned long long* a) { init_small_at(a, threadIdx.x); }
 
 // acc[0..2] = min x, y, z; acc[3..5] = max x, y, z; acc[6] = max h; acc[7] = min h (order-encoded).
 __global__ void __launch_bounds__(BLOCK) k_bounds(DevState s, unsigned long long* acc, size_t n) {
@@ -248,13 +249,17 @@ __device__ __forceinline__ void cell_of(const GridClass& g, double x, double y, 
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cell_keys(DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ uint32_t cell_key_of(const DevState& s, const Grid& g, size_t i) {
     const int cls = h_class(s.d[H][i], g.hmin, g.nclass);
     int c[3];
     cell_of(g.c[cls], s.d[X][i], s.d[Y][i], s.d[Z][i], c);
-    keys[i] = g.c[cls].base + row_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]), g.c[cls].bits);
+    return g.c[cls].base + row_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]), g.c[cls].bits);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cell_keys(DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = cell_key_of(s, g, i);
     idx[i] = uint32_t(i);
 }
 
@@ -298,10 +303,8 @@ __device__ __forceinline__ bool row_of(const Grid& g, size_t q, int& cls, uint32
     return false;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_row_range(const uint32_t* __restrict__ keys, Grid g,
-                                                     uint32_t* __restrict__ range, size_t rows, uint32_t n) {
-    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q >= rows) return;
+__device__ __forceinline__ void row_range_at(const uint32_t* __restrict__ keys, const Grid& g,
+                                             uint32_t* __restrict__ range, size_t q, uint32_t n) {
     int cls;
     uint32_t cy, cz;
     if (!row_of(g, q, cls, cy, cz)) return;
@@ -311,11 +314,16 @@ __global__ void __launch_bounds__(BLOCK) k_row_range(const uint32_t* __restrict_
     range[2 * q + 1] = first_at_least(keys, g, row + (1u << G.bits), 0u, n);
 }
 
+__global__ void __launch_bounds__(BLOCK) k_row_range(const uint32_t* __restrict__ keys, Grid g,
+                                                     uint32_t* __restrict__ range, size_t rows, uint32_t n) {
+    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < rows) row_range_at(keys, g, range, q, n);
+}
+
 // One thread per cell x in [0, dim_x] of a row: the binary search runs inside the row's own range.
-__global__ void __launch_bounds__(BLOCK) k_cell_first(const uint32_t* __restrict__ keys, Grid g,
-                                                      const uint32_t* __restrict__ range, uint32_t* __restrict__ first,
-                                                      size_t rows, int xslots) {
-    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cell_first_at(const uint32_t* __restrict__ keys, const Grid& g,
+                                              const uint32_t* __restrict__ range, uint32_t* __restrict__ first,
+                                              size_t t, size_t rows, int xslots) {
     const size_t q = t / size_t(xslots);
     const uint32_t x = uint32_t(t % size_t(xslots));
     if (q >= rows) return;
@@ -328,19 +336,23 @@ __global__ void __launch_bounds__(BLOCK) k_cell_first(const uint32_t* __restrict
     first[cell] = first_at_least(keys, g, cell, range[2 * q], range[2 * q + 1]);
 }
 
+__global__ void __launch_bounds__(BLOCK) k_cell_first(const uint32_t* __restrict__ keys, Grid g,
+                                                      const uint32_t* __restrict__ range, uint32_t* __restrict__ first,
+                                                      size_t rows, int xslots) {
+    cell_first_at(keys, g, range, first, size_t(blockIdx.x) * blockDim.x + threadIdx.x, rows, xslots);
+}
+
 // Pack the neighbour-side data of particle order[k] into rec[k].
 // Origin-relative FP32 position of the screened neighbour test (one rounding of the FP64 offset).
 __device__ __forceinline__ float4 screen_pos(const Screen& sc, double x, double y, double z, double h) {
     return make_float4(float(x - sc.o[0]), float(y - sc.o[1]), float(z - sc.o[2]), float(h));
 }
 
-__global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const uint32_t* __restrict__ order,
-                                                PairRec* __restrict__ rec, double4* __restrict__ pos,
-                                                float4* __restrict__ posf, Screen sc, uint32_t* __restrict__ group,
-                                                size_t n) {
+__device__ __forceinline__ void pack_at(const DevState& s, const Params& prm, const uint32_t* __restrict__ order,
+                                        PairRec* __restrict__ rec, double4* __restrict__ pos,
+                                        float4* __restrict__ posf, const Screen& sc, uint32_t* __restrict__ group,
+                                        size_t k) {
 #pragma clang fp contract(off)
-    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
     const uint32_t i = order[k];
     PairRec r;
     r.x = s.d[X][i];
@@ -377,6 +389,14 @@ __global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const ui
     group[k] = gid;
 }
 
+__global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const uint32_t* __restrict__ order,
+                                                PairRec* __restrict__ rec, double4* __restrict__ pos,
+                                                float4* __restrict__ posf, Screen sc, uint32_t* __restrict__ group,
+                                                size_t n) {
+    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    if (k < n) pack_at(s, prm, order, rec, pos, posf, sc, group, k);
+}
+
 // The pair test of SymmetricSolver<3>::loop (core/sph/solvers/SymmetricSolver.cpp:126-131) evaluated
 // bit for bit: getSqrLength sums the squares left to right without contraction and
 // sqr(2 * 0.5 * (h_i + h_j)) == (h_i + h_j)^2 exactly.
@@ -404,13 +424,10 @@ constexpr uint32_t LIST_BATCH = SCREEN ? 4u : 2u;
 // the host from the domain extent and h_min), so d2 < R2 (1 - band) and d2 > R2 (1 + band) decide the
 // exact test; only candidates inside the band are re-tested in FP64, and the pair set is unchanged.
 template <bool SCREEN>
-__global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restrict__ pos,
-                                                          const float4* __restrict__ posf, Screen sc, Grid g,
-                                                          const uint32_t* __restrict__ first,
-                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ count,
-                                                          uint32_t kmax, uint32_t* overflow, size_t n) {
-    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ __forceinline__ void neighbor_list_at(const double4* __restrict__ pos, const float4* __restrict__ posf,
+                                                 const Screen& sc, const Grid& g, const uint32_t* __restrict__ first,
+                                                 uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                                                 uint32_t kmax, uint32_t* overflow, size_t k) {
     const double4 me = pos[k];
     const float4 mf = SCREEN ? posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     const double p[3] = { me.x, me.y, me.z };
@@ -541,6 +558,16 @@ __global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restr
     if (cnt > kmax) atomicOr(overflow, 1u);
 }
 
+template <bool SCREEN>
+__global__ void __launch_bounds__(BLOCK) k_neighbor_lists(const double4* __restrict__ pos,
+                                                          const float4* __restrict__ posf, Screen sc, Grid g,
+                                                          const uint32_t* __restrict__ first,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                                                          uint32_t kmax, uint32_t* overflow, size_t n) {
+    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    if (k < n) neighbor_list_at<SCREEN>(pos, posf, sc, g, first, list, count, kmax, overflow, k);
+}
+
 // LutKernel<3>::gradImpl (core/sph/kernel/Kernel.h:129-144): linear interpolation in q^2.
 __device__ __forceinline__ double lut_grad(const double* __restrict__ grads, double q2) {
     if (!(q2 < 4.0)) return 0.0;  // outside the support (also catches NaN, which must not become an index)
@@ -576,8 +603,7 @@ struct PairLane {
     size_t k;
     uint32_t sub;
     bool valid;
-    __device__ __forceinline__ PairLane(size_t n) {
-        const size_t t = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    __device__ __forceinline__ PairLane(size_t t, size_t n) {
         k = t / LPP;
         sub = uint32_t(t % LPP);
         valid = k < n;
@@ -736,14 +762,12 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
 // LPP lanes share one particle: each sums its share of the list, then the partial sums are added across
 // the group (fewer particles in flight per XCD, so their neighbour records stay in its L2).
 template <int LPP, bool FUSED>
-__global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, const PairRec* __restrict__ rec,
-                                                      const uint32_t* __restrict__ group,
-                                                      const uint32_t* __restrict__ order,
-                                                      const uint32_t* __restrict__ list,
-                                                      const uint32_t* __restrict__ count,
-                                                      const double* __restrict__ lut, uint32_t kmax, size_t n,
-                                                      FuseArgs fa) {
-    const PairLane<LPP> ln(n);
+__device__ __forceinline__ void pairs_fp64_body(size_t t, DevState s, const Params& prm,
+                                                const PairRec* __restrict__ rec, const uint32_t* __restrict__ group,
+                                                const uint32_t* __restrict__ order, const uint32_t* __restrict__ list,
+                                                const uint32_t* __restrict__ count, const double* __restrict__ lut,
+                                                uint32_t kmax, size_t n, const FuseArgs& fa) {
+    const PairLane<LPP> ln(t, n);
     const size_t k = ln.k;
     const PairRec me = rec[k];
     const uint32_t gi = group[k];
@@ -847,17 +871,26 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
     if (FUSED && !*fa.overflow) reduce_criteria(cv, fa.mins);
 }
 
+template <int LPP, bool FUSED>
+__global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, const PairRec* __restrict__ rec,
+                                                      const uint32_t* __restrict__ group,
+                                                      const uint32_t* __restrict__ order,
+                                                      const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count,
+                                                      const double* __restrict__ lut, uint32_t kmax, size_t n,
+                                                      FuseArgs fa) {
+    pairs_fp64_body<LPP, FUSED>(size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x, s, prm, rec,
+                                group, order, list, count, lut, kmax, n, fa);
+}
+
 // ---- mixed precision ---------------------------------------------------------------------------------
 
 // Packs the mixed-precision neighbour record of particle order[k] (and the exact FP64 position used by
 // the neighbour-list build): position and velocity as 10-bit lattice cells plus FP32 offsets inside them.
-__global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, Lattice L,
-                                                      const uint32_t* __restrict__ order, PairRecF* __restrict__ rec,
-                                                      double4* __restrict__ pos, float4* __restrict__ posf,
-                                                      Screen sc, uint32_t* __restrict__ group,
-                                                      size_t n) {
-    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ __forceinline__ void pack_mixed_at(const DevState& s, const Params& prm, const Lattice& L,
+                                              const uint32_t* __restrict__ order, PairRecF* __restrict__ rec,
+                                              double4* __restrict__ pos, float4* __restrict__ posf, const Screen& sc,
+                                              uint32_t* __restrict__ group, size_t k) {
     const uint32_t i = order[k];
     const double x[3] = { s.d[X][i], s.d[Y][i], s.d[Z][i] };
     const double h = s.d[H][i];
@@ -915,6 +948,15 @@ __global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, La
     group[k] = gid;
 }
 
+__global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, Lattice L,
+                                                      const uint32_t* __restrict__ order, PairRecF* __restrict__ rec,
+                                                      double4* __restrict__ pos, float4* __restrict__ posf,
+                                                      Screen sc, uint32_t* __restrict__ group,
+                                                      size_t n) {
+    const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    if (k < n) pack_mixed_at(s, prm, L, order, rec, pos, posf, sc, group, k);
+}
+
 // M4 gradient factor grad W / q of CubicSpline<3> (core/sph/kernel/Kernel.h:172-187) in closed form,
 // FP32: for q < 1 the bracket divided by q simplifies to (2.25 q - 3) / pi (finite at q = 0, where the
 // reference returns -3 / pi); for 1 <= q < 2 it is -0.75 (2 - q)^2 / (pi q).  The square root and the
@@ -941,13 +983,11 @@ struct Acc {
 // it keeps FP32 precision relative to the cell edge S, whatever the magnitudes of the absolute values.
 // Divisions are hardware reciprocals (about 1 ulp) times the numerator.
 template <int LPP, bool FUSED>
-__global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
-                                                       const PairRecF* __restrict__ rec,
-                                                       const uint32_t* __restrict__ order,
-                                                       const uint32_t* __restrict__ list,
-                                                       const uint32_t* __restrict__ count, uint32_t kmax, size_t n,
-                                                       FuseArgs fa) {
-    const PairLane<LPP> ln(n);
+__device__ __forceinline__ void pairs_mixed_body(size_t t, DevState s, const Params& prm, const Lattice& L,
+                                                 const PairRecF* __restrict__ rec, const uint32_t* __restrict__ order,
+                                                 const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                                                 uint32_t kmax, size_t n, const FuseArgs& fa) {
+    const PairLane<LPP> ln(t, n);
     const size_t k = ln.k;
     const PairRecF me = rec[k];
     const uint32_t gi = me.group;
@@ -1054,6 +1094,17 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, L
         }
     }
     if (FUSED && !*fa.overflow) reduce_criteria(cv, fa.mins);
+}
+
+template <int LPP, bool FUSED>
+__global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
+                                                       const PairRecF* __restrict__ rec,
+                                                       const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ count, uint32_t kmax, size_t n,
+                                                       FuseArgs fa) {
+    pairs_mixed_body<LPP, FUSED>(size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x, s, prm, L, rec,
+                                 order, list, count, kmax, n, fa);
 }
 
 // ScalarGradyKippModel::integrate (core/physics/Damage.cpp:128-170) for particle i.
@@ -1296,11 +1347,10 @@ __device__ void correct_at(DevState& s, const osph_material& m, const Params& pr
 // finalize, the corrections, then the criteria of the corrected state, reduced into mins[0..3].  When
 // the neighbour lists overflowed (*overflow != 0) the kernel changes nothing: the step is repeated with
 // larger lists from the same predicted state.
-__global__ void __launch_bounds__(BLOCK) k_finalize_correct(DevState s, const osph_material* __restrict__ mats,
-                                                            Params prm, double dt, unsigned long long* mins,
-                                                            const uint32_t* overflow, size_t n) {
+__device__ __forceinline__ void finalize_correct_body(size_t i, DevState s, const osph_material* __restrict__ mats,
+                                                      const Params& prm, double dt, unsigned long long* mins,
+                                                      const uint32_t* overflow, size_t n) {
     if (*overflow) return;  // uniform over the grid
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     double v[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
     if (i < n) {
         const osph_material m = mats[s.u[MAT][i]];
@@ -1309,6 +1359,12 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_correct(DevState s, const os
         criteria_values(s, m, prm, i, v);
     }
     reduce_criteria(v, mins);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_finalize_correct(DevState s, const osph_material* __restrict__ mats,
+                                                            Params prm, double dt, unsigned long long* mins,
+                                                            const uint32_t* overflow, size_t n) {
+    finalize_correct_body(size_t(blockIdx.x) * blockDim.x + threadIdx.x, s, mats, prm, dt, mins, overflow, n);
 }
 
 // Per-block partial grid bounds.  Values v = CLASS_VALUES * c + k for class c (k: 0..2 min x, y, z;
@@ -1340,7 +1396,7 @@ __device__ __forceinline__ unsigned long long bound_combine(int v, unsigned long
 // block must call it.
 template <bool VEL>
 __device__ void block_bounds(int cls, double x, double y, double z, double h, const double* v,
-                             unsigned long long* __restrict__ part, uint32_t nblocks) {
+                             unsigned long long* __restrict__ part, uint32_t nblocks, uint32_t block) {
     __shared__ unsigned long long red[BLOCK / 64][BOUND_VALUES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long b[4] = { ord_bits(x), ord_bits(y), ord_bits(z), ord_bits(h) };
@@ -1384,7 +1440,7 @@ __device__ void block_bounds(int cls, double x, double y, double z, double h, co
         const int vi = threadIdx.x;
         unsigned long long acc = red[0][vi];
         for (int w = 1; w < int(blockDim.x / 64); ++w) acc = bound_combine(vi, acc, red[w][vi]);
-        part[size_t(vi) * nblocks + blockIdx.x] = acc;
+        part[size_t(vi) * nblocks + block] = acc;
     }
 }
 
@@ -1392,10 +1448,10 @@ __device__ void block_bounds(int cls, double x, double y, double z, double h, co
 // MODE 1: the material initialize only; MODE 0: neither.  Every mode ends with the per-block partial
 // grid bounds of the resulting positions (class of h relative to href) and, with VEL, velocities.
 template <int MODE, bool VEL>
-__global__ void __launch_bounds__(BLOCK) k_advance(DevState s, const osph_material* __restrict__ mats, Params prm,
-                                                   double dt, double href, unsigned long long* __restrict__ part,
-                                                   size_t n) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void advance_body(uint32_t bid, uint32_t nb, DevState s,
+                                             const osph_material* __restrict__ mats, const Params& prm, double dt,
+                                             double href, unsigned long long* __restrict__ part, size_t n) {
+    const size_t i = size_t(bid) * BLOCK + threadIdx.x;
     int cls = -1;
     double x = 0.0, y = 0.0, z = 0.0, h = 0.0, v[3] = { 0.0, 0.0, 0.0 };
     if (i < n) {
@@ -1415,16 +1471,22 @@ __global__ void __launch_bounds__(BLOCK) k_advance(DevState s, const osph_materi
             v[2] = s.d[VZ][i];
         }
     }
-    block_bounds<VEL>(cls, x, y, z, h, v, part, gridDim.x);
+    block_bounds<VEL>(cls, x, y, z, h, v, part, nb, bid);
+}
+
+template <int MODE, bool VEL>
+__global__ void __launch_bounds__(BLOCK) k_advance(DevState s, const osph_material* __restrict__ mats, Params prm,
+                                                   double dt, double href, unsigned long long* __restrict__ part,
+                                                   size_t n) {
+    advance_body<MODE, VEL>(blockIdx.x, gridDim.x, s, mats, prm, dt, href, part, n);
 }
 
 // Second level of the bounds: block v reduces value v over the nblocks partials into acc (the class
 // values, at CLASS_STRIDE * c + k) or vel.
-__global__ void __launch_bounds__(BLOCK) k_bounds_final(const unsigned long long* __restrict__ part, uint32_t nblocks,
-                                                        unsigned long long* __restrict__ acc,
-                                                        unsigned long long* __restrict__ vel) {
+__device__ __forceinline__ void bounds_final_at(int vi, const unsigned long long* __restrict__ part,
+                                                uint32_t nblocks, unsigned long long* __restrict__ acc,
+                                                unsigned long long* __restrict__ vel) {
     __shared__ unsigned long long red[BLOCK / 64];
-    const int vi = blockIdx.x;
     unsigned long long r = bound_identity(vi);
     for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) r = bound_combine(vi, r, part[size_t(vi) * nblocks + b]);
     for (int o = 32; o > 0; o >>= 1) r = bound_combine(vi, r, __shfl_xor(r, o, 64));
@@ -1439,6 +1501,12 @@ __global__ void __launch_bounds__(BLOCK) k_bounds_final(const unsigned long long
             vel[vi - CLASS_VALUES * MAX_CLASSES] = r;
         }
     }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_bounds_final(const unsigned long long* __restrict__ part, uint32_t nblocks,
+                                                        unsigned long long* __restrict__ acc,
+                                                        unsigned long long* __restrict__ vel) {
+    bounds_final_at(blockIdx.x, part, nblocks, acc, vel);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_criteria(DevState s, const osph_material* __restrict__ mats, Params prm,
@@ -1492,12 +1560,9 @@ __global__ void __launch_bounds__(BLOCK) k_planes_to_stage(unsigned char* __rest
 
 // Moves the state into cell order: slot k of dst receives slot order[k] of src, and the slot <-> storage
 // maps follow (perm_out[k] = storage index of slot k, slot_of[storage index] = k).
-__global__ void __launch_bounds__(BLOCK) k_permute_state(DevState src, DevState dst, const uint32_t* __restrict__ order,
-                                                         const uint32_t* __restrict__ perm_in,
-                                                         uint32_t* __restrict__ perm_out,
-                                                         uint32_t* __restrict__ slot_of, size_t n) {
-    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ __forceinline__ void permute_at(const DevState& src, const DevState& dst, const uint32_t* __restrict__ order,
+                                           const uint32_t* __restrict__ perm_in, uint32_t* __restrict__ perm_out,
+                                           uint32_t* __restrict__ slot_of, size_t k) {
     const uint32_t i = order[k];
 #pragma unroll 4
     for (int p = 0; p < NDPLANES; ++p) dst.d[p][k] = src.d[p][i];
@@ -1506,6 +1571,14 @@ __global__ void __launch_bounds__(BLOCK) k_permute_state(DevState src, DevState 
     const uint32_t g = perm_in[i];
     perm_out[k] = g;
     slot_of[g] = uint32_t(k);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_permute_state(DevState src, DevState dst, const uint32_t* __restrict__ order,
+                                                         const uint32_t* __restrict__ perm_in,
+                                                         uint32_t* __restrict__ perm_out,
+                                                         uint32_t* __restrict__ slot_of, size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) permute_at(src, dst, order, perm_in, perm_out, slot_of, k);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_iota_u32(uint32_t* p, size_t n) {
@@ -1628,6 +1701,7 @@ void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* 
 }
 
 size_t bounds_partials(size_t n) { return size_t(grid_for(n)) * BOUND_VALUES; }
+size_t bounds_values() { return BOUND_VALUES; }
 
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
                     double href, unsigned long long* acc, unsigned long long* vel, unsigned long long* part,
@@ -1774,6 +1848,234 @@ void launch_permute_state(const Launch& l, DevState src, DevState dst, const uin
 void launch_iota(const Launch& l, uint32_t* p, size_t n) {
     if (!n) return;
     k_iota_u32<<<grid_for(n), BLOCK, 0, l.stream>>>(p, n);
+}
+
+// ---- batched stages: one launch for every member of an ensemble ------------------------------------------
+// Block b runs the single-run kernel's block bid = b - first[m] of the member m = member[b], out of the
+// member's nb = first[m + 1] - first[m] blocks, on that member's buffers and parameters.
+
+namespace {
+
+__device__ __forceinline__ const BatchMember& batch_block(const BatchMember* __restrict__ M, const BatchBlocks& t,
+                                                         uint32_t& m, uint32_t& bid, uint32_t& nb) {
+    m = t.member[blockIdx.x];
+    const uint32_t f = t.first[m];
+    bid = blockIdx.x - f;
+    nb = t.first[m + 1] - f;
+    return M[m];
+}
+
+__device__ __forceinline__ uint32_t* overflow_of(const BatchMember& a) {
+    return reinterpret_cast<uint32_t*>(a.small + SMALL_OVERFLOW);
+}
+
+__global__ void __launch_bounds__(SMALL_WORDS) kb_init_small(const BatchMember* __restrict__ M) {
+    init_small_at(M[blockIdx.x].small, threadIdx.x);
+}
+
+template <bool VEL>
+__global__ void __launch_bounds__(BLOCK) kb_advance(const BatchMember* __restrict__ M, BatchBlocks pb) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, pb, m, bid, nb);
+    advance_body<2, VEL>(bid, nb, a.s, a.mats, a.prm, a.dt, a.href, a.part, a.n);
+}
+
+__global__ void __launch_bounds__(BLOCK) kb_bounds_final(const BatchMember* __restrict__ M, BatchBlocks pb, int nv) {
+    const uint32_t m = blockIdx.x / uint32_t(nv);
+    const int vi = int(blockIdx.x % uint32_t(nv));
+    const BatchMember& a = M[m];
+    bounds_final_at(vi, a.part, pb.first[m + 1] - pb.first[m], a.small + SMALL_CLASSES, a.small + SMALL_VEL);
+}
+
+// The sort key of a batch holds the member above the member's own cell key, so that one sort groups the
+// particles by member and, within a member, orders them as the member's own sort does (stable, by slot).
+__global__ void __launch_bounds__(BLOCK) kb_cell_keys(const BatchMember* __restrict__ M, BatchBlocks pb, int key_bits,
+                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, pb, m, bid, nb);
+    const size_t i = size_t(bid) * BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t at = a.key_off + uint32_t(i);
+    keys[at] = (uint64_t(m) << key_bits) | uint64_t(cell_key_of(a.s, a.grid, i));
+    vals[at] = at;
+}
+
+// The member's part of the sorted arrays: its cell keys and its slot order.
+__global__ void __launch_bounds__(BLOCK) kb_split(const BatchMember* __restrict__ M, BatchBlocks pb, int key_bits,
+                                                  const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, pb, m, bid, nb);
+    const size_t j = size_t(bid) * BLOCK + threadIdx.x;
+    if (j >= a.n) return;
+    const uint32_t at = a.key_off + uint32_t(j);
+    a.keys_sorted[j] = uint32_t(keys[at] & ((uint64_t(1) << key_bits) - 1));
+    a.order[j] = vals[at] - a.key_off;
+}
+
+__global__ void __launch_bounds__(BLOCK) kb_row_range(const BatchMember* __restrict__ M, BatchBlocks pb) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, pb, m, bid, nb);
+    for (size_t q = size_t(bid) * BLOCK + threadIdx.x; q < a.rows; q += size_t(nb) * BLOCK) {
+        row_range_at(a.keys_sorted, a.grid, a.row_range, q, a.n);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) kb_cell_first(const BatchMember* __restrict__ M, BatchBlocks pb) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, pb, m, bid, nb);
+    const size_t total = size_t(a.rows) * size_t(a.xslots);
+    for (size_t t = size_t(bid) * BLOCK + threadIdx.x; t < total; t += size_t(nb) * BLOCK) {
+        cell_first_at(a.keys_sorted, a.grid, a.row_range, a.cell_first, t, a.rows, a.xslots);
+    }
+}
+
+// Members whose step reorders: the state into cell order (s -> s2), and the order becomes the identity
+// (each thread reads its own order entry before it overwrites it).
+__global__ void __launch_bounds__(BLOCK) kb_permute(const BatchMember* __restrict__ M, BatchBlocks pb) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, pb, m, bid, nb);
+    if (!a.reorder) return;
+    const size_t k = size_t(bid) * BLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    permute_at(a.s, a.s2, a.order, a.perm, a.perm2, a.slot_of, k);
+    a.order[k] = uint32_t(k);
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(BLOCK) kb_pack(const BatchMember* __restrict__ M, BatchBlocks pb) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, pb, m, bid, nb);
+    const size_t k = size_t(xcd_block(bid, nb)) * BLOCK + threadIdx.x;
+    if (k >= a.n) return;
+    if constexpr (MIXED) {
+        pack_mixed_at(a.s2, a.prm, a.lat, a.order, static_cast<PairRecF*>(a.rec), a.pos, a.posf, a.screen, a.group, k);
+    } else {
+        pack_at(a.s2, a.prm, a.order, static_cast<PairRec*>(a.rec), a.pos, a.posf, a.screen, a.group, k);
+    }
+}
+
+template <bool SCREEN>
+__global__ void __launch_bounds__(BLOCK) kb_lists(const BatchMember* __restrict__ M, BatchBlocks lb) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, lb, m, bid, nb);
+    if ((a.screen.on != 0) != SCREEN) return;
+    const size_t k = size_t(xcd_block(bid, nb)) * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    neighbor_list_at<SCREEN>(a.pos, a.posf, a.screen, a.grid, a.cell_first, a.list, a.count, a.kmax, overflow_of(a), k);
+}
+
+template <bool MIXED, int LPP, bool FUSED>
+__global__ void __launch_bounds__(BLOCK) kb_pairs(const BatchMember* __restrict__ M, BatchBlocks qb,
+                                                  const double* __restrict__ lut) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, qb, m, bid, nb);
+    const size_t t = size_t(xcd_block(bid, nb)) * blockDim.x + threadIdx.x;
+    const FuseArgs fa{ a.mats, a.dt, a.small + SMALL_CRITERIA, overflow_of(a) };
+    if constexpr (MIXED) {
+        pairs_mixed_body<LPP, FUSED>(t, a.s2, a.prm, a.lat, static_cast<const PairRecF*>(a.rec), a.order, a.list,
+                                     a.count, a.kmax, a.n, fa);
+    } else {
+        pairs_fp64_body<LPP, FUSED>(t, a.s2, a.prm, static_cast<const PairRec*>(a.rec), a.group, a.order, a.list,
+                                    a.count, lut, a.kmax, a.n, fa);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) kb_finalize_correct(const BatchMember* __restrict__ M, BatchBlocks pb) {
+    uint32_t m, bid, nb;
+    const BatchMember& a = batch_block(M, pb, m, bid, nb);
+    finalize_correct_body(size_t(bid) * BLOCK + threadIdx.x, a.s2, a.mats, a.prm, a.dt, a.small + SMALL_CRITERIA,
+                          overflow_of(a), a.n);
+}
+
+}  // namespace
+
+void batch_advance(const Launch& l, const BatchMember* M, int count, BatchBlocks pb, bool vel) {
+    if (count <= 0) return;
+    kb_init_small<<<count, SMALL_WORDS, 0, l.stream>>>(M);
+    if (vel) {
+        kb_advance<true><<<pb.total, BLOCK, 0, l.stream>>>(M, pb);
+    } else {
+        kb_advance<false><<<pb.total, BLOCK, 0, l.stream>>>(M, pb);
+    }
+    const int nv = vel ? BOUND_VALUES : CLASS_VALUES * MAX_CLASSES;
+    kb_bounds_final<<<unsigned(count) * unsigned(nv), BLOCK, 0, l.stream>>>(M, pb, nv);
+}
+
+void batch_cell_keys(const Launch& l, const BatchMember* M, BatchBlocks pb, int key_bits, uint64_t* keys,
+                     uint32_t* vals) {
+    kb_cell_keys<<<pb.total, BLOCK, 0, l.stream>>>(M, pb, key_bits, keys, vals);
+}
+
+void batch_split(const Launch& l, const BatchMember* M, BatchBlocks pb, int key_bits, const uint64_t* keys,
+                 const uint32_t* vals) {
+    kb_split<<<pb.total, BLOCK, 0, l.stream>>>(M, pb, key_bits, keys, vals);
+}
+
+void batch_cells(const Launch& l, const BatchMember* M, BatchBlocks pb) {
+    kb_row_range<<<pb.total, BLOCK, 0, l.stream>>>(M, pb);
+    kb_cell_first<<<pb.total, BLOCK, 0, l.stream>>>(M, pb);
+}
+
+void batch_permute(const Launch& l, const BatchMember* M, BatchBlocks pb) {
+    kb_permute<<<pb.total, BLOCK, 0, l.stream>>>(M, pb);
+}
+
+void batch_pack(const Launch& l, const BatchMember* M, BatchBlocks pb, bool mixed) {
+    if (mixed) {
+        kb_pack<true><<<pb.total, BLOCK, 0, l.stream>>>(M, pb);
+    } else {
+        kb_pack<false><<<pb.total, BLOCK, 0, l.stream>>>(M, pb);
+    }
+}
+
+void batch_lists(const Launch& l, const BatchMember* M, BatchBlocks lb, bool screened, bool plain) {
+    const int b = l.tune.lists_block;
+    const size_t lds = size_t(l.tune.lists_lds);
+    if (screened) kb_lists<true><<<lb.total, b, lds, l.stream>>>(M, lb);
+    if (plain) kb_lists<false><<<lb.total, b, lds, l.stream>>>(M, lb);
+}
+
+void batch_pairs(const Launch& l, const BatchMember* M, BatchBlocks qb, bool mixed, bool fuse, const double* lut) {
+    const int b = l.tune.pairs_block, lpp = l.tune.pairs_lanes;
+    const size_t lds = size_t(l.tune.pairs_lds);
+#define OSPH_BPAIRS(MX, P, F) kb_pairs<MX, P, F><<<qb.total, b, lds, l.stream>>>(M, qb, lut)
+    if (mixed) {
+        if (lpp == 4) {
+            if (fuse) OSPH_BPAIRS(true, 4, true); else OSPH_BPAIRS(true, 4, false);
+        } else if (lpp == 2) {
+            if (fuse) OSPH_BPAIRS(true, 2, true); else OSPH_BPAIRS(true, 2, false);
+        } else {
+            if (fuse) OSPH_BPAIRS(true, 1, true); else OSPH_BPAIRS(true, 1, false);
+        }
+    } else {
+        if (lpp == 4) {
+            if (fuse) OSPH_BPAIRS(false, 4, true); else OSPH_BPAIRS(false, 4, false);
+        } else if (lpp == 2) {
+            if (fuse) OSPH_BPAIRS(false, 2, true); else OSPH_BPAIRS(false, 2, false);
+        } else {
+            if (fuse) OSPH_BPAIRS(false, 1, true); else OSPH_BPAIRS(false, 1, false);
+        }
+    }
+#undef OSPH_BPAIRS
+}
+
+void batch_finalize_correct(const Launch& l, const BatchMember* M, BatchBlocks pb) {
+    kb_finalize_correct<<<pb.total, BLOCK, 0, l.stream>>>(M, pb);
+}
+
+// 64-bit keys (member above cell) of the batched sort, rocPRIM's default configuration.
+size_t sort_pairs64_temp_bytes(size_t n, int key_bits) {
+    size_t bytes = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                    (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0u, unsigned(key_bits));
+    return bytes;
+}
+
+hipError_t sort_pairs64(const Launch& l, void* temp, size_t temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+                        const uint32_t* vals_in, uint32_t* vals_out, size_t n, int key_bits) {
+    if (!n) return hipSuccess;
+    return rocprim::radix_sort_pairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0u,
+                                     unsigned(key_bits), l.stream);
 }
 
 // Radix sort of the cell keys: rocPRIM's onesweep with launch shapes measured at the 1M impact (24-bit

@@ -2,6 +2,8 @@
 
 * `Oracle`  wraps liboracle.so, the plain-C restatement of the hot path (osph_oracle.c).
 * `RefRun`  wraps _ref/libosph_ref.so, the real OpenSPH core plus ref_driver.cpp (only where it was built).
+* `AdapterRun` wraps _ref/libosph_adapter.so: the OpenSPH-side adapter (opensph_amd/adapter) stepping the
+  reference-built Storage on the GPU, with adapter_driver.cpp.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 """
@@ -15,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libosph_ref.so")
+ADAPTER_SO = os.path.join(HERE, "_ref", "libosph_adapter.so")
 
 # plane indices, in the order of the enums in osph_oracle.h
 _DPLANES = ["X", "Y", "Z", "H", "VX", "VY", "VZ", "VH", "AX", "AY", "AZ", "RHO", "DRHO", "U", "DU",
@@ -273,6 +276,70 @@ class RefRun:
             if cnt <= cap:
                 return pi[:cnt], pj[:cnt]
             cap = cnt
+
+
+class AdapterRun:
+    """GpuPredictorCorrector + GpuSymmetricSolver (the OpenSPH-side adapter) on the Storage the real OpenSPH
+    core builds from a copy of the given state; read() brings the resident state back through the storage."""
+
+    _lib = None
+
+    @classmethod
+    def available(cls):
+        return os.path.exists(ADAPTER_SO)
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            L = C.CDLL(ADAPTER_SO)
+            L.adapter_create.restype = C.c_void_p
+            L.adapter_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t,
+                                         C.c_double, C.c_int]
+            L.adapter_destroy.argtypes = [C.c_void_p]
+            L.adapter_error.restype = C.c_char_p
+            L.adapter_error.argtypes = [C.c_void_p]
+            dp = C.POINTER(C.c_double)
+            L.adapter_step.argtypes = [C.c_void_p, dp, dp, C.POINTER(C.c_int)]
+            L.adapter_integrate.argtypes = [C.c_void_p]
+            L.adapter_read.argtypes = [C.c_void_p, dp, C.POINTER(C.c_uint32), C.c_size_t]
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, st, cfg, precision="fp64", dt0=None):
+        L = self.lib()
+        self.st = st.copy()
+        self.P, self.I = to_planes(st)
+        self.c = make_config(cfg, st.materials)
+        dt = cfg.initial_dt if dt0 is None else dt0
+        self.h = L.adapter_create(C.byref(self.c), _ptr(self.P), _ptr(self.I, C.c_uint32), st.n, dt,
+                                  1 if precision == "mixed" else 0)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib().adapter_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib().adapter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self):
+        dt, nxt, crit = C.c_double(), C.c_double(), C.c_int()
+        self._check(self.lib().adapter_step(self.h, C.byref(dt), C.byref(nxt), C.byref(crit)))
+        return dt.value, nxt.value, crit.value
+
+    def integrate(self):
+        self._check(self.lib().adapter_integrate(self.h))
+        return self.read()
+
+    def read(self):
+        self._check(self.lib().adapter_read(self.h, _ptr(self.P), _ptr(self.I, C.c_uint32), self.st.n))
+        out = self.st.copy()
+        from_planes(out, self.P, self.I)
+        return out
 
 
 class GravityOracle:

@@ -251,6 +251,18 @@ double ref_step(void* h, int* crit) {
     return run->pc->getTimeStep();
 }
 
+void* ref_storage(void* h) {
+    return &static_cast<RefRun*>(h)->storage;
+}
+
+void* ref_settings(void* h) {
+    return &static_cast<RefRun*>(h)->settings;
+}
+
+void* ref_scheduler(void* h) {
+    return &*static_cast<RefRun*>(h)->scheduler;
+}
+
 double ref_get_dt(void* h) {
     return static_cast<RefRun*>(h)->pc->getTimeStep();
 }

@@ -34,6 +34,13 @@ double ref_get_dt(void* h);
 /* Copies the reference Storage back into the plane layout. */
 void ref_read(void* h, double* planes, uint32_t* iplanes, size_t n);
 
+/* The run's Storage, RunSettings and IScheduler (as Sph::SharedPtr<Sph::Storage>*, Sph::RunSettings*,
+ * Sph::IScheduler*), for drivers that step the same reference-built storage with other solvers
+ * (oracle/adapter_driver.cpp). */
+void* ref_storage(void* h);
+void* ref_settings(void* h);
+void* ref_scheduler(void* h);
+
 /* KdTree<KdNode> (leaf 25) findLowerRank(i, 2 h_i) filtered by the SymmetricSolver cutoff; pairs are
  * returned as (min, max) sorted.  Returns the count (writes at most cap). */
 size_t ref_pairs(void* h, uint32_t* pi, uint32_t* pj, size_t cap);
