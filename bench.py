@@ -290,20 +290,15 @@ def run_ensemble(args):
     """Impact ensemble: all members stepped together (osph_step_pc_batch) against one after another."""
     from opensph_amd import gpu, initial
 
-    # set-up: lattices and placement on the host, the Grady-Kipp flaws drawn on the device
+    # set-up on the device: lattices, placement and the Grady-Kipp flaws (osph_setup_spheres, osph_set_flaws)
     t0 = time.perf_counter()
-    ms = initial.impact_ensemble(args.ensemble_n, flaws=False)
-    t_host = time.perf_counter() - t0
-    solvers = [gpu.GpuSolver(st, cfg, precision="mixed") for st, cfg in ms]
-    t_upload = time.perf_counter() - t0 - t_host
-    for k, (s, (st, _)) in enumerate(zip(solvers, ms)):
-        initial.device_flaws(s, st, seed=1000 * k)
+    specs = initial.ensemble_specs(args.ensemble_n)
+    solvers = [initial.device_impact(precision="mixed", flaw_seed=1000 * k, **spec) for k, spec in enumerate(specs)]
     for s in solvers:
         s.sync()
     t_setup = time.perf_counter() - t0
-    total = sum(st.n for st, _ in ms)
-    log(f"ensemble of {len(ms)} members, {total} particles, set up in {t_setup:.1f}s (host {t_host:.1f}s, "
-        f"upload {t_upload:.1f}s, device flaws {t_setup - t_host - t_upload:.2f}s)")
+    total = sum(s.n for s in solvers)
+    log(f"ensemble of {len(solvers)} members, {total} particles, set up on the device in {t_setup:.1f}s")
     for _ in range(args.warmup):
         gpu.step_batch(solvers)
     for s in solvers:
@@ -327,9 +322,8 @@ def run_ensemble(args):
            "value": total * args.steps / batched, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * batched / args.steps, "higher_is_better": True,
            "dtype": "f32+f64", "data": "synthetic",
-           "config": {"workload": "impact ensemble", "members": len(ms), "particles": total, "device": name},
-           "sequential_value": total * args.steps / sequential,
-           "setup_s": {"host_lattices": t_host, "upload": t_upload, "device_flaws": t_setup - t_host - t_upload}}
+           "config": {"workload": "impact ensemble", "members": len(solvers), "particles": total, "device": name},
+           "sequential_value": total * args.steps / sequential, "setup_s": t_setup}
     print(json.dumps(out), flush=True)
 
 

@@ -151,6 +151,52 @@ int osph_step_pc_batch(osph_ctx** ctxs, int count, const double* dt, double* nex
 int osph_set_flaws(osph_ctx* ctx, size_t first, size_t count, double volume, double k_weibull, double m_weibull,
                    double growth_speed, uint64_t seed);
 
+/* ---- initial conditions (InitialConditions + HexagonalPacking + CollisionGeometrySetupJob) ------------- */
+/* One monolithic spherical body: a MonolithicBody job with a SphericalDomain and HexagonalPacking
+ * (BodySettingsId::PARTICLE_COUNT, SMOOTHING_LENGTH_ETA), its material and flag, and the velocity that every
+ * particle of the body starts with. */
+typedef struct {
+    double particle_count;  /* requested count; the lattice inside the sphere decides the actual count */
+    double radius;
+    double center[3];
+    double eta;
+    double velocity[3];
+    int material;           /* index into the context's material table */
+    uint32_t flag;          /* QuantityId::FLAG of the body */
+} osph_sphere_body;
+
+/* CollisionGeometrySettingsId::IMPACT_ANGLE (degrees) and IMPACTOR_OFFSET (in target smoothing lengths). */
+typedef struct {
+    double impact_angle_deg;
+    double impactor_offset_h;
+} osph_collision;
+
+/* Where a body landed in the storage, and its volume sum(m / rho) (what osph_set_flaws takes). */
+typedef struct {
+    size_t first, count;
+    double volume;
+    double h;               /* smoothing length of its particles */
+    double mass;            /* mass of each of its particles */
+} osph_body_info;
+
+/* Builds the bodies on the device, one after another in the storage, and resizes the context to them:
+ *   HexagonalPacking::generate with CENTER in a SphericalDomain (core/sph/initial/Distribution.cpp:126-201,
+ *     lattice order of Box::iterateWithIndices, core/objects/geometry/Box.h:182-195);
+ *   InitialConditions::addMonolithicBody's quantities (core/sph/initial/Initial.cpp:293-333): h = eta x the
+ *     lattice h, masses proportional to h^3 summing to rho0 x the sphere volume, density rho0, zero energy,
+ *     unit stress reduction, the body's flag, material and velocity;
+ *   with collision set and exactly two bodies (target, then impactor), CollisionGeometrySetupJob::evaluate
+ *     (core/run/jobs/ParticleJobs.cpp:532-577): bounding spheres about the centres of mass, the target's
+ *     centre moved to the origin, the impactor's to (d cos phi + offset h_target, d sin phi, 0) with d the
+ *     sum of the bounding radii (the impactor velocity is the body's own velocity field);
+ *   then the density floor of ContinuityEquation::create (core/sph/equations/EquationTerm.cpp:318-336):
+ *     each material's lower density bound becomes max(min_i m_i W(0) / h_i^3, its bound).
+ * Every position is bit-identical to the sequential construction (the centres of mass are left-to-right
+ * sums).  info (optional) receives one osph_body_info per body.  Flaws are drawn afterwards with
+ * osph_set_flaws. */
+int osph_setup_spheres(osph_ctx* ctx, const osph_sphere_body* bodies, int count, const osph_collision* collision,
+                       osph_body_info* info);
+
 /* Replaces GravitySolver<SymmetricSolver<3>> (core/sph/solvers/GravitySolver.cpp:64-91): with a self-gravity
  * context attached (osph_gravity.h), every derivative evaluation of ctx (osph_integrate, osph_step_pc and
  * the batch) first evaluates the self-gravity of the current positions, smoothing lengths and masses

@@ -284,6 +284,109 @@ void GpuPredictorCorrector::stepParticles(IScheduler& UNUSED(scheduler), ISolver
     stats.set(StatisticsId::TIMESTEP_CRITERION, lastCriterion);
 }
 
+// ---- GpuAdjointRun ---------------------------------------------------------------------------------------
+
+GpuAdjointRun::GpuAdjointRun(const RunSettings& settings, const int precision)
+    : settings(settings)
+    , precision(precision) {}
+
+GpuAdjointRun::~GpuAdjointRun() {
+    osph_destroy(ctx);
+}
+
+void GpuAdjointRun::check(const int rc) const {
+    if (rc != OSPH_OK) {
+        throw InvalidSetup(String::fromAscii(ctx ? osph_last_error(ctx) : "no device context"));
+    }
+}
+
+void GpuAdjointRun::forward(const Storage& storage, ArrayView<const Float> dt, const Size checkpointEvery) {
+    if (dt.empty() || checkpointEvery == 0) {
+        throw InvalidSetup("the adjoint run needs time steps and a checkpoint interval");
+    }
+    if (recorded) {
+        check(osph_adjoint_clear(ctx));
+        recorded = false;
+    }
+    if (!ctx) {
+        const osph_config cfg = gpuRunConfig(settings, storage, precision);
+        std::vector<osph_material> mats;
+        for (Size k = 0; k < storage.getMaterialCnt(); ++k) {
+            mats.push_back(gpuMaterial(storage.getMaterial(k).material()));
+        }
+        if (mats.empty()) {
+            throw InvalidSetup("the storage holds no material");
+        }
+        if (osph_create(&ctx, &cfg, mats.data(), int(mats.size())) != OSPH_OK) {
+            ctx = nullptr;
+            throw InvalidSetup("osph_create failed: no usable GPU");
+        }
+        materialCnt = mats.size();
+    }
+    uploadStorage(ctx, storage);
+    particleCnt = storage.getParticleCnt();
+    check(osph_adjoint_forward(ctx, int(dt.size()), &dt[0], int(checkpointEvery)));
+    recorded = true;
+}
+
+void GpuAdjointRun::readState(Storage& storage) const {
+    if (!ctx || storage.getParticleCnt() != particleCnt) {
+        throw InvalidSetup("the storage does not hold the adjoint run's particles");
+    }
+    downloadStorage(ctx, storage);
+}
+
+GpuAdjointRun::Gradients GpuAdjointRun::backward(ArrayView<const Vector> seedR,
+    ArrayView<const Vector> seedV,
+    ArrayView<const Float> seedRho,
+    ArrayView<const Float> seedU) {
+    if (!recorded) {
+        throw InvalidSetup("backward() needs a forward() run first");
+    }
+    const Size n = particleCnt;
+    // the C ABI takes n x 3 vectors; Vector holds 4 lanes (x, y, z, h)
+    auto xyz = [n](ArrayView<const Vector> a, std::vector<double>& out) -> const double* {
+        if (a.empty()) {
+            return nullptr;
+        }
+        if (a.size() != n) {
+            throw InvalidSetup("a seed does not match the particle count");
+        }
+        out.resize(3 * n);
+        for (Size i = 0; i < n; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                out[3 * i + c] = a[i][c];
+            }
+        }
+        return out.data();
+    };
+    auto plain = [n](ArrayView<const Float> a) -> const double* {
+        if (a.empty()) {
+            return nullptr;
+        }
+        if (a.size() != n) {
+            throw InvalidSetup("a seed does not match the particle count");
+        }
+        return &a[0];
+    };
+    std::vector<double> sr, sv, gr(3 * n), gv(3 * n);
+    Gradients g;
+    g.rho.resize(n);
+    g.u.resize(n);
+    g.eos.resize(3 * materialCnt);
+    unsigned clamped = 0;
+    check(osph_adjoint_backward(ctx, xyz(seedR, sr), xyz(seedV, sv), plain(seedRho), plain(seedU), gr.data(),
+        gv.data(), n ? &g.rho[0] : nullptr, n ? &g.u[0] : nullptr, &g.eos[0], &clamped));
+    g.r.resize(n);
+    g.v.resize(n);
+    for (Size i = 0; i < n; ++i) {
+        g.r[i] = Vector(gr[3 * i], gr[3 * i + 1], gr[3 * i + 2]);
+        g.v[i] = Vector(gv[3 * i], gv[3 * i + 1], gv[3 * i + 2]);
+    }
+    g.clamped = clamped;
+    return g;
+}
+
 // ---- GpuGravity ------------------------------------------------------------------------------------------
 
 GpuGravity::GpuGravity(const RunSettings& settings) {

@@ -102,6 +102,46 @@ protected:
     void stepParticles(IScheduler& scheduler, ISolver& solver, Statistics& stats) override;
 };
 
+/// Reverse-mode gradients of a gas-dynamics run on the GPU (PressureForce + StandardAV + ContinuityEquation,
+/// Tillotson, constant h; no reference counterpart: OpenSPH has no derivatives of its runs).  One object holds
+/// the device context from forward() to backward(): forward() uploads the storage and runs the
+/// PredictorCorrector steps (TimeStepping.cpp:286-346) with the given time steps, keeping a checkpoint every
+/// `checkpointEvery` steps; backward() takes dJ/d(final state) and returns dJ/d(initial state) and
+/// dJ/d(Tillotson a, b, A) of every material, replaying each segment from its checkpoint.  Afterwards the
+/// device state is the initial state again, and forward() may run again.
+class GpuAdjointRun : public Noncopyable {
+    osph_ctx* ctx = nullptr;
+    RunSettings settings;
+    int precision;
+    Size particleCnt = 0;
+    Size materialCnt = 0;
+    bool recorded = false;
+
+public:
+    explicit GpuAdjointRun(const RunSettings& settings, int precision = OSPH_PRECISION_FP64);
+    ~GpuAdjointRun();
+
+    /// Runs dt.size() steps from the storage's state (create() ran for every body).
+    void forward(const Storage& storage, ArrayView<const Float> dt, Size checkpointEvery);
+    /// The device state (after forward(): the final state) into the storage.
+    void readState(Storage& storage) const;
+
+    struct Gradients {
+        Array<Vector> r, v;  ///< dJ/d(initial r, v), storage order (h lane 0)
+        Array<Float> rho, u;  ///< dJ/d(initial rho, u)
+        Array<Float> eos;  ///< dJ/d(a, b, A) of material k at 3 k + 0, 1, 2
+        Size clamped = 0;  ///< particle-steps at a range bound of rho or u (their derivative is not taken)
+    };
+    /// Seeds dJ/d(final r, v, rho, u) in storage order; an empty view is a zero seed.
+    Gradients backward(ArrayView<const Vector> seedR,
+        ArrayView<const Vector> seedV,
+        ArrayView<const Float> seedRho,
+        ArrayView<const Float> seedU);
+
+private:
+    void check(int rc) const;
+};
+
 /// BarnesHut / BruteForceGravity on the GPU with the settings Factory::getGravity reads.
 class GpuGravity : public IGravity {
     osph_gravity* g = nullptr;

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "lib", "obj")
 LIB = os.path.join(LIBDIR, "libosph_gpu.so")
-SOURCES = ["osph_kernels.hip", "osph_capi.hip", "osph_gravity.hip", "osph_adjoint.hip"]
+SOURCES = ["osph_kernels.hip", "osph_capi.hip", "osph_gravity.hip", "osph_adjoint.hip", "osph_initial.hip"]
 ARCH = os.environ.get("OSPH_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

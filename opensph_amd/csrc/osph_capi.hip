@@ -1521,6 +1521,214 @@ int osph_set_flaws(osph_ctx* c, size_t first, size_t count, double volume, doubl
     return sync_check(c);
 }
 
+int osph_setup_spheres(osph_ctx* c, const osph_sphere_body* bodies, int count, const osph_collision* collision,
+                       osph_body_info* info) {
+    if (!c) return OSPH_ERR_ARG;
+    if (count < 0 || (count > 0 && !bodies)) return c->fail(OSPH_ERR_ARG, "bodies must be given");
+    constexpr double PI = 3.14159265358979323846264338327950288;  // core/math/MathUtils.h:379
+    // the lattice of each body, as HexagonalPacking::generate lays it out (host: O(n^(1/3)) axis values)
+    struct Plan {
+        double h, dx, dy, dz;
+        std::vector<double> axis[3];
+        HcpLattice P{};
+        size_t block0 = 0, count = 0, first = 0;
+        double h_eta = 0.0, m = 0.0;
+    };
+    std::vector<Plan> plans(static_cast<size_t>(count));
+    size_t blocks = 0, table = 0;
+    for (int b = 0; b < count; ++b) {
+        const osph_sphere_body& B = bodies[b];
+        Plan& p = plans[size_t(b)];
+        if (!(B.particle_count >= 1.0) || !(B.radius > 0.0) || !std::isfinite(B.radius) || !(B.eta > 0.0) ||
+            !std::isfinite(B.center[0] + B.center[1] + B.center[2])) {
+            return c->fail(OSPH_ERR_ARG, "a body needs a positive particle count, radius and eta");
+        }
+        if (B.material < 0 || size_t(B.material) >= c->mats.size()) {
+            return c->fail(OSPH_ERR_ARG, "body material outside the material table");
+        }
+        // Distribution.cpp:131-138: the interparticle distance of the requested density
+        const double volume = 1.3333333333333333333333 * PI * (B.radius * B.radius * B.radius);
+        const double density = B.particle_count / volume;
+        p.h = 1.0 / std::cbrt(density);
+        p.dx = 1.1 * p.h;
+        p.dy = std::sqrt(3.0) * 0.5 * p.dx;
+        p.dz = std::sqrt(6.0) / 3.0 * p.dx;
+        const double step[3] = { p.dx, p.dy, p.dz };
+        size_t pts = 1;
+        for (int a = 0; a < 3; ++a) {
+            // Box(lower + 0.5 step, upper).iterateWithIndices: v = lower; v <= upper; v += step
+            const double upper = B.center[a] + B.radius;
+            double v = (B.center[a] - B.radius) + 0.5 * step[a];
+            for (; v <= upper; v += step[a]) {
+                p.axis[a].push_back(v);
+                if (p.axis[a].size() > (size_t(1) << 20)) return c->fail(OSPH_ERR_ARG, "lattice too large");
+            }
+            pts *= std::max<size_t>(p.axis[a].size(), 1);
+            if (p.axis[a].empty()) pts = 0;
+        }
+        if (pts > (size_t(1) << 33)) return c->fail(OSPH_ERR_ARG, "lattice too large");
+        p.P.nx = p.axis[0].size();
+        p.P.ny = p.axis[1].size();
+        p.P.nz = p.axis[2].size();
+        p.P.points = pts;
+        p.P.delta_x = 0.5 * p.dx;
+        p.P.delta_y = std::sqrt(3.0) / 6.0 * p.dx;
+        p.P.cx = B.center[0];
+        p.P.cy = B.center[1];
+        p.P.cz = B.center[2];
+        p.P.r2 = B.radius * B.radius;
+        p.block0 = blocks;
+        blocks += pts ? hcp_blocks(p.P) : 0;
+        table += p.P.nx + p.P.ny + p.P.nz;
+    }
+    Launch l = L(c);
+    DevBuf<double> axes, sums_dev;
+    DevBuf<uint32_t> bcount;
+    DevBuf<SerialSum> jobs_dev;
+    DevBuf<unsigned long long> far;
+    struct StreamIdle {  // the stream is idle before the buffers above are freed
+        hipStream_t s;
+        ~StreamIdle() { (void)hipStreamSynchronize(s); }
+    } idle{ c->stream };
+    HIPCHK(c, axes.alloc(table));
+    HIPCHK(c, bcount.alloc(blocks));
+    {
+        std::vector<double> host;
+        host.reserve(table);
+        for (Plan& p : plans) {
+            const size_t at = host.size();
+            for (int a = 0; a < 3; ++a) host.insert(host.end(), p.axis[a].begin(), p.axis[a].end());
+            p.P.xs = axes.p + at;
+            p.P.ys = p.P.xs + p.P.nx;
+            p.P.zs = p.P.ys + p.P.ny;
+        }
+        if (table) HIPCHK(c, hipMemcpy(axes.p, host.data(), table * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (const Plan& p : plans) {
+        if (p.P.points) launch_hcp_count(l, p.P, bcount.p + p.block0);
+    }
+    std::vector<uint32_t> bfirst(blocks);
+    if (blocks) {
+        HIPCHK(c, hipMemcpyAsync(bfirst.data(), bcount.p, blocks * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    // each block's first slot: the bodies one after another, each in lattice order
+    size_t total = 0;
+    for (Plan& p : plans) {
+        p.first = total;
+        const size_t nb = p.P.points ? hcp_blocks(p.P) : 0;
+        for (size_t k = 0; k < nb; ++k) {
+            const uint32_t in = bfirst[p.block0 + k];
+            bfirst[p.block0 + k] = uint32_t(total);
+            total += in;
+        }
+        p.count = total - p.first;
+        if (p.count == 0) return c->fail(OSPH_ERR_ARG, "a body holds no lattice point");
+    }
+    if (int e = osph_resize(c, total)) return e;
+    if (blocks) {
+        HIPCHK(c, hipMemcpy(bcount.p, bfirst.data(), blocks * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    for (const Plan& p : plans) {
+        if (p.P.points) launch_hcp_fill(l, p.P, bcount.p + p.block0, c->s, p.h);
+    }
+    // left-to-right sums on the device (one thread each); `jobs` are read back as `sums`
+    HIPCHK(c, jobs_dev.alloc(size_t(3) * count));
+    HIPCHK(c, sums_dev.alloc(size_t(3) * count));
+    std::vector<double> sums(size_t(3) * count);
+    auto serial = [&](const double* weight_plane) -> int {
+        std::vector<SerialSum> jobs;
+        for (const Plan& p : plans) {
+            for (int a = 0; a < 3; ++a) {
+                jobs.push_back({ c->s.d[X + a] + p.first, weight_plane ? weight_plane + p.first : nullptr, p.count });
+            }
+        }
+        HIPCHK(c, hipMemcpyAsync(jobs_dev.p, jobs.data(), jobs.size() * sizeof(SerialSum), hipMemcpyHostToDevice,
+                                 c->stream));
+        launch_serial_sums(l, jobs_dev.p, int(jobs.size()), sums_dev.p);
+        HIPCHK(c, hipMemcpyAsync(sums.data(), sums_dev.p, jobs.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return OSPH_OK;
+    };
+    // CENTER (Distribution.cpp:181-193): the lattice's centre of mass onto the domain centre; then the
+    // quantities of InitialConditions::setQuantities (h = eta h, masses ~ h^3 normalised to rho0 V)
+    if (int e = serial(nullptr)) return e;
+    for (int b = 0; b < count; ++b) {
+        const osph_sphere_body& B = bodies[b];
+        Plan& p = plans[size_t(b)];
+        const osph_material& mat = c->mats[size_t(B.material)];
+        BodyFill f{};
+        f.move = 1;
+        f.fill = 1;
+        for (int a = 0; a < 3; ++a) f.t[a] = B.center[a] - sums[size_t(3 * b + a)] / double(p.count);
+        p.h_eta = p.h * B.eta;
+        const double m0 = p.h_eta * p.h_eta * p.h_eta;
+        double prelim = 0.0;
+        for (size_t k = 0; k < p.count; ++k) prelim += m0;
+        const double total_mass = 1.3333333333333333333333 * PI * (B.radius * B.radius * B.radius) * mat.rho0;
+        p.m = m0 * (total_mass / prelim);
+        f.h = p.h_eta;
+        f.m = p.m;
+        f.rho = mat.rho0;
+        for (int a = 0; a < 3; ++a) f.v[a] = B.velocity[a];
+        f.flag = B.flag;
+        f.mat = uint32_t(B.material);
+        launch_body_finish(l, c->s, p.first, p.count, f);
+    }
+    if (collision && count == 2) {
+        // getBoundingSphere (ParticleJobs.cpp:499-510): centre of mass sum(m r) / sum(m), largest distance
+        if (int e = serial(c->s.d[M])) return e;
+        double ctr[2][3], rad[2];
+        HIPCHK(c, far.alloc(2));
+        launch_clear_u32(l, reinterpret_cast<uint32_t*>(far.p), 4, 0u);
+        for (int b = 0; b < 2; ++b) {
+            const Plan& p = plans[size_t(b)];
+            double msum = 0.0;
+            for (size_t k = 0; k < p.count; ++k) msum += p.m;
+            for (int a = 0; a < 3; ++a) ctr[b][a] = sums[size_t(3 * b + a)] / msum;
+            launch_max_dist2(l, c->s.d[X] + p.first, c->s.d[Y] + p.first, c->s.d[Z] + p.first, p.count, ctr[b],
+                             far.p + b);
+        }
+        unsigned long long d2[2];
+        HIPCHK(c, hipMemcpyAsync(d2, far.p, sizeof(d2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int b = 0; b < 2; ++b) rad[b] = d2[b] ? std::sqrt(decode_ord_host(d2[b])) : 0.0;
+        // the target's centre to the origin; the impactor to the impact angle (ParticleJobs.cpp:545-560)
+        const double dist = rad[0] + rad[1];
+        const double phi = collision->impact_angle_deg * (PI / 180.0);
+        const double x = dist * std::cos(phi) + collision->impactor_offset_h * plans[0].h_eta;
+        const double y = dist * std::sin(phi);
+        BodyFill ft{}, fi{};
+        ft.move = fi.move = 1;
+        for (int a = 0; a < 3; ++a) ft.t[a] = -ctr[0][a];
+        fi.t[0] = -ctr[1][0] + x;
+        fi.t[1] = -ctr[1][1] + y;
+        fi.t[2] = -ctr[1][2] + 0.0;
+        launch_body_finish(l, c->s, plans[0].first, plans[0].count, ft);
+        launch_body_finish(l, c->s, plans[1].first, plans[1].count, fi);
+    }
+    // ContinuityEquation::create: the density floor min_i m_i W(0) / h_i^3 (constant within each body)
+    constexpr double W0 = 1.0 / PI;  // CubicSpline<3>::valueImpl(0) = (0.25 * 2^3 - 1) / pi
+    double limit = double(1.e20f);   // LARGE
+    for (const Plan& p : plans) limit = std::min(limit, p.m * W0 / (p.h_eta * p.h_eta * p.h_eta));
+    if (count > 0) {
+        for (osph_material& m : c->mats) m.rho_lo = std::max(limit, m.rho_lo);
+        HIPCHK(c, hipMemcpyAsync(c->mats_dev, c->mats.data(), c->mats.size() * sizeof(osph_material),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    if (info) {
+        for (int b = 0; b < count; ++b) {
+            const Plan& p = plans[size_t(b)];
+            const double rho0 = c->mats[size_t(bodies[b].material)].rho0;
+            double vol = 0.0;
+            for (size_t k = 0; k < p.count; ++k) vol += p.m / rho0;
+            info[b] = { p.first, p.count, vol, p.h_eta, p.m };
+        }
+    }
+    return sync_check(c);
+}
+
 int osph_attach_gravity(osph_ctx* c, osph_gravity* g) {
     if (!c) return OSPH_ERR_ARG;
     if (g && c->adj.ready) return c->fail(OSPH_ERR_UNSUPPORTED, "the adjoint has no self-gravity");

@@ -34,6 +34,24 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Single IEEE operations rounded to nearest, never fused with a neighbouring one.  HIP's __dadd_rn and
+// __dmul_rn are plain `+` and `*` in the device headers, so after inlining the compiler contracts a
+// product and a sum of them into one FMA (seen as v_fmac_f64 in the gfx950 code): a squared distance
+// summed that way differs in the last bit from the reference's separately rounded one.  The pragma keeps
+// these operations out of any contraction.
+__device__ __forceinline__ double rn_add(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double rn_sub(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ double rn_mul(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }  // MathBasic.h:15-17
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }  // MathBasic.h:20-22
 

@@ -353,6 +353,41 @@ void flaws_finish(const Launch& l, DevState s, const uint32_t* slot_of, size_t b
                   const uint32_t* first, const uint32_t* last, const uint32_t* cnt, double denom, double inv_m,
                   double cg);
 
+// ---- initial conditions (osph_initial.hip) -------------------------------------------------------------
+// The hexagonal lattice of one spherical body: axis values xs[nx], ys[ny], zs[nz] (device), the shifts of
+// odd rows and layers, and the sphere (centre, squared radius) whose points are kept.
+struct HcpLattice {
+    const double *xs, *ys, *zs;
+    size_t nx, ny, nz, points;  // points = nx ny nz
+    double delta_x, delta_y;
+    double cx, cy, cz, r2;
+};
+// Blocks of the lattice kernels (one thread per lattice point).
+unsigned hcp_blocks(const HcpLattice& P);
+// One left-to-right sum: a[0..n) (times w[i] when w is set).
+struct SerialSum {
+    const double* a;
+    const double* w;
+    size_t n;
+};
+// What k_body_finish writes: the translation t (move), and (fill) the per-body constants.
+struct BodyFill {
+    int move, fill;
+    double t[3];
+    double h, m, rho;
+    double v[3];
+    uint32_t flag, mat;
+};
+// block_count[b] = the inside points among lattice indices [256 b, 256 b + 256).
+void launch_hcp_count(const Launch& l, const HcpLattice& P, uint32_t* block_count);
+// The inside points, in lattice order, to slots block_first[b] + rank of the X, Y, Z planes; H = h.
+void launch_hcp_fill(const Launch& l, const HcpLattice& P, const uint32_t* block_first, DevState s, double h);
+void launch_serial_sums(const Launch& l, const SerialSum* jobs, int count, double* out);
+// atomicMax of the order-encoded squared distances |r_i - c|^2 into *out (which starts at 0).
+void launch_max_dist2(const Launch& l, const double* x, const double* y, const double* z, size_t n, const double c[3],
+                      unsigned long long* out);
+void launch_body_finish(const Launch& l, DevState s, size_t first, size_t n, const BodyFill& f);
+
 }  // namespace osph
 
 // The gravity half of GravitySolver::loop for particles in another context's device planes (defined with

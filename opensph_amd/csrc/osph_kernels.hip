@@ -402,10 +402,10 @@ __global__ void __launch_bounds__(BLOCK) k_pack(DevState s, Params prm, const ui
 // sqr(2 * 0.5 * (h_i + h_j)) == (h_i + h_j)^2 exactly.
 __device__ __forceinline__ bool exact_pair(double xi, double yi, double zi, double hi, double xj, double yj, double zj,
                                            double hj) {
-    const double dx = __dsub_rn(xi, xj), dy = __dsub_rn(yi, yj), dz = __dsub_rn(zi, zj);
-    const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-    const double hs = __dadd_rn(hi, hj);
-    return d2 < __dmul_rn(hs, hs);
+    const double dx = rn_sub(xi, xj), dy = rn_sub(yi, yj), dz = rn_sub(zi, zj);
+    const double d2 = rn_add(rn_add(rn_mul(dx, dx), rn_mul(dy, dy)), rn_mul(dz, dz));
+    const double hs = rn_add(hi, hj);
+    return d2 < rn_mul(hs, hs);
 }
 
 // Candidates loaded per batch in the neighbour-list kernel: 4 FP32 positions (16 B each) in the screened

@@ -30,7 +30,7 @@ EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
            "osph_stream_copy", "osph_tune", "osph_integrals", "osph_step_pc_batch",
            "osph_profile_enable", "osph_profile_read", "osph_adjoint_forward", "osph_adjoint_backward",
-           "osph_adjoint_clear", "osph_attach_gravity", "osph_set_flaws"]
+           "osph_adjoint_clear", "osph_attach_gravity", "osph_set_flaws", "osph_setup_spheres"]
 
 
 class Material(C.Structure):
@@ -48,6 +48,20 @@ class Config(C.Structure):
                 ("av_alpha", C.c_double), ("av_beta", C.c_double), ("h_lo", C.c_double), ("h_hi", C.c_double),
                 ("criteria", C.c_int), ("courant", C.c_double), ("deriv_factor", C.c_double),
                 ("div_factor", C.c_double), ("max_dt", C.c_double), ("precision", C.c_int)]
+
+
+class SphereBody(C.Structure):
+    _fields_ = [("particle_count", C.c_double), ("radius", C.c_double), ("center", C.c_double * 3),
+                ("eta", C.c_double), ("velocity", C.c_double * 3), ("material", C.c_int), ("flag", C.c_uint32)]
+
+
+class Collision(C.Structure):
+    _fields_ = [("impact_angle_deg", C.c_double), ("impactor_offset_h", C.c_double)]
+
+
+class BodyInfo(C.Structure):
+    _fields_ = [("first", C.c_size_t), ("count", C.c_size_t), ("volume", C.c_double), ("h", C.c_double),
+                ("mass", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -92,6 +106,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_adjoint_clear.argtypes = [vp]
     L.osph_attach_gravity.argtypes = [vp, vp]
     L.osph_set_flaws.argtypes = [vp, sz, sz, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64]
+    L.osph_setup_spheres.argtypes = [vp, C.POINTER(SphereBody), i, C.POINTER(Collision), C.POINTER(BodyInfo)]
     L.osph_profile_enable.argtypes = [vp, i]
     L.osph_profile_read.argtypes = [vp, C.c_char_p, sz, dp, C.POINTER(C.c_uint64), i, C.POINTER(i)]
     _lib = L
@@ -180,6 +195,46 @@ class GpuSolver:
         self.upload(st)
         if getattr(cfg, "gravity", 0):
             self.attach_gravity_from_config()
+
+    @classmethod
+    def from_bodies(cls, bodies, materials, cfg: RunConfig, precision: str = "fp64", collision=None):
+        """A solver whose state is built on the device by osph_setup_spheres: `bodies` is a list of dicts
+        (particle_count, radius, center, eta, velocity, material, flag), `collision` None or (impact angle
+        in degrees, impactor offset in target h).  Returns (solver, [(first, count, volume, h) per body]);
+        the materials' density floors are updated as the device updated them."""
+        st = ParticleState(0, materials)
+        solver = cls(st, cfg, precision=precision)
+        solver.materials = [dict(m) for m in materials]
+        info = solver.setup_spheres(bodies, collision)
+        return solver, info
+
+    def setup_spheres(self, bodies, collision=None):
+        """osph_setup_spheres on this context (see from_bodies)."""
+        arr = (SphereBody * len(bodies))()
+        for k, b in enumerate(bodies):
+            arr[k].particle_count = float(b["particle_count"])
+            arr[k].radius = float(b["radius"])
+            arr[k].center = (C.c_double * 3)(*[float(x) for x in b.get("center", (0.0, 0.0, 0.0))])
+            arr[k].eta = float(b.get("eta", 1.3))
+            arr[k].velocity = (C.c_double * 3)(*[float(x) for x in b.get("velocity", (0.0, 0.0, 0.0))])
+            arr[k].material = int(b.get("material", 0))
+            arr[k].flag = int(b.get("flag", k))
+        col = None
+        if collision is not None:
+            col = Collision(float(collision[0]), float(collision[1]))
+        info = (BodyInfo * max(len(bodies), 1))()
+        self._check(self.L.osph_setup_spheres(self.ctx, arr, len(bodies), C.byref(col) if col is not None else None,
+                                              info), "osph_setup_spheres")
+        out = [(int(info[k].first), int(info[k].count), float(info[k].volume), float(info[k].h))
+               for k in range(len(bodies))]
+        self.n = sum(c for _, c, _, _ in out)
+        # the density floor of ContinuityEquation::create, as the device applied it to its material table
+        w0 = 1.0 / 3.14159265358979323846264338327950288
+        limit = min((info[k].mass * w0 / (info[k].h * info[k].h * info[k].h) for k in range(len(bodies))),
+                    default=float(np.float32(1e20)))
+        for m in self.materials:
+            m["rho_lo"] = max(limit, m["rho_lo"])
+        return out
 
     def attach_gravity_from_config(self):
         """GravitySolver<SymmetricSolver> with the gravity Factory::getGravity builds from the run settings

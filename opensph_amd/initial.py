@@ -19,6 +19,8 @@ same arrays are fed to every implementation that is compared.
 """
 from __future__ import annotations
 
+import ctypes
+import ctypes.util
 import math
 from typing import List, Tuple
 
@@ -34,6 +36,17 @@ def sphere_volume(radius: float) -> float:
     return 1.3333333333333333333333 * PI * (radius * radius * radius)
 
 
+_libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+_libm.cbrt.restype = ctypes.c_double
+_libm.cbrt.argtypes = [ctypes.c_double]
+
+
+def cbrt(x: float) -> float:
+    """root<3> (core/math/MathUtils.h:104-107): the C library's cbrt, which numpy's vectorised np.cbrt does
+    not always reproduce to the last bit."""
+    return float(_libm.cbrt(float(x)))
+
+
 def _axis_values(lo: float, hi: float, step: float) -> List[float]:
     out = []
     v = lo
@@ -47,7 +60,7 @@ def hcp_sphere(n_requested: int, radius: float, center=(0.0, 0.0, 0.0)) -> np.nd
     """Hexagonal close packing inside a sphere; returns (N, 4) positions with lane 3 = h (before eta)."""
     volume = sphere_volume(radius)
     particle_density = float(n_requested) / volume
-    h = 1.0 / np.cbrt(particle_density)
+    h = 1.0 / cbrt(particle_density)
     dx = 1.1 * h
     dy = math.sqrt(3.0) * 0.5 * dx
     dz = math.sqrt(6.0) / 3.0 * dx
@@ -376,11 +389,10 @@ def nbody_sphere(n: int, radius: float = 1.0e5, mass: float = 1.25e12, h: float 
     return r, np.full(n, mass)
 
 
-def impact_ensemble(count: int, seed: int = 11, n_range=(5e3, 2e5), flaws: bool = True, offset_h: float = 4.0):
-    """Impact ensemble: N ~ exp(U(ln n_lo, ln n_hi)), R_p / R_t ~ U(0.05, 0.2) at equal particle spacing
-    (N_p / N_t = (R_p / R_t)^3, N_p >= 100), angle ~ U(0, 90) deg, speed ~ U(3, 7) km/s, R_t = 50 km,
-    basalt; one seed per member.  flaws=False leaves the Grady-Kipp flaws to the device
-    (device_flaws after the upload).  Returns a list of (state, config)."""
+def ensemble_specs(count: int, seed: int = 11, n_range=(5e3, 2e5), offset_h: float = 4.0) -> List[dict]:
+    """The members of the impact ensemble as impact() arguments: N ~ exp(U(ln n_lo, ln n_hi)),
+    R_p / R_t ~ U(0.05, 0.2) at equal particle spacing (N_p / N_t = (R_p / R_t)^3, N_p >= 100),
+    angle ~ U(0, 90) deg, speed ~ U(3, 7) km/s, R_t = 50 km, basalt; one seed per member."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -388,8 +400,43 @@ def impact_ensemble(count: int, seed: int = 11, n_range=(5e3, 2e5), flaws: bool 
         ratio = rng.uniform(0.05, 0.2)
         n_p = max(100, int(n * ratio ** 3 / (1 + ratio ** 3)))
         n_t = max(100, int(n) - n_p)
-        st, cfg = impact(n_target=n_t, n_projectile=n_p, r_target=50e3, r_projectile=50e3 * ratio,
-                         speed=rng.uniform(3e3, 7e3), angle_deg=rng.uniform(0.0, 90.0),
-                         seed=int(rng.integers(1 << 30)), flaws=flaws, offset_h=offset_h)
-        out.append((st, cfg))
+        out.append(dict(n_target=n_t, n_projectile=n_p, r_target=50e3, r_projectile=50e3 * ratio,
+                        speed=rng.uniform(3e3, 7e3), angle_deg=rng.uniform(0.0, 90.0),
+                        seed=int(rng.integers(1 << 30)), offset_h=offset_h))
     return out
+
+
+def impact_ensemble(count: int, seed: int = 11, n_range=(5e3, 2e5), flaws: bool = True, offset_h: float = 4.0):
+    """Impact ensemble (ensemble_specs) built on the host.  flaws=False leaves the Grady-Kipp flaws to the
+    device (device_flaws after the upload).  Returns a list of (state, config)."""
+    return [impact(flaws=flaws, **spec) for spec in ensemble_specs(count, seed, n_range, offset_h)]
+
+
+def impact_setup(n_target: int = 950_000, n_projectile: int = 50_000, r_target: float = 50e3,
+                 r_projectile: float = 5e3, speed: float = 5e3, angle_deg: float = 45.0, offset_h: float = 4.0,
+                 seed: int = 1234):
+    """impact()'s configuration as the arguments of osph_setup_spheres: (bodies, materials, config,
+    collision).  The device builds the same particles as impact() bit for bit."""
+    mat = basalt(yielding=1, damage=1)
+    mat["young_modulus"] = young_modulus(mat)
+    bodies = [dict(particle_count=n_target, radius=r_target, eta=1.3, material=0, flag=0),
+              dict(particle_count=n_projectile, radius=r_projectile, eta=1.3, velocity=(-speed, 0.0, 0.0),
+                   material=1, flag=1)]
+    cfg = RunConfig(solid_stress=1, adaptive_h=1, has_reduce=1, has_damage=1, criteria=15)
+    return bodies, [mat, dict(mat)], cfg, (angle_deg, offset_h)
+
+
+def device_impact(precision: str = "fp64", flaw_seed: int = 0, **kw):
+    """impact(**kw) built on the device (osph_setup_spheres) and resident in a new GpuSolver, the Grady-Kipp
+    flaws of body b drawn on the device with seed flaw_seed + b (as device_flaws draws them).  Returns the
+    solver."""
+    from .gpu import GpuSolver
+
+    bodies, mats, cfg, col = impact_setup(**kw)
+    solver, info = GpuSolver.from_bodies(bodies, mats, cfg, precision=precision, collision=col)
+    for b, (first, count, volume, _) in enumerate(info):
+        material = solver.materials[b]
+        mu, A, rho0 = material["shear_modulus"], material["A"], material["rho0"]
+        cg = 0.4 * math.sqrt((A + 4.0 / 3.0 * mu) / rho0)
+        solver.set_flaws(first, count, volume, cg, flaw_seed + b)
+    return solver

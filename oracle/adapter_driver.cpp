@@ -5,8 +5,12 @@
 // GpuSymmetricSolver::integrate evaluates it; ref_read copies the storage back.  Compiled against the
 // reference headers where they lie (oracle/Makefile target `adapter`).
 #include "osph_adapter.h"
+
+#include <cstdio>
 #include "ref_driver.h"
 
+#include "io/Output.h"
+#include "objects/Exceptions.h"
 #include "sph/solvers/StandardSets.h"
 
 using namespace Sph;
@@ -91,6 +95,83 @@ int adapter_read(void* h, double* P, uint32_t* I, size_t n) {
         return 0;
     } catch (const std::exception& e) {
         a->err = e.what();
+        return -1;
+    }
+}
+
+/* Checkpoint of a C++ host: GpuSymmetricSolver::sync (the resident state into the storage), then the
+   reference's own BinaryOutput::dump (core/io/Output.cpp:481-595) of that storage to `path`, with the run time
+   and the integrator's current time step in the header. */
+int adapter_save(void* h, const char* path, double run_time) {
+    AdapterRun* a = static_cast<AdapterRun*>(h);
+    try {
+        a->solver->sync(storageOf(a));
+        Statistics stats;
+        stats.set(StatisticsId::RUN_TIME, run_time);
+        stats.set(StatisticsId::TIMESTEP_VALUE, a->pc->getTimeStep());
+        BinaryOutput out{ OutputFile(Path(String::fromUtf8(path))) };
+        if (!out.dump(storageOf(a), stats)) {
+            a->err = "BinaryOutput::dump failed";
+            return -1;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        a->err = e.what();
+        return -1;
+    }
+}
+
+/* GpuAdjointRun (the C++ forward/backward wrapper of the reverse mode) on the Storage OpenSPH builds from the
+   planes: forward with `steps` time steps and a checkpoint every `every` steps, then backward with the seed
+   dJ/d(final v) (n x 3; null: zero) into the gradients of r and v (n x 3), rho and u (n) and the Tillotson
+   parameters (3 per material of the storage: OpenSPH makes one material per body; grad_eos holds eos_cap
+   doubles, and *eos_count receives how many were written).  The final state of the forward run goes to P, I
+   first. */
+int adjoint_run(const orc_config* cfg, const double* P0, const uint32_t* I0, size_t n, int precision, const double* dt,
+    int steps, int every, const double* seed_v, double* P, uint32_t* I, double* grad_r, double* grad_v,
+    double* grad_rho, double* grad_u, double* grad_eos, size_t eos_cap, size_t* eos_count, unsigned* clamped,
+    char* err, size_t err_cap) {
+    void* ref = ref_create(cfg, P0, I0, n, dt[0], 0);
+    try {
+        const RunSettings& settings = *static_cast<RunSettings*>(ref_settings(ref));
+        Storage& storage = **static_cast<SharedPtr<Storage>*>(ref_storage(ref));
+        GpuAdjointRun run(settings, precision);
+        Array<Float> dts;
+        for (int k = 0; k < steps; ++k) {
+            dts.push(dt[k]);
+        }
+        run.forward(storage, dts, Size(every));
+        run.readState(storage);
+        ref_read(ref, P, I, n);
+        Array<Vector> sv;
+        if (seed_v) {
+            sv.resize(n);
+            for (size_t i = 0; i < n; ++i) {
+                sv[i] = Vector(seed_v[3 * i], seed_v[3 * i + 1], seed_v[3 * i + 2]);
+            }
+        }
+        const GpuAdjointRun::Gradients g = run.backward(nullptr, sv, nullptr, nullptr);
+        for (size_t i = 0; i < n; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                grad_r[3 * i + c] = g.r[i][c];
+                grad_v[3 * i + c] = g.v[i][c];
+            }
+            grad_rho[i] = g.rho[i];
+            grad_u[i] = g.u[i];
+        }
+        if (g.eos.size() > eos_cap) {
+            throw InvalidSetup("the EoS gradient buffer is smaller than 3 x the storage's materials");
+        }
+        for (Size k = 0; k < g.eos.size(); ++k) {
+            grad_eos[k] = g.eos[k];
+        }
+        *eos_count = g.eos.size();
+        *clamped = unsigned(g.clamped);
+        ref_destroy(ref);
+        return 0;
+    } catch (const std::exception& e) {
+        std::snprintf(err, err_cap, "%s", e.what());
+        ref_destroy(ref);
         return -1;
     }
 }

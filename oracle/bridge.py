@@ -302,6 +302,10 @@ class AdapterRun:
             L.adapter_step.argtypes = [C.c_void_p, dp, dp, C.POINTER(C.c_int)]
             L.adapter_integrate.argtypes = [C.c_void_p]
             L.adapter_read.argtypes = [C.c_void_p, dp, C.POINTER(C.c_uint32), C.c_size_t]
+            L.adapter_save.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+            L.adjoint_run.argtypes = [C.POINTER(Config), dp, C.POINTER(C.c_uint32), C.c_size_t, C.c_int, dp, C.c_int,
+                                      C.c_int, dp, dp, C.POINTER(C.c_uint32), dp, dp, dp, dp, dp, C.c_size_t,
+                                      C.POINTER(C.c_size_t), C.POINTER(C.c_uint), C.c_char_p, C.c_size_t]
             cls._lib = L
         return cls._lib
 
@@ -334,6 +338,41 @@ class AdapterRun:
     def integrate(self):
         self._check(self.lib().adapter_integrate(self.h))
         return self.read()
+
+    @classmethod
+    def adjoint(cls, st, cfg, dts, every, seed_v=None, precision="fp64"):
+        """GpuAdjointRun (the adapter's C++ forward/backward wrapper) on the Storage OpenSPH builds from st:
+        returns (final state, gradients dict like GpuSolver.adjoint_backward).  The storage holds one material
+        per body (run of equal flags), so out["eos"] has one row per body."""
+        L = cls.lib()
+        P0, I0 = to_planes(st)
+        P, I = to_planes(st)
+        c = make_config(cfg, st.materials)
+        n = st.n
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        sv = None if seed_v is None else np.ascontiguousarray(seed_v, dtype=np.float64)
+        flag = np.asarray(st["flag"])
+        nbodies = 1 + int(np.count_nonzero(np.diff(flag))) if n else 0
+        eos = np.zeros(3 * max(nbodies, len(st.materials)))
+        out = {"r": np.zeros((n, 3)), "v": np.zeros((n, 3)), "rho": np.zeros(n), "u": np.zeros(n)}
+        clamped = C.c_uint()
+        count = C.c_size_t()
+        err = C.create_string_buffer(512)
+        rc = L.adjoint_run(C.byref(c), _ptr(P0), _ptr(I0, C.c_uint32), n, 1 if precision == "mixed" else 0,
+                           _ptr(dts), int(dts.size), int(every), None if sv is None else _ptr(sv), _ptr(P),
+                           _ptr(I, C.c_uint32), _ptr(out["r"]), _ptr(out["v"]), _ptr(out["rho"]), _ptr(out["u"]),
+                           _ptr(eos), eos.size, C.byref(count), C.byref(clamped), err, 512)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        out["eos"] = eos[:count.value].reshape(-1, 3)
+        fin = st.copy()
+        from_planes(fin, P, I)
+        out["clamped"] = int(clamped.value)
+        return fin, out
+
+    def save(self, path, run_time=0.0):
+        """The resident state through the storage into the reference's BinaryOutput (an .ssf file)."""
+        self._check(self.lib().adapter_save(self.h, str(path).encode(), float(run_time)))
 
     def read(self):
         self._check(self.lib().adapter_read(self.h, _ptr(self.P), _ptr(self.I, C.c_uint32), self.st.n))

@@ -91,3 +91,28 @@ def test_quantity_ids_follow_header():
     body = text[text.index("OSPH_Q_POSITION"):text.index("OSPH_Q_COUNT")]
     names = re.findall(r"OSPH_Q_(\w+)", body)
     assert {k: i for i, k in enumerate(names)} == gpu.Q
+
+
+def test_setup_struct_layouts_match_header():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "osph_gpu.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(osph_sphere_body), offsetof(osph_sphere_body, velocity),
+           offsetof(osph_sphere_body, material), offsetof(osph_sphere_body, flag), sizeof(osph_collision),
+           sizeof(osph_body_info), offsetof(osph_body_info, volume), offsetof(osph_body_info, mass));
+    return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    got = [int(x) for x in out]
+    want = [C.sizeof(gpu.SphereBody), gpu.SphereBody.velocity.offset, gpu.SphereBody.material.offset,
+            gpu.SphereBody.flag.offset, C.sizeof(gpu.Collision), C.sizeof(gpu.BodyInfo), gpu.BodyInfo.volume.offset,
+            gpu.BodyInfo.mass.offset]
+    assert got == want

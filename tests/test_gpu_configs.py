@@ -155,6 +155,17 @@ def test_ensemble_members_match_oracle(oracle):
                     a, b = a[:, :3], b[:, :3]
                 if prec == "mixed" and f == "damage":
                     continue  # a flaw's activation is a threshold decision, which FP32 rounding may move
+                if f == "damage":
+                    # a flaw activates where strain / eps_min passes 1 (ScalarGradyKippModel::integrate,
+                    # Damage.cpp:128-170): a particle at that threshold in the 0.03 s first step and one
+                    # that crosses it in the 0.003 s second step end 10x apart.  The device's pow, cbrt
+                    # and acos round within an ulp of the C library's, which can move such a particle
+                    # across; so can a 1-ulp change of the input h in the oracle itself (member 2 of this
+                    # draw: particle 5426, damage 1.7e-3 against 1.5e-4).  At most one particle in a
+                    # thousand may be such a case; every other particle is held at the parity bar.
+                    off = np.abs(a - b) > tol * max(float(np.abs(b).max()), 1e-300)
+                    assert off.sum() <= max(1, st.n // 1000), (prec, k, int(off.sum()))
+                    a, b = a[~off], b[~off]
                 # mixed: positions, velocities and densities integrate FP32-rounded rates over a few steps
                 # on top of FP64 values (the h lane of v is h / 3 div v, with div v's own error); u, S and
                 # the rates themselves carry the rates' own error
@@ -231,28 +242,36 @@ def test_gas_adjoint_10k_against_central_differences(gas10k, oracle):
 
     # directional derivatives along seeded random directions of the whole initial state: the step is
     # eps times each quantity's largest magnitude.  The forward map is only piecewise smooth: the M4
-    # table's gradient is linear between its nodes in q^2 and StandardAV switches on where
+    # table's gradient is linear between its nodes in q^2 (spacing 1e-4) and StandardAV switches on where
     # (v_i - v_j).(r_i - r_j) changes sign, so a central difference across such a kink differs from the
-    # one-sided derivative the adjoint takes.  Over 50 steps of ~350k pair evaluations each the kinks a
-    # step of 1e-6 crosses add up to a few 1e-5 of the derivative; the bar is 1e-4, and the differences
-    # at eps and 4 eps are both recorded.
+    # one-sided derivative the adjoint takes, by an amount that grows with the number of kinks the step
+    # crosses: linearly in eps, not as eps^2.  At eps = 1e-6 this is a few 1e-5 for v, r and u.  A density
+    # step moves the pressure through the bulk modulus (A / rho0 ~ 1e7 m^2/s^2), and a relative 1e-6 of it
+    # displaces the particles by ~1e-5 h over the run, enough for a large share of the ~350k pair
+    # evaluations per step to cross a table node: the difference is 4e-4 at 1e-6, 1.3e-4 at 1e-7 and 9e-6
+    # at 1e-8 (measured), converging to the adjoint.  The density is therefore checked at eps = 1e-8 and
+    # the convergence from 1e-6 down is asserted; the other quantities at eps = 1e-6.
+    steps = {"v": (EPS,), "r": (EPS,), "rho": (1e-6, 1e-7, 1e-8), "u": (EPS,)}
     for what in ("v", "r", "rho", "u"):
         shape = st[what][:, :3].shape if what in ("r", "v") else st[what].shape
         d = rng.standard_normal(shape)
-        step = EPS * scale[what]
-
-        def perturb(s, sign, what=what, d=d, step=step):
-            if what in ("r", "v"):
-                s[what][:, :3] += sign * step * d
-            else:
-                s[what][:] += sign * step * d
-
-        fd = central(perturb) / step
-        fd4 = central(lambda s, sign, p=perturb: p(s, 4.0 * sign)) / (4.0 * step)
         ad = float(np.sum(out[what] * d))
-        print(f"[result] d/d{what}: adjoint {ad:.9e}, central eps {fd:.9e} ({(fd - ad) / ad:+.2e}), "
-              f"4 eps {fd4:.9e} ({(fd4 - ad) / ad:+.2e})", flush=True)
-        assert abs(fd - ad) <= 1e-4 * abs(ad) + 1e-12 * J0 / scale[what], (what, fd, fd4, ad)
+        errs = []
+        for eps in steps[what]:
+            step = eps * scale[what]
+
+            def perturb(s, sign, what=what, d=d, step=step):
+                if what in ("r", "v"):
+                    s[what][:, :3] += sign * step * d
+                else:
+                    s[what][:] += sign * step * d
+
+            fd = central(perturb) / step
+            errs.append(abs(fd - ad))
+            print(f"[result] d/d{what}: adjoint {ad:.9e}, central eps {eps:.0e} {fd:.9e} ({(fd - ad) / ad:+.2e})",
+                  flush=True)
+        assert errs[-1] <= 1e-4 * abs(ad) + 1e-12 * J0 / scale[what], (what, errs, ad)
+        assert all(b < a for a, b in zip(errs, errs[1:])), (what, errs)  # the differences converge
     # the Tillotson a, b, A at a relative step eps
     for q, name in enumerate(("a", "b", "A")):
         theta = st.materials[0][name]

@@ -21,14 +21,16 @@ def test_adapter_builds_links_and_loads():
     assert r.returncode == 0, r.stderr[-4000:]
     lib = C.CDLL(ADAPTER_SO)
     for name in ("adapter_create", "adapter_destroy", "adapter_step", "adapter_integrate", "adapter_read",
-                 "adapter_error"):
+                 "adapter_error", "adapter_save", "adjoint_run"):
         assert hasattr(lib, name), name
     # the adapter's classes are compiled in, and it resolves the C ABI from libosph_gpu.so
     syms = subprocess.run(["nm", "-D", "-C", ADAPTER_SO], capture_output=True, text=True).stdout
     for cls in ("Sph::GpuSymmetricSolver::integrate", "Sph::GpuPredictorCorrector::stepParticles",
-                "Sph::GpuGravity::evalSelfGravity", "Sph::uploadStorage", "Sph::downloadStorage"):
+                "Sph::GpuGravity::evalSelfGravity", "Sph::uploadStorage", "Sph::downloadStorage",
+                "Sph::GpuAdjointRun::forward", "Sph::GpuAdjointRun::backward"):
         assert cls in syms, cls
-    for abi in ("osph_step_pc", "osph_integrate", "osph_set_quantity", "osph_get_quantity", "osph_gravity_eval_self"):
+    for abi in ("osph_step_pc", "osph_integrate", "osph_set_quantity", "osph_get_quantity", "osph_gravity_eval_self",
+                "osph_adjoint_forward", "osph_adjoint_backward"):
         assert f"U {abi}" in syms, abi
     deps = subprocess.run(["ldd", ADAPTER_SO], capture_output=True, text=True).stdout
     assert "libosph_gpu.so" in deps and "libosph_ref.so" in deps and "not found" not in deps, deps
