@@ -47,6 +47,14 @@ MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (473765 + 97454) * 1024.0,
 PAIR_BYTES_PER_PARTICLE = {"pairs_mixed": 180.0, "pairs_fp64": 232.0,
                            "pairs_fp64_finalize_correct": 232.0 + 680.0 - 92.0}
 PAIR_BYTES_PER_NEIGHBOR = 4.0
+# VALU wave-instructions per dispatch of the pair kernels at the 1M impact after contact (rocprofv3 --pmc
+# SQ_INSTS_VALU, averaged over 40 dispatches), per 64 pair evaluations (the gather form evaluates each
+# unordered pair from both sides): the issue-rate roof counts instructions, not flops.  A CU issues one
+# wave64 VALU instruction per cycle (four SIMDs, four cycles each), so the chip's roof is
+# 256 CUs x 2.4 GHz = 6.14e11 wave-instructions/s (x 64 lanes x 2 = the 78.6 TFLOP/s FMA peak).
+VALU_INSTR_PER_64_PAIR_EVALS = {"pairs_mixed": 1.57e8 / (2 * 35218390 / 64),
+                                "pairs_fp64_finalize_correct": 1.96e8 / (2 * 35218390 / 64)}
+VALU_ISSUE_PEAK = 256 * 2.4e9
 # The same for the Barnes-Hut evaluation of the 8,000,000-particle N-body sphere (FETCH_SIZE x 2 +
 # WRITE_SIZE, KiB per dispatch, averaged over 5 dispatches).
 NBODY_TRAFFIC_BYTES = {"eval_mixed": (2 * 3746826 + 751902) * 1024.0, "eval": (2 * 3631237 + 751959) * 1024.0}
@@ -490,6 +498,13 @@ def main():
     if main_prec == "mixed":
         # FP32 without packed instructions (disabled in this build) issues at the FP64 rate
         roofline["frac_of_unpacked_fp32"] = achieved / FP64_PEAK_TFLOPS
+    if kernel in VALU_INSTR_PER_64_PAIR_EVALS:
+        # the instructions the kernel issues (measured count per pair evaluation x this launch's pairs)
+        # over the launch time, against the chip's VALU issue rate
+        instr = VALU_INSTR_PER_64_PAIR_EVALS[kernel] * 2.0 * pairs / 64.0
+        roofline["valu_issue"] = {"instr_per_64_pair_evals": VALU_INSTR_PER_64_PAIR_EVALS[kernel],
+                                  "instr_per_launch": instr, "peak_instr_per_s": VALU_ISSUE_PEAK,
+                                  "frac": instr / (avg_ms * 1e-3) / VALU_ISSUE_PEAK}
     accuracy = None
     if "fp64" in runs and "mixed" in runs:
         accuracy = compare_precisions(gpu, st, cfg, runs)
