@@ -215,6 +215,14 @@ int osph_sync(osph_ctx* ctx);
  * out = { E, E_kin, E_int, p x y z, L x y z, com x y z, M }, FP64 with a fixed reduction order. */
 enum { OSPH_INTEGRALS = 13 };
 int osph_integrals(osph_ctx* ctx, double out[OSPH_INTEGRALS]);
+/* osph_integrals for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, hosts that forbid hipMalloc).  No reference counterpart.  The same kernels and results as
+ * osph_integrals (bit for bit), with the per-block partial sums in a static device array of 16384 doubles
+ * (128 KiB) that each GPU holds until the process ends, instead of a context buffer allocated on first use.
+ * Calls no allocator.  Rejects a state whose partial sums would not fit (none does: the partials are
+ * capped at 1025 x 12).  Every context on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (from several threads or contexts) race. */
+int osph_integrals_noalloc(osph_ctx* ctx, double out[OSPH_INTEGRALS]);
 
 /* ---- reverse mode (no reference counterpart: OpenSPH has no derivatives of its runs) ---------------- */
 /* Gradients of a function of the final state of a PredictorCorrector run with the gas terms

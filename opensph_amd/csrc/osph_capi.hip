@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -827,6 +828,47 @@ struct DevBuf {
     hipError_t alloc(size_t count) { return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)); }
 };
 
+// Per-block partial sums of osph_integrals_noalloc: 12 per block of the integrals kernels (at most 1024
+// blocks) and their totals, integrals_scratch_doubles(n) <= 1025 * 12 for every n.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = 16384;  // 128 KiB per device
+
+}  // namespace
+
+// The static scratch of osph_integrals_noalloc.  Every GPU the library runs on holds its own copy until the
+// process ends.  In host code this name stands for a host-side shadow, not a device address: kernels take
+// the per-device address that bind_noalloc_scratch looks up (noalloc_scratch[device]).
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mutex;
+double* noalloc_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `device`'s copy of the scratch, looked up once per device with that device
+// current (osph_create calls this for its device).
+hipError_t bind_noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (noalloc_scratch[device]) return hipSuccess;
+    int current = -1;
+    hipError_t e = hipGetDevice(&current);
+    if (e != hipSuccess) return e;
+    if (current != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (current != device) (void)hipSetDevice(current);
+    if (e != hipSuccess) return e;
+    noalloc_scratch[device] = static_cast<double*>(ptr);
+    return hipSuccess;
+}
+
+double* noalloc_scratch_of(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    return noalloc_scratch[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -895,6 +937,7 @@ int osph_create(osph_ctx** out, const osph_config* cfg, const osph_material* mat
         return code;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(OSPH_ERR_HIP);
+    if (bind_noalloc_scratch(c->device) != hipSuccess) return bail(OSPH_ERR_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(OSPH_ERR_HIP);
     if (hipMalloc(&c->mats_dev, sizeof(osph_material) * n_materials) != hipSuccess) return bail(OSPH_ERR_HIP);
     if (hipMemcpy(c->mats_dev, materials, sizeof(osph_material) * n_materials, hipMemcpyHostToDevice) != hipSuccess)
@@ -1103,6 +1146,38 @@ int osph_integrals(osph_ctx* c, double out[OSPH_INTEGRALS]) {
     HIPCHK(c, hipMemcpyAsync(v, c->scratch + need - 12, sizeof(v), hipMemcpyDeviceToHost, c->stream));
     if (int e = sync_check(c)) return e;
     // TotalEnergy = kinetic + internal; CenterOfMass = sum m r / sum m (Integrals.cpp:95-180)
+    out[0] = v[0] + v[1];
+    out[1] = v[0];
+    out[2] = v[1];
+    for (int k = 0; k < 3; ++k) {
+        out[3 + k] = v[2 + k];
+        out[6 + k] = v[5 + k];
+        out[9 + k] = v[8 + k] / v[11];
+    }
+    out[12] = v[11];
+    return OSPH_OK;
+}
+
+int osph_integrals_noalloc(osph_ctx* c, double out[OSPH_INTEGRALS]) {
+    if (!c || !out) return OSPH_ERR_ARG;
+    std::memset(out, 0, OSPH_INTEGRALS * sizeof(double));
+    const size_t n = c->n;
+    if (!n) return OSPH_OK;
+    const size_t need = integrals_scratch_doubles(n);
+    if (need > NOALLOC_SCRATCH_DOUBLES) {
+        return c->fail(OSPH_ERR_ARG, "osph_integrals_noalloc: " + std::to_string(need) +
+                                         " partial sums exceed the static scratch of " +
+                                         std::to_string(NOALLOC_SCRATCH_DOUBLES) + " doubles");
+    }
+    double* buf = noalloc_scratch_of(c->device);
+    if (!buf) return c->fail(OSPH_ERR_STATE, "osph_integrals_noalloc: no scratch bound for this device");
+    // the first kernel writes every partial the second reads: nothing needs to start at zero, and what an
+    // earlier call left in the array is never read
+    launch_integrals(L(c), c->s, buf, n);
+    double v[12];
+    HIPCHK(c, hipMemcpyAsync(v, buf + need - 12, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    if (int e = sync_check(c)) return e;
+    // the combination of osph_integrals (Integrals.cpp:95-180)
     out[0] = v[0] + v[1];
     out[1] = v[0];
     out[2] = v[1];

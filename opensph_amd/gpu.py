@@ -30,7 +30,8 @@ EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info",
            "osph_get_pairs", "osph_integrate", "osph_step_pc", "osph_timestep", "osph_sync", "osph_kernel_lut",
            "osph_stream_copy", "osph_tune", "osph_integrals", "osph_step_pc_batch",
            "osph_profile_enable", "osph_profile_read", "osph_adjoint_forward", "osph_adjoint_backward",
-           "osph_adjoint_clear", "osph_attach_gravity", "osph_set_flaws", "osph_setup_spheres"]
+           "osph_adjoint_clear", "osph_attach_gravity", "osph_set_flaws", "osph_setup_spheres",
+           "osph_integrals_noalloc"]
 
 
 class Material(C.Structure):
@@ -100,6 +101,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_stream_copy.argtypes = [sz, i, dp]
     L.osph_tune.argtypes = [vp, C.c_char_p, C.c_longlong]
     L.osph_integrals.argtypes = [vp, dp]
+    L.osph_integrals_noalloc.argtypes = [vp, dp]
     L.osph_step_pc_batch.argtypes = [C.POINTER(vp), i, dp, dp, C.POINTER(i)]
     L.osph_adjoint_forward.argtypes = [vp, i, dp, i]
     L.osph_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_uint)]
@@ -367,6 +369,14 @@ class GpuSolver:
         """Integrals.cpp diagnostics of the resident state (osph_integrals)."""
         out = np.zeros(13)
         self._check(self.L.osph_integrals(self.ctx, out.ctypes.data_as(C.POINTER(C.c_double))), "osph_integrals")
+        return {"energy": out[0], "kinetic_energy": out[1], "internal_energy": out[2], "momentum": out[3:6],
+                "angular_momentum": out[6:9], "center_of_mass": out[9:12], "mass": out[12]}
+
+    def integrals_noalloc(self) -> Dict[str, object]:
+        """integrals() through osph_integrals_noalloc (partial sums in the library's static scratch)."""
+        out = np.zeros(13)
+        self._check(self.L.osph_integrals_noalloc(self.ctx, out.ctypes.data_as(C.POINTER(C.c_double))),
+                    "osph_integrals_noalloc")
         return {"energy": out[0], "kinetic_energy": out[1], "internal_energy": out[2], "momentum": out[3:6],
                 "angular_momentum": out[6:9], "center_of_mass": out[9:12], "mass": out[12]}
 
