@@ -34,11 +34,11 @@ FLOP_PER_PAIR = 198.0
 # |r|^2 5, 1/sqrt and powers 8, q2 d 15, d q2 d 6, terms 15, monopole 9); counted as in the pair figure.
 FLOP_PER_GRAVITY_PAIR = 22.0
 FLOP_PER_QUADRUPOLE = 61.0
-# HBM bytes per launch of the pair kernel at the default workload and tuning, from two rocprofv3 --pmc
-# passes over this bench (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000: the factors the 1 GiB device
-# copy of --calibrate reads).  None where not measured.
-MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (473765 + 97454) * 1024.0,
-                          "pairs_fp64_finalize_correct": (1170388 + 272513) * 1024.0}
+# HBM bytes per launch of the pair kernel at the default workload and tuning, after contact, from two
+# rocprofv3 --pmc passes over this bench (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000: the factors the
+# 1 GiB device copy of --calibrate reads).  None where not measured.
+MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (2 * 237493 + 97504) * 1024.0,
+                          "pairs_fp64_finalize_correct": (2 * 631505 + 307617) * 1024.0}
 # Algorithmic HBM bytes of one pair-kernel launch per particle: each particle's own neighbour record
 # (80 B mixed, 128 B FP64), its order, count (and group) entries and the 92 B it writes (acceleration,
 # du, divergence, velocity gradient: 11 doubles, and the neighbour count); plus 4 B per listed neighbour.
