@@ -253,6 +253,18 @@ int osph_adjoint_clear(osph_ctx* ctx);
  * finalize, corrector and criteria run in the pair kernel, 0: as a separate pass, -1: fused for FP64 only,
  * the default). */
 int osph_tune(osph_ctx* ctx, const char* key, long long value);
+/* Launch-shape autotuner (no reference counterpart): coordinate descent over osph_tune keys.  keys[k] takes
+ * the n_values[k] candidates values[sum(n_values[<k]) ...], the first of them the starting value (keys = NULL:
+ * pairs_lanes 1/2/4, cell_divisor 3/2/4, lists_block 256/128, pairs_block 256/128, fuse_finalize -1/0/1).
+ * Each candidate is timed over `steps` osph_step_pc calls with time step dt (after one untimed step) on the
+ * resident state, which is restored on the device after every candidate: the tuning leaves the run where it
+ * was.  One key at a time, in order, the fastest value is kept.  The chosen values are applied and written
+ * to chosen[n_keys] (5 for the default space); *best_ms is their time per step; trial_ms[0 ..] (up to
+ * trial_cap) the time of every candidate, *n_trials their number.  Every candidate gives the same
+ * neighbour sets; the lanes per particle change the order of the pair sums. */
+int osph_autotune(osph_ctx* ctx, double dt, int steps, const char* const* keys, const int* n_values,
+                  const long long* values, int n_keys, long long* chosen, double* best_ms, double* trial_ms,
+                  int trial_cap, int* n_trials);
 
 /* ---- kernel table and profiling --------------------------------------------------------------------- */
 /* The 40001-entry M4 look-up tables the device uses (LutKernel<3>, core/sph/kernel/Kernel.h:83-101). */

@@ -5,6 +5,7 @@
 #include "osph_gravity.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1013,8 +1014,11 @@ int osph_resize(osph_ctx* c, size_t n) {
         } else {
             HIPCHK(c, hipMalloc(&c->rec, sizeof(PairRec) * cap));
         }
-        HIPCHK(c, hipMalloc(&c->pos, sizeof(double4) * cap));
-        HIPCHK(c, hipMalloc(&c->posf, sizeof(float4) * cap));
+        HIPCHK(c, hipMalloc(&c->pos, sizeof(double4) * (cap + POS_SLACK)));
+        HIPCHK(c, hipMalloc(&c->posf, sizeof(float4) * (cap + POS_SLACK)));
+        // the slack past the last particle holds defined (zero) values
+        launch_clear_u32(L(c), reinterpret_cast<uint32_t*>(c->pos + cap), 8 * POS_SLACK, 0u);
+        launch_clear_u32(L(c), reinterpret_cast<uint32_t*>(c->posf + cap), 4 * POS_SLACK, 0u);
         HIPCHK(c, hipMalloc(&c->group, sizeof(uint32_t) * cap));
         HIPCHK(c, hipMalloc(&c->count, sizeof(uint32_t) * cap));
         c->sort_temp_bytes = sort_pairs_temp_bytes(cap, 32);
@@ -1696,6 +1700,7 @@ int osph_setup_spheres(osph_ctx* c, const osph_sphere_body* bodies, int count, c
             const uint32_t in = bfirst[p.block0 + k];
             bfirst[p.block0 + k] = uint32_t(total);
             total += in;
+            if (total > 0x7fffffffull) return c->fail(OSPH_ERR_ARG, "the bodies hold more than 2^31-1 particles");
         }
         p.count = total - p.first;
         if (p.count == 0) return c->fail(OSPH_ERR_ARG, "a body holds no lattice point");
@@ -1851,6 +1856,132 @@ int osph_tune(osph_ctx* c, const char* key, long long value) {
     } else {
         return c->fail(OSPH_ERR_ARG, "unknown tuning key or value out of range: " + k);
     }
+    return OSPH_OK;
+}
+
+int osph_autotune(osph_ctx* c, double dt, int steps, const char* const* keys, const int* n_values,
+                  const long long* values, int n_keys, long long* chosen, double* best_ms, double* trial_ms,
+                  int trial_cap, int* n_trials) {
+    if (!c) return OSPH_ERR_ARG;
+    if (!(dt > 0.0) || !std::isfinite(dt) || steps < 1) return c->fail(OSPH_ERR_ARG, "autotune needs dt > 0 and steps >= 1");
+    if (c->adj.ready) return c->fail(OSPH_ERR_STATE, "autotune would change a recorded adjoint run");
+    // the search space: the given keys, or the default one (the first value of each key starts the descent)
+    static const char* const default_keys[] = { "pairs_lanes", "cell_divisor", "lists_block", "pairs_block",
+                                                "fuse_finalize" };
+    static const int default_counts[] = { 3, 3, 2, 2, 3 };
+    static const long long default_values[] = { 1, 2, 4, 3, 2, 4, 256, 128, 256, 128, -1, 0, 1 };
+    if (!keys) {
+        keys = default_keys;
+        n_values = default_counts;
+        values = default_values;
+        n_keys = 5;
+    }
+    if (n_keys < 1 || !n_values || !values) return c->fail(OSPH_ERR_ARG, "autotune needs a search space");
+    std::vector<size_t> first(size_t(n_keys) + 1, 0);
+    for (int k = 0; k < n_keys; ++k) {
+        if (!keys[k] || n_values[k] < 1) return c->fail(OSPH_ERR_ARG, "autotune: every key needs a value");
+        first[size_t(k) + 1] = first[size_t(k)] + size_t(n_values[k]);
+    }
+    const Tuning saved_tune = c->tune;
+    std::vector<long long> best(static_cast<size_t>(n_keys));
+    for (int k = 0; k < n_keys; ++k) {
+        best[size_t(k)] = values[first[size_t(k)]];
+        if (int e = osph_tune(c, keys[k], best[size_t(k)])) {
+            c->tune = saved_tune;
+            return e;
+        }
+    }
+    if (c->n == 0) {
+        if (chosen) std::copy(best.begin(), best.end(), chosen);
+        if (best_ms) *best_ms = 0.0;
+        if (n_trials) *n_trials = 0;
+        return OSPH_OK;
+    }
+    // the resident state (the current block, the slot maps and the search's reference h), restored after every
+    // candidate so that each candidate steps from the same state and the tuning leaves the run where it was
+    DevBuf<double> dsave;
+    DevBuf<uint32_t> usave;
+    const size_t cap = c->cap;
+    HIPCHK(c, dsave.alloc(size_t(NDPLANES) * cap));
+    HIPCHK(c, usave.alloc(size_t(NIPLANES + 2) * cap));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(dsave.p, c->dblock, sizeof(double) * NDPLANES * cap, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(usave.p, c->ublock, sizeof(uint32_t) * NIPLANES * cap, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(usave.p + size_t(NIPLANES) * cap, c->perm, sizeof(uint32_t) * cap, hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(usave.p + size_t(NIPLANES + 1) * cap, c->slot_of, sizeof(uint32_t) * cap,
+                             hipMemcpyDeviceToDevice, c->stream));
+    const int since = c->since_reorder;
+    const double href = c->href;
+    auto restore = [&]() -> int {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->dblock, dsave.p, sizeof(double) * NDPLANES * cap, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->ublock, usave.p, sizeof(uint32_t) * NIPLANES * cap, hipMemcpyDeviceToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->perm, usave.p + size_t(NIPLANES) * cap, sizeof(uint32_t) * cap,
+                                 hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->slot_of, usave.p + size_t(NIPLANES + 1) * cap, sizeof(uint32_t) * cap,
+                                 hipMemcpyDeviceToDevice, c->stream));
+        for (int k = 0; k < NDPLANES; ++k) c->s.d[k] = c->dblock + size_t(k) * cap;
+        for (int k = 0; k < NIPLANES; ++k) c->s.u[k] = c->ublock + size_t(k) * cap;
+        c->since_reorder = since;
+        c->href = href;
+        c->neighbors_valid = false;
+        return sync_check(c);
+    };
+    // mean wall time of `steps` PC steps after one untimed step (which sizes buffers for the settings)
+    auto time_steps = [&](double* ms) -> int {
+        double next = 0.0;
+        int crit = 0;
+        if (int e = osph_step_pc(c, dt, &next, &crit)) return e;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < steps; ++k) {
+            if (int e = osph_step_pc(c, dt, &next, &crit)) return e;
+        }
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        return restore();
+    };
+    int trials = 0;
+    auto record = [&](double ms) {
+        if (trial_ms && trials < trial_cap) trial_ms[trials] = ms;
+        ++trials;
+    };
+    auto apply = [&](const std::vector<long long>& v) -> int {
+        for (int k = 0; k < n_keys; ++k) {
+            if (int e = osph_tune(c, keys[k], v[size_t(k)])) return e;
+        }
+        return OSPH_OK;
+    };
+    auto fail_restore = [&](int e) {
+        const std::string msg = c->err;
+        (void)restore();
+        c->tune = saved_tune;
+        c->err = msg;
+        return e;
+    };
+    double warm = 0.0, best_t = 0.0;
+    // the starting settings once untimed-to-time (the first steps on a state size its buffers), then timed
+    if (int e = time_steps(&warm)) return fail_restore(e);
+    if (int e = time_steps(&best_t)) return fail_restore(e);
+    record(best_t);
+    for (int k = 0; k < n_keys; ++k) {
+        for (size_t j = first[size_t(k)] + 1; j < first[size_t(k) + 1]; ++j) {
+            std::vector<long long> trial = best;
+            trial[size_t(k)] = values[j];
+            if (int e = apply(trial)) return fail_restore(e);
+            double t = 0.0;
+            if (int e = time_steps(&t)) return fail_restore(e);
+            record(t);
+            if (t < best_t) {
+                best = trial;
+                best_t = t;
+            }
+        }
+        if (int e = apply(best)) return fail_restore(e);
+    }
+    if (chosen) std::copy(best.begin(), best.end(), chosen);
+    if (best_ms) *best_ms = best_t;
+    if (n_trials) *n_trials = trials;
     return OSPH_OK;
 }
 

@@ -146,6 +146,10 @@ __host__ __device__ inline uint32_t row_key(uint32_t x, uint32_t y, uint32_t z, 
 // lines when they store their j-th quads, and the pair kernels still read each particle's quads from
 // one small region.  The list holds round_up(n, LIST_TILE) * kmax entries.
 constexpr uint32_t LIST_TILE = 8;
+// The sorted position arrays (double4, float4) hold this many entries past the last particle, so that the
+// list kernel loads whole batches of candidates without clamping each index (entries past a run are
+// loaded and never used).
+constexpr size_t POS_SLACK = 4;
 __host__ __device__ inline size_t list_quad(size_t k, uint32_t j, uint32_t quads) {
     return ((k / LIST_TILE) * quads + j) * LIST_TILE + (k % LIST_TILE);
 }

@@ -432,13 +432,17 @@ __device__ __forceinline__ void neighbor_list_at(const double4* __restrict__ pos
     const float4 mf = SCREEN ? posf[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     const double p[3] = { me.x, me.y, me.z };
     uint4* list4 = reinterpret_cast<uint4*>(list);
-    uint32_t buf[4] = { 0, 0, 0, 0 };
+    // the last four entries, oldest first (shifted in: no register indexed by the count)
+    uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
     uint32_t cnt = 0;
     auto append = [&](uint32_t k2) {
-        buf[cnt & 3u] = k2;
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = k2;
         ++cnt;
         if ((cnt & 3u) == 0 && cnt <= kmax) {
-            list4[list_quad(k, (cnt >> 2) - 1, kmax >> 2)] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+            list4[list_quad(k, (cnt >> 2) - 1, kmax >> 2)] = make_uint4(b0, b1, b2, b3);
         }
     };
     for (int cls = 0; cls < g.nclass; ++cls) {
@@ -481,13 +485,24 @@ __device__ __forceinline__ void neighbor_list_at(const double4* __restrict__ pos
             zkey = spread2(uint32_t(cz)) << 1;
         };
         plane();
+        // the y half of the row key, spread2(cy), stepped along with cy: adding one to the even-bit
+        // interleaved value (the odd bits set so that the carry runs through them) gives spread2(cy + 1)
+        const uint32_t y0key = spread2(uint32_t(lo[1]));
+        uint32_t ykey = y0key;
+        bool started = false;
         // the next row within R after (cy, cz): its first and past-the-end sorted positions
         auto next_row = [&](uint32_t& b, uint32_t& e) -> bool {
             for (;;) {
-                if (++cy > up[1]) {
+                if (!started) {
+                    started = true;
                     cy = lo[1];
+                } else if (++cy > up[1]) {
+                    cy = lo[1];
+                    ykey = y0key;
                     if (++cz > up[2]) return false;
                     plane();
+                } else {
+                    ykey = ((ykey | 0xaaaaaaaau) + 1u) & 0x55555555u;
                 }
                 if (tz2 > R2c) {
                     cy = up[1];
@@ -503,7 +518,7 @@ __device__ __forceinline__ void neighbor_list_at(const double4* __restrict__ pos
                 x0 = x0 < lo[0] ? lo[0] : x0;
                 x1 = x1 > up[0] ? up[0] : x1;
                 if (x0 > x1) continue;
-                const uint32_t row = G.base + ((spread2(uint32_t(cy)) | zkey) << G.bits);
+                const uint32_t row = G.base + ((ykey | zkey) << G.bits);
                 b = first[row + uint32_t(x0)];
                 e = first[row + uint32_t(x1) + 1u];
                 return true;
@@ -515,12 +530,13 @@ __device__ __forceinline__ void neighbor_list_at(const double4* __restrict__ pos
             const uint32_t b = nb, e = ne;
             more = next_row(nb, ne);
             // candidates in batches of LIST_BATCH<SCREEN>: all loads of a batch are issued before the first
-            // test (clamped to the last candidate, so every load stays inside [b, e))
+            // test; a batch may read up to LIST_BATCH - 1 entries past e, which are never tested (e <= n and
+            // the arrays hold POS_SLACK >= LIST_BATCH entries past n)
             for (uint32_t k0 = b; k0 < e; k0 += LIST_BATCH<SCREEN>) {
                 if (SCREEN) {
                     float4 o[LIST_BATCH<true>];
 #pragma unroll
-                    for (uint32_t u = 0; u < LIST_BATCH<true>; ++u) o[u] = posf[k0 + u < e ? k0 + u : e - 1];
+                    for (uint32_t u = 0; u < LIST_BATCH<true>; ++u) o[u] = posf[k0 + u];  // POS_SLACK past n
 #pragma unroll
                     for (uint32_t u = 0; u < LIST_BATCH<true>; ++u) {
                         const uint32_t k2 = k0 + u;
@@ -539,7 +555,7 @@ __device__ __forceinline__ void neighbor_list_at(const double4* __restrict__ pos
                 } else {
                     double4 o[LIST_BATCH<false>];
 #pragma unroll
-                    for (uint32_t u = 0; u < LIST_BATCH<false>; ++u) o[u] = pos[k0 + u < e ? k0 + u : e - 1];
+                    for (uint32_t u = 0; u < LIST_BATCH<false>; ++u) o[u] = pos[k0 + u];  // POS_SLACK past n
 #pragma unroll
                     for (uint32_t u = 0; u < LIST_BATCH<false>; ++u) {
                         const uint32_t k2 = k0 + u;
@@ -552,7 +568,10 @@ __device__ __forceinline__ void neighbor_list_at(const double4* __restrict__ pos
         }
     }
     if ((cnt & 3u) != 0 && cnt < kmax) {
-        list4[list_quad(k, cnt >> 2, kmax >> 2)] = make_uint4(buf[0], buf[1], buf[2], buf[3]);
+        // the r = cnt % 4 entries of the last quad are b(4-r) .. b3; the rest of the quad is not read
+        const uint32_t r = cnt & 3u;
+        const uint4 q = r == 1 ? make_uint4(b3, 0u, 0u, 0u) : (r == 2 ? make_uint4(b2, b3, 0u, 0u) : make_uint4(b1, b2, b3, 0u));
+        list4[list_quad(k, cnt >> 2, kmax >> 2)] = q;
     }
     count[k] = cnt;
     if (cnt > kmax) atomicOr(overflow, 1u);

@@ -31,7 +31,7 @@ EXPORTS = ["osph_create", "osph_destroy", "osph_last_error", "osph_device_info",
            "osph_stream_copy", "osph_tune", "osph_integrals", "osph_step_pc_batch",
            "osph_profile_enable", "osph_profile_read", "osph_adjoint_forward", "osph_adjoint_backward",
            "osph_adjoint_clear", "osph_attach_gravity", "osph_set_flaws", "osph_setup_spheres",
-           "osph_integrals_noalloc"]
+           "osph_integrals_noalloc", "osph_autotune"]
 
 
 class Material(C.Structure):
@@ -100,6 +100,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.osph_kernel_lut.argtypes = [dp, dp]
     L.osph_stream_copy.argtypes = [sz, i, dp]
     L.osph_tune.argtypes = [vp, C.c_char_p, C.c_longlong]
+    L.osph_autotune.argtypes = [vp, C.c_double, i, C.POINTER(C.c_char_p), C.POINTER(C.c_int),
+                                C.POINTER(C.c_longlong), i, C.POINTER(C.c_longlong), dp, dp, i, C.POINTER(i)]
     L.osph_integrals.argtypes = [vp, dp]
     L.osph_integrals_noalloc.argtypes = [vp, dp]
     L.osph_step_pc_batch.argtypes = [C.POINTER(vp), i, dp, dp, C.POINTER(i)]

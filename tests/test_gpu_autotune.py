@@ -1,5 +1,5 @@
-"""The launch-shape autotuner: the chosen settings come from the search space, the resident state is the
-one it found, and a tuned solver steps exactly like a fresh solver given the same settings."""
+"""The launch-shape autotuner (osph_autotune): the chosen settings come from the search space, the resident
+state is the one it found, and the tuned solver steps exactly like an untuned twin given the same settings."""
 import numpy as np
 import pytest
 
@@ -18,6 +18,8 @@ def test_autotune_keeps_state_and_picks_from_space(precision):
     print(f"[case] autotune {precision} n={st.n}", flush=True)
     g = gpu.GpuSolver(st, cfg, precision=precision)
     g.integrate()
+    twin = gpu.GpuSolver(st, cfg, precision=precision)  # the same history, never tuned
+    twin.integrate()
     before, dt0 = g.download(st), g.dt
     res = autotune.autotune(g, steps=2, space=SPACE)
     after = g.download(st)
@@ -31,15 +33,13 @@ def test_autotune_keeps_state_and_picks_from_space(precision):
     assert len(res["trials"]) == 1 + sum(len(v) - 1 for _, v in SPACE)
     assert res["ms_per_step"] > 0.0
 
-    # the tuned solver and a fresh one with the same settings take identical steps
-    fresh = gpu.GpuSolver(before, cfg, precision=precision)
-    fresh.tune(**res["settings"])
-    fresh.dt = dt0
+    # the tuning left the run where it was (state, slot order, search reference): the tuned solver and an
+    # untuned twin with the same history, given the chosen settings, take identical steps
+    twin.tune(**res["settings"])
     for _ in range(3):
-        g.step()
-        fresh.step()
-    a, b = g.download(st), fresh.download(st)
+        assert g.step() == twin.step()
+    a, b = g.download(st), twin.download(st)
     for k in ("r", "v", "rho", "u"):
         assert np.array_equal(a[k], b[k]), k
     g.close()
-    fresh.close()
+    twin.close()

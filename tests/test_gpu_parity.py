@@ -266,6 +266,31 @@ def test_full_size_impact_exact_pairs_and_derivatives(oracle):
 
 
 @pytest.mark.slow
+def test_full_size_impact_pc_steps_match_oracle(oracle):
+    """Two FP64 PredictorCorrector steps of the 1,062,568-particle impact, perturbed so that every term and
+    criterion is active, against the oracle's steps: time steps and criteria equal, the state and its
+    rates within 1e-10 (the damage at the parity bar except where a flaw sits on its activation
+    threshold, at most one particle in a thousand; see test_gpu_configs.py)."""
+    from cases import exercised_impact
+
+    st, cfg = exercised_impact(n_target=950_000, n_projectile=50_000, seed=17)
+    _print_case("impact_1M_pc", st)
+    g = gpu.GpuSolver(st, cfg)
+    log = [g.step() for _ in range(2)]
+    a = g.download(st)
+    b, rlog = oracle.run_pc(st, cfg, 2)
+    for (dt, nx, cr), (odt, onx, ocr) in zip(log, rlog):
+        assert dt == pytest.approx(odt, rel=1e-12) and nx == pytest.approx(onx, rel=1e-10) and cr == ocr
+    for k in ("r", "v", "dv", "rho", "drho", "u", "du", "s", "ds"):
+        x, y = (a[k][:, :3], b[k][:, :3]) if k in ("v", "dv") else (a[k], b[k])
+        assert rel_l2(x, y) <= 1e-10, (k, rel_l2(x, y))
+    off = np.abs(a["damage"] - b["damage"]) > 1e-10 * max(float(np.abs(b["damage"]).max()), 1e-300)
+    assert off.sum() <= st.n // 1000
+    assert rel_l2(a["damage"][~off], b["damage"][~off]) <= 1e-10
+    g.close()
+
+
+@pytest.mark.slow
 def test_full_size_impact_steps():
     """The unperturbed 1M impact: neighbour statistics of the reference probe (29 / 66.3 / 68), first
     step leaves the bodies undeformed (reference Impact test, core/sph/solvers/test/Impact.cpp:41-58),
