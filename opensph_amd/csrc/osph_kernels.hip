@@ -455,74 +455,61 @@ __device__ __forceinline__ void neighbor_list_at(const double4* __restrict__ pos
             if (t > 0.0) d2b += t * t;
         }
         if (d2b > R * R * (1.0 + 1e-6)) continue;
-        const int r = int(ceil(R * G.inv_cell * (1.0 + 1e-9)));
-        int lo[3], up[3];
-        for (int a = 0; a < 3; ++a) {
-            const int c = cell_coord(G, p[a], a);
-            lo[a] = c - r < 0 ? 0 : c - r;
-            up[a] = c + r >= G.dim[a] ? G.dim[a] - 1 : c + r;
-        }
-        if (lo[0] > up[0] || lo[1] > up[1] || lo[2] > up[2]) continue;
         // Rows of cells (cy, cz) whose yz-square lies farther than R hold no neighbour; in the others a
         // neighbour lies within the chord half-width w = sqrt(R^2 - d_yz^2) of the particle along x, and
-        // the cells of that x-run are one contiguous range of sorted particles.  The row geometry is
-        // evaluated in FP32 in cell units with R widened by 1e-3 cells: the FP32 errors (below 1e-4
-        // cells for the at most a few hundred cells per axis) then only ever add candidates, and the
-        // exact test decides.  The ranges of the next row are loaded before the current row's
-        // candidates are tested, so the two loads overlap.
+        // the cells of that x-run are one contiguous range of sorted particles.  The planes cz within R,
+        // the rows cy of a plane within R (half-width sqrt(R^2 - d_z^2) in y) and the x-runs are computed
+        // directly, so every step of the walk yields a row.  The geometry is evaluated in FP32 in cell
+        // units with R widened by 1e-3 cells: the FP32 errors (square roots of about 1 ulp included; below
+        // 1e-4 cells for the at most a few thousand cells per axis) then only ever add candidates, and the
+        // exact test decides.  The ranges of the next row are loaded before the current row's candidates
+        // are tested, so the two loads overlap.
         const float pcx = float((p[0] - G.lo[0]) * G.inv_cell);
         const float pcy = float((p[1] - G.lo[1]) * G.inv_cell);
         const float pcz = float((p[2] - G.lo[2]) * G.inv_cell);
         const float Rc = float(R * G.inv_cell) * (1.0f + 1e-6f) + 1e-3f;
         const float R2c = Rc * Rc;
-        int cy = lo[1] - 1, cz = lo[2];
+        const int zlast = min(G.dim[2] - 1, int(floorf(pcz + Rc)));
+        int cz = max(0, int(floorf(pcz - Rc))) - 1;  // the plane before the first
+        int cy = 0, ylast = -1;                       // the row and the last row of the plane within R
         float tz2 = 0.0f;
-        uint32_t zkey = 0;
-        auto plane = [&]() {
-            const float fz = float(cz);
-            const float tz = fmaxf(fmaxf(fz - pcz, pcz - (fz + 1.0f)), 0.0f);
-            tz2 = tz * tz;
-            zkey = spread2(uint32_t(cz)) << 1;
-        };
-        plane();
-        // the y half of the row key, spread2(cy), stepped along with cy: adding one to the even-bit
-        // interleaved value (the odd bits set so that the carry runs through them) gives spread2(cy + 1)
-        const uint32_t y0key = spread2(uint32_t(lo[1]));
-        uint32_t ykey = y0key;
-        bool started = false;
-        // the next row within R after (cy, cz): its first and past-the-end sorted positions
+        uint32_t zkey = 0, ykey = 0;
+        // the next row within R: its first and past-the-end sorted positions (an empty range where the
+        // x-run misses the grid)
         auto next_row = [&](uint32_t& b, uint32_t& e) -> bool {
-            for (;;) {
-                if (!started) {
-                    started = true;
-                    cy = lo[1];
-                } else if (++cy > up[1]) {
-                    cy = lo[1];
-                    ykey = y0key;
-                    if (++cz > up[2]) return false;
-                    plane();
-                } else {
-                    ykey = ((ykey | 0xaaaaaaaau) + 1u) & 0x55555555u;
+            if (++cy > ylast) {
+                for (;;) {
+                    if (++cz > zlast) return false;
+                    const float fz = float(cz);
+                    const float tz = fmaxf(fmaxf(fz - pcz, pcz - (fz + 1.0f)), 0.0f);
+                    tz2 = tz * tz;
+                    if (tz2 > R2c) continue;
+                    const float wy = __builtin_amdgcn_sqrtf(R2c - tz2);
+                    cy = max(0, int(floorf(pcy - wy)));
+                    ylast = min(G.dim[1] - 1, int(floorf(pcy + wy)));
+                    if (cy > ylast) continue;
+                    zkey = spread2(uint32_t(cz)) << 1;
+                    ykey = spread2(uint32_t(cy));
+                    break;
                 }
-                if (tz2 > R2c) {
-                    cy = up[1];
-                    continue;
-                }
-                const float fy = float(cy);
-                const float ty = fmaxf(fmaxf(fy - pcy, pcy - (fy + 1.0f)), 0.0f);
-                const float dzy = tz2 + ty * ty;
-                if (dzy > R2c) continue;
-                const float w = sqrtf(R2c - dzy);
-                int x0 = int(floorf(pcx - w));
-                int x1 = int(floorf(pcx + w));
-                x0 = x0 < lo[0] ? lo[0] : x0;
-                x1 = x1 > up[0] ? up[0] : x1;
-                if (x0 > x1) continue;
-                const uint32_t row = G.base + ((ykey | zkey) << G.bits);
-                b = first[row + uint32_t(x0)];
-                e = first[row + uint32_t(x1) + 1u];
+            } else {
+                // spread2(cy + 1) from spread2(cy): adding one to the even-bit interleaved value with the odd
+                // bits set carries through them
+                ykey = ((ykey | 0xaaaaaaaau) + 1u) & 0x55555555u;
+            }
+            const float fy = float(cy);
+            const float ty = fmaxf(fmaxf(fy - pcy, pcy - (fy + 1.0f)), 0.0f);
+            const float w = __builtin_amdgcn_sqrtf(fmaxf(R2c - (tz2 + ty * ty), 0.0f));
+            const int x0 = max(0, int(floorf(pcx - w)));
+            const int x1 = min(G.dim[0] - 1, int(floorf(pcx + w)));
+            if (x0 > x1) {
+                b = e = 0u;
                 return true;
             }
+            const uint32_t row = G.base + ((ykey | zkey) << G.bits);
+            b = first[row + uint32_t(x0)];
+            e = first[row + uint32_t(x1) + 1u];
+            return true;
         };
         uint32_t nb = 0, ne = 0;
         bool more = next_row(nb, ne);
