@@ -40,11 +40,11 @@ FLOP_PER_QUADRUPOLE = 61.0
 MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (2 * 237493 + 97504) * 1024.0,
                           "pairs_fp64_finalize_correct": (2 * 631505 + 307617) * 1024.0}
 # Algorithmic HBM bytes of one pair-kernel launch per particle: each particle's own neighbour record
-# (80 B mixed, 128 B FP64), its order, count (and group) entries and the 92 B it writes (acceleration,
+# (64 B mixed, 128 B FP64), its order, count and group entries and the 92 B it writes (acceleration,
 # du, divergence, velocity gradient: 11 doubles, and the neighbour count); plus 4 B per listed neighbour.
 # The fused FP64 kernel adds the finalize + corrector + criteria pass (about 60 doubles read and 25
 # written per particle), less the 92 B it no longer writes for that pass to read back.
-PAIR_BYTES_PER_PARTICLE = {"pairs_mixed": 180.0, "pairs_fp64": 232.0,
+PAIR_BYTES_PER_PARTICLE = {"pairs_mixed": 168.0, "pairs_fp64": 232.0,
                            "pairs_fp64_finalize_correct": 232.0 + 680.0 - 92.0}
 PAIR_BYTES_PER_NEIGHBOR = 4.0
 # VALU wave-instructions per dispatch of the pair kernels at the 1M impact after contact (rocprofv3 --pmc

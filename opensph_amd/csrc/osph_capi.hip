@@ -99,7 +99,7 @@ struct osph_ctx {
     size_t cell_cap = 0;
     PairRec* rec = nullptr;
     PairRecF* recf = nullptr;
-    Lattice lattice{};
+    Fixed fixed{};
     double4* pos = nullptr;
     float4* posf = nullptr;  // origin-relative FP32 positions of the screened list kernel
     uint32_t* group = nullptr;
@@ -493,7 +493,7 @@ int search_begin(osph_ctx* c, int mode, double dt) {
 }
 
 // The plan of a neighbour search from the class (and velocity) bounds in sh (a host copy of the SMALL_*
-// accumulators): the grid, the FP32 screen and, in mixed precision, the lattices; the class reference h
+// accumulators): the grid, the FP32 screen and, in mixed precision, the fixed-point frames; the class reference h
 // of the next search; the cell and row tables sized.  *rows_out = the rows of the grid.  Host only.
 int plan_search(osph_ctx* c, const unsigned long long* sh, size_t* rows_out) {
     const double href = c->href;
@@ -574,32 +574,35 @@ int plan_search(osph_ctx* c, const unsigned long long* sh, size_t* rows_out) {
     if (int e = ensure_rows(c, rows)) return e;
     *rows_out = rows;
     if (c->cfg.precision == OSPH_PRECISION_MIXED) {
-        // lattice of power-of-two edge >= 2 h_min (10 bits per axis) for the FP32 relative positions
-        Lattice lat{};
-        double S = std::exp2(std::ceil(std::log2(2.0 * hmin)));
-        double ext = 0.0;
-        for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
-        while (ext / S > 1000.0) S *= 2.0;
-        for (int k = 0; k < 3; ++k) lat.lo[k] = lo[k];
-        lat.S = S;
-        lat.Sf = float(S);
-        // velocity lattice: power-of-two edge with the velocity range within 1000 cells
-        double vrange = 0.0;
+        // fixed-point frames: the smallest power-of-two steps with the position and velocity boxes within
+        // 2^31 - 2 steps per axis, so that the difference of any two coordinates fits in an int32 (two
+        // neighbours can differ in velocity by the whole range), with steps that stay normal FP32 numbers
+        auto frame_step = [](double extent) {
+            const double span = 2147483646.0;
+            double st = std::exp2(-100.0);
+            if (extent > 0.0) st = std::max(st, std::exp2(std::ceil(std::log2(extent / span))));
+            while (extent / st > span) st *= 2.0;
+            return st;
+        };
+        Fixed fx{};
+        double ext = 0.0, vrange = 0.0;
         for (int k = 0; k < 3; ++k) {
-            lat.vlo[k] = decode_ord(sh[SMALL_VEL + k]);
+            fx.lo[k] = lo[k];
+            ext = std::max(ext, hi[k] - lo[k]);
+            fx.vlo[k] = decode_ord(sh[SMALL_VEL + k]);
             const double vhi = decode_ord(sh[SMALL_VEL + 3 + k]);
-            if (!std::isfinite(lat.vlo[k]) || !std::isfinite(vhi)) {
+            if (!std::isfinite(fx.vlo[k]) || !std::isfinite(vhi)) {
                 return c->fail(OSPH_ERR_STATE, "velocities must be finite");
             }
-            vrange = std::max(vrange, vhi - lat.vlo[k]);
+            vrange = std::max(vrange, vhi - fx.vlo[k]);
         }
-        double Sv = 1.0;
-        if (vrange > 0.0) Sv = std::exp2(std::ceil(std::log2(vrange / 1000.0)));
-        while (vrange / Sv > 1000.0) Sv *= 2.0;
-        Sv = std::max(Sv, std::exp2(-100.0));  // keeps the FP32 copy of the edge normal
-        lat.Sv = Sv;
-        lat.Svf = float(Sv);
-        c->lattice = lat;
+        fx.step = frame_step(ext);
+        fx.inv_step = 1.0 / fx.step;
+        fx.stepf = float(fx.step);
+        fx.vstep = frame_step(vrange);
+        fx.inv_vstep = 1.0 / fx.vstep;
+        fx.vstepf = float(fx.vstep);
+        c->fixed = fx;
     }
     return OSPH_OK;
 }
@@ -669,7 +672,7 @@ int search_finish(osph_ctx* c) {
     }
     if (c->cfg.precision == OSPH_PRECISION_MIXED) {
         timed(c, "pack", [&] {
-            launch_pack_mixed(l, c->s, c->prm, c->lattice, c->order, c->recf, c->pos, c->posf, c->screen, c->group, n);
+            launch_pack_mixed(l, c->s, c->prm, c->fixed, c->order, c->recf, c->pos, c->posf, c->screen, c->group, n);
         });
     } else {
         timed(c, "pack", [&] { launch_pack(l, c->s, c->prm, c->order, c->rec, c->pos, c->posf, c->screen, c->group, n); });
@@ -767,7 +770,7 @@ void launch_pair_sums(osph_ctx* c, const FuseArgs* fuse = nullptr) {
     }
     if (c->cfg.precision == OSPH_PRECISION_MIXED) {
         timed(c, fuse ? "pairs_mixed_finalize_correct" : "pairs_mixed", [&] {
-            launch_pairs_mixed(l, c->s, c->prm, c->lattice, c->recf, c->order, c->list, c->count, c->kmax, n, fuse);
+            launch_pairs_mixed(l, c->s, c->prm, c->fixed, c->recf, c->group, c->order, c->list, c->count, c->kmax, n, fuse);
         });
     } else {
         timed(c, fuse ? "pairs_fp64_finalize_correct" : "pairs_fp64", [&] {
@@ -1403,7 +1406,7 @@ int batch_step(const std::vector<osph_ctx*>& ms, const std::vector<double>& dts,
     HIPCHK(lead, hipMemcpyAsync(P->small_host, P->small, count * SMALL_WORDS * sizeof(unsigned long long),
                                 hipMemcpyDeviceToHost, st));
     if (int e = sync_check(lead)) return e;
-    // each member's grid, screen and lattices, planned as its own search plans them
+    // each member's grid, screen and fixed-point frames, planned as its own search plans them
     int key_bits = 1;
     bool screened = false, plain = false;
     uint32_t off = 0;
@@ -1419,7 +1422,7 @@ int batch_step(const std::vector<osph_ctx*>& ms, const std::vector<double>& dts,
         }
         b.grid = c->grid;
         b.screen = c->screen;
-        b.lat = c->lattice;
+        b.fx = c->fixed;
         b.kmax = c->kmax;
         b.rows = uint32_t(rows);
         b.xslots = xslots;

@@ -42,19 +42,22 @@ struct alignas(16) PairRec {
 };
 static_assert(sizeof(PairRec) == 128, "PairRec must be 128 bytes");
 
-// Mixed-precision neighbour record, 80 B.  Positions are split into an integer lattice cell (10 bits
-// per axis, packed in `lat`) of power-of-two edge S and an FP32 offset inside that cell, so that
-// r_i - r_j = (L_i - L_j) S + (d_i - d_j) keeps FP32 relative precision at any distance from the origin.
-// Velocities get the same split on a second lattice (`vlat`, edge Sv): bodies move at km/s with m/s
-// internal differences, which absolute FP32 velocities would round away.
+// Mixed-precision neighbour record, 64 B (four 16-byte loads per neighbour).  Positions and velocities
+// are 32-bit fixed-point coordinates on power-of-two steps from the lower corner of their bounding boxes
+// (Fixed): the difference of two coordinates is an exact integer, rounded once when it is converted to
+// FP32, so r_i - r_j and v_j - v_i keep FP32 relative precision at any distance from the origin and at any
+// bulk speed (bodies move at km/s with m/s internal differences, which absolute FP32 values would round
+// away).  The four low mantissa bits of cs carry the particle's group code (group_code): cs is rounded to
+// 20 bits (relative 1e-6), and it enters only the mean sound speed of the artificial viscosity.
 struct alignas(16) PairRecF {
-    float dx, dy, dz, h;     // position relative to its lattice cell, smoothing length
-    float ux, uy, uz, m;     // velocity relative to its velocity-lattice cell, mass
-    float rho, cs, pr, sxx;  // pr = p / rho^2, s.. = S / rho^2
+    uint32_t x, y, z;        // position lo + (x, y, z) * step
+    float h;                 // smoothing length
+    uint32_t vx, vy, vz;     // velocity vlo + (vx, vy, vz) * vstep
+    float m;                 // mass
+    float rho, cs, pr, sxx;  // pr = p / rho^2, s.. = S / rho^2; cs with the group code in its low 4 bits
     float syy, sxy, sxz, syz;
-    uint32_t lat, vlat, group, pad;  // lattice cells (10 bits per axis) of the position and the velocity
 };
-static_assert(sizeof(PairRecF) == 80, "PairRecF must be 80 bytes");
+static_assert(sizeof(PairRecF) == 64, "PairRecF must be 64 bytes");
 
 // FP32 screen of the neighbour test: positions relative to the origin o, and the band [lo, hi] = [1 - b,
 // 1 + b] around the squared radius outside which the FP32 comparison equals the exact FP64 one.
@@ -64,15 +67,16 @@ struct Screen {
     int on;
 };
 
-// Power-of-two lattices of the mixed-precision records: positions on cells of edge S from lo, velocities
-// on cells of edge Sv from vlo (10 bits per axis each); the FP32 parts are offsets inside the cell.
-struct Lattice {
+// Fixed-point frames of the mixed-precision records: power-of-two steps such that the bounding box of
+// the positions (of the velocities) spans at most 2^31 - 2 steps per axis, so that the difference of any
+// two coordinates is an exact int32.
+struct Fixed {
     double lo[3];
-    double S;     // power of two
-    float Sf;
+    double step, inv_step;
+    float stepf;
     double vlo[3];
-    double Sv;    // power of two
-    float Svf;
+    double vstep, inv_vstep;
+    float vstepf;
 };
 
 // Particles are binned into smoothing-length classes, class = min(nclass - 1, floor(log2(h / h_min))).
@@ -110,6 +114,13 @@ struct Params {
 
 constexpr int LUT_ENTRIES = 40000;
 constexpr uint32_t NO_GROUP = 0xffffffffu;
+// Group code of the mixed-precision records (4 bits): group g < 14 -> g + 1, NO_GROUP -> 15, and 0 for
+// the larger groups, which the pair kernel compares through the group array.
+constexpr uint32_t GROUP_CODE_NONE = 15u;
+constexpr uint32_t GROUP_CODE_WIDE = 0u;
+__host__ __device__ inline uint32_t group_code(uint32_t g) {
+    return g == NO_GROUP ? GROUP_CODE_NONE : (g < 14u ? g + 1u : GROUP_CODE_WIDE);
+}
 
 __host__ __device__ inline uint32_t morton_spread(uint32_t v) {
     v &= 0x3ffu;
@@ -253,9 +264,9 @@ struct FuseArgs {
 void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* rec, const uint32_t* group,
                        const uint32_t* order, const uint32_t* list, const uint32_t* count, const double* lut_grads,
                        uint32_t kmax, size_t n, const FuseArgs* fuse = nullptr);
-void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
+void launch_pack_mixed(const Launch& l, DevState s, Params prm, Fixed fx, const uint32_t* order, PairRecF* rec,
                        double4* pos, float4* posf, Screen sc, uint32_t* group, size_t n);
-void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
+void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Fixed fx, const PairRecF* rec, const uint32_t* group,
                         const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n,
                         const FuseArgs* fuse = nullptr);
 void launch_finalize(const Launch& l, DevState s, const osph_material* mats, Params prm, size_t n);
@@ -299,7 +310,7 @@ struct BatchMember {
     double dt, href;
     Grid grid;
     Screen screen;
-    Lattice lat;
+    Fixed fx;
     uint32_t n, kmax;
     uint32_t rows;    // rows of the member's grid (all classes)
     int xslots;       // largest dim_x + 1 over its classes

@@ -624,9 +624,12 @@ __device__ void reduce_criteria(double v[4], unsigned long long* mins);
 
 
 // How the pair kernels gather neighbour records (for_each_neighbor): 0 directly, 1 through the LDS stage,
-// 2 through the LDS stage with the next step's loads issued before the current step's compute.
+// 2 through the LDS stage with the next step's loads issued before the current step's compute, 3 directly
+// with the next neighbour's record loaded before the current one's terms.  For the 64-B mixed record at
+// the 1M impact, in one A/B on one box: 0.432 ms (0), 0.417 ms (3); the 80-B record it replaced ran at
+// 0.415 ms directly.
 constexpr int PAIRS64_STAGE = 1;
-constexpr int PAIRSMX_STAGE = 0;
+constexpr int PAIRSMX_STAGE = 3;
 
 // Per-wavefront LDS staging of neighbour records: 64 records of CH 16-byte chunks each, at an odd stride
 // of chunks (CH | 1) so that the lanes' 16-byte reads of their own record spread over all banks.
@@ -648,7 +651,7 @@ struct StageOf {
 // lanes fetch consecutive chunks of one record, so a load instruction touches ~64 / CH records' cache
 // lines instead of one line per lane: CH times fewer line accesses in the texture path, for a longer
 // dependency chain (shuffle, load, LDS write, LDS read).  It pays for the 128-B FP64 record (1.16 ->
-// 1.06 ms at the impact) and not for the 80-B mixed one (0.45 -> 0.51 ms), which gathers directly.
+// 1.06 ms at the impact) and not for the (then 80-B) mixed one (0.45 -> 0.51 ms), which gathers directly.
 // With STAGED == 2 the next step's loads are issued before the current step's compute.
 // Every lane of the wavefront must call this (lanes without entries load their own record as a
 // stand-in, and run no f).
@@ -667,20 +670,36 @@ __device__ __forceinline__ void for_each_neighbor(const Rec* __restrict__ rec, c
     }
     const uint4* list4 = reinterpret_cast<const uint4*>(list);
     auto row = [&](uint32_t j) { return list4[list_quad(k, j, quads)]; };
-    if constexpr (!STAGED) {
+    if constexpr (STAGED == 0 || STAGED == 3) {
         if (myq == 0) return;
         uint4 quad = row(sub);
         uint4 nquad = myq > 1 ? row(sub + LPP) : quad;
-        for (uint32_t t = 0; t < total; ++t) {
+        auto entry_of = [&](uint32_t t) -> uint32_t {
             const uint32_t sel = t & 3u;
             if (sel == 0 && t > 0) {
                 quad = nquad;
                 const uint32_t j = (t >> 2) + 1;
                 nquad = j < myq ? row(sub + j * LPP) : quad;
             }
-            const uint32_t e = sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
-            const Rec o = rec[e];
-            f(o, e);
+            return sel == 0 ? quad.x : (sel == 1 ? quad.y : (sel == 2 ? quad.z : quad.w));
+        };
+        if constexpr (STAGED == 0) {
+            for (uint32_t t = 0; t < total; ++t) {
+                const uint32_t e = entry_of(t);
+                const Rec o = rec[e];
+                f(o, e);
+            }
+        } else {
+            // the next neighbour's record is loaded before the current one's terms are evaluated
+            uint32_t e = entry_of(0);
+            Rec o = rec[e];
+            for (uint32_t t = 0; t < total; ++t) {
+                const uint32_t en = t + 1 < total ? entry_of(t + 1) : e;
+                const Rec on = rec[en];
+                f(o, e);
+                o = on;
+                e = en;
+            }
         }
         return;
     }
@@ -891,9 +910,16 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_fp64(DevState s, Params prm, co
 
 // ---- mixed precision ---------------------------------------------------------------------------------
 
+// Coordinate v on the fixed-point axis from lo with step 1 / inv (a power of two), rounded to nearest.
+__device__ __forceinline__ uint32_t fixed_coord(double v, double lo, double inv) {
+    const double t = floor((v - lo) * inv + 0.5);
+    return uint32_t(t < 0.0 ? 0.0 : (t > 4294967295.0 ? 4294967295.0 : t));
+}
+
 // Packs the mixed-precision neighbour record of particle order[k] (and the exact FP64 position used by
-// the neighbour-list build): position and velocity as 10-bit lattice cells plus FP32 offsets inside them.
-__device__ __forceinline__ void pack_mixed_at(const DevState& s, const Params& prm, const Lattice& L,
+// the neighbour-list build): position and velocity as 32-bit fixed-point coordinates, cs rounded to 20
+// bits with the group code in the low 4.
+__device__ __forceinline__ void pack_mixed_at(const DevState& s, const Params& prm, const Fixed& F,
                                               const uint32_t* __restrict__ order, PairRecF* __restrict__ rec,
                                               double4* __restrict__ pos, float4* __restrict__ posf, const Screen& sc,
                                               uint32_t* __restrict__ group, size_t k) {
@@ -901,35 +927,17 @@ __device__ __forceinline__ void pack_mixed_at(const DevState& s, const Params& p
     const double x[3] = { s.d[X][i], s.d[Y][i], s.d[Z][i] };
     const double h = s.d[H][i];
     PairRecF r;
-    uint32_t c[3];
-    float d[3];
-    for (int a = 0; a < 3; ++a) {
-        double f = floor((x[a] - L.lo[a]) / L.S);
-        f = f < 0.0 ? 0.0 : (f > 1023.0 ? 1023.0 : f);
-        c[a] = uint32_t(f);
-        d[a] = float(x[a] - (L.lo[a] + f * L.S));
-    }
-    const double v[3] = { s.d[VX][i], s.d[VY][i], s.d[VZ][i] };
-    uint32_t cv[3];
-    float u[3];
-    for (int a = 0; a < 3; ++a) {
-        double f = floor((v[a] - L.vlo[a]) / L.Sv);
-        f = f < 0.0 ? 0.0 : (f > 1023.0 ? 1023.0 : f);
-        cv[a] = uint32_t(f);
-        u[a] = float((v[a] - L.vlo[a]) - f * L.Sv);
-    }
-    r.ux = u[0];
-    r.uy = u[1];
-    r.uz = u[2];
-    r.dx = d[0];
-    r.dy = d[1];
-    r.dz = d[2];
+    r.x = fixed_coord(x[0], F.lo[0], F.inv_step);
+    r.y = fixed_coord(x[1], F.lo[1], F.inv_step);
+    r.z = fixed_coord(x[2], F.lo[2], F.inv_step);
+    r.vx = fixed_coord(s.d[VX][i], F.vlo[0], F.inv_vstep);
+    r.vy = fixed_coord(s.d[VY][i], F.vlo[1], F.inv_vstep);
+    r.vz = fixed_coord(s.d[VZ][i], F.vlo[2], F.inv_vstep);
     r.h = float(h);
     r.m = float(s.d[M][i]);
     const double rho = s.d[RHO][i];
     const double rho2 = rho * rho;
     r.rho = float(rho);
-    r.cs = float(s.d[CS][i]);
     r.pr = float(s.d[P][i] / rho2);
     if (prm.solid_stress) {
         r.sxx = float(s.d[SXX][i] / rho2);
@@ -940,27 +948,26 @@ __device__ __forceinline__ void pack_mixed_at(const DevState& s, const Params& p
     } else {
         r.sxx = r.syy = r.sxy = r.sxz = r.syz = 0.f;
     }
-    r.lat = c[0] | (c[1] << 10) | (c[2] << 20);
-    r.vlat = cv[0] | (cv[1] << 10) | (cv[2] << 20);
     uint32_t gid = 0;
     if (prm.sum_only_undamaged && prm.has_reduce) {
         gid = (s.d[RED][i] == 0.0) ? NO_GROUP : s.u[FLAG][i];
     }
-    r.group = gid;
-    r.pad = 0u;
+    // cs >= 0: rounded to a multiple of 16 units in the last place (ties up), then the code
+    const uint32_t cb = __float_as_uint(float(s.d[CS][i]));
+    r.cs = __uint_as_float(((cb + 8u) & ~15u) | group_code(gid));
     rec[k] = r;
     pos[k] = make_double4(x[0], x[1], x[2], h);
     if (sc.on) posf[k] = screen_pos(sc, x[0], x[1], x[2], h);
     group[k] = gid;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, Lattice L,
+__global__ void __launch_bounds__(BLOCK) k_pack_mixed(DevState s, Params prm, Fixed F,
                                                       const uint32_t* __restrict__ order, PairRecF* __restrict__ rec,
                                                       double4* __restrict__ pos, float4* __restrict__ posf,
                                                       Screen sc, uint32_t* __restrict__ group,
                                                       size_t n) {
     const size_t k = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    if (k < n) pack_mixed_at(s, prm, L, order, rec, pos, posf, sc, group, k);
+    if (k < n) pack_mixed_at(s, prm, F, order, rec, pos, posf, sc, group, k);
 }
 
 // M4 gradient factor grad W / q of CubicSpline<3> (core/sph/kernel/Kernel.h:172-187) in closed form,
@@ -983,36 +990,39 @@ struct Acc {
     __device__ __forceinline__ void add(float v) { sum += double(v); }
 };
 
-// Mixed-precision pair sums: the same terms and gather form as k_pairs_fp64, evaluated in FP32 on
-// lattice-relative positions and velocities and accumulated in FP64; results are written in FP64.  A
-// difference (L_j - L_i) S + (d_j - d_i) is exact in its lattice part and rounds the FP32 offsets once, so
-// it keeps FP32 precision relative to the cell edge S, whatever the magnitudes of the absolute values.
-// Divisions are hardware reciprocals (about 1 ulp) times the numerator.
+// Mixed-precision pair sums: the same terms and gather form as k_pairs_fp64, evaluated in FP32 on the
+// records' fixed-point positions and velocities and accumulated in FP64; results are written in FP64.  A
+// difference of fixed-point coordinates is an exact int32 (each frame spans less than 2^31 steps), rounded
+// once by the conversion to FP32 and scaled by the power-of-two step exactly, so it keeps FP32 precision
+// whatever the magnitudes of the absolute values.  Divisions are hardware reciprocals
+// (about 1 ulp) times the numerator.  Groups (SUM_ONLY_UNDAMAGED) compare by the records' 4-bit codes,
+// and through the group array for groups beyond the codes' range.
 template <int LPP, bool FUSED>
-__device__ __forceinline__ void pairs_mixed_body(size_t t, DevState s, const Params& prm, const Lattice& L,
-                                                 const PairRecF* __restrict__ rec, const uint32_t* __restrict__ order,
+__device__ __forceinline__ void pairs_mixed_body(size_t t, DevState s, const Params& prm, const Fixed& F,
+                                                 const PairRecF* __restrict__ rec, const uint32_t* __restrict__ group,
+                                                 const uint32_t* __restrict__ order,
                                                  const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
                                                  uint32_t kmax, size_t n, const FuseArgs& fa) {
     const PairLane<LPP> ln(t, n);
     const size_t k = ln.k;
     const PairRecF me = rec[k];
-    const uint32_t gi = me.group;
-    const int lxi = int(me.lat & 1023u), lyi = int((me.lat >> 10) & 1023u), lzi = int(me.lat >> 20);
-    const int vxi = int(me.vlat & 1023u), vyi = int((me.vlat >> 10) & 1023u), vzi = int(me.vlat >> 20);
+    const uint32_t gi = group[k];
+    const uint32_t gci = __float_as_uint(me.cs) & 15u;
+    const float csi = __uint_as_float(__float_as_uint(me.cs) & ~15u);
     const float rhoInv = 1.0f / me.rho;
     const float alpha = float(prm.av_alpha), beta = float(prm.av_beta);
     Acc ax, ay, az, du, divv, gxx, gyy, gzz, gxy, gxz, gyz;
     const uint32_t cnt = ln.valid ? count[k] : 0u;
     const uint32_t listed = cnt < kmax ? cnt : kmax;
     const bool solid = prm.solid_stress != 0;
+    const bool grouped = solid && gci != GROUP_CODE_NONE;  // stress terms with the particle's own group
     // the LDS stage exists whenever the gather mode uses it (one uint4 otherwise)
     __shared__ uint4 stage[PAIRSMX_STAGE ? (BLOCK / 64) * StageOf<PairRecF>::WORDS : 1];
     uint4* my_stage = PAIRSMX_STAGE ? stage + (threadIdx.x >> 6) * StageOf<PairRecF>::WORDS : stage;
-    for_each_neighbor<LPP, PAIRSMX_STAGE>(rec, list, k, kmax >> 2, listed, ln.sub, my_stage, [&](const PairRecF& o, uint32_t) {
-        const int lxj = int(o.lat & 1023u), lyj = int((o.lat >> 10) & 1023u), lzj = int(o.lat >> 20);
-        const float rx = float(lxi - lxj) * L.Sf + (me.dx - o.dx);
-        const float ry = float(lyi - lyj) * L.Sf + (me.dy - o.dy);
-        const float rz = float(lzi - lzj) * L.Sf + (me.dz - o.dz);
+    for_each_neighbor<LPP, PAIRSMX_STAGE>(rec, list, k, kmax >> 2, listed, ln.sub, my_stage, [&](const PairRecF& o, uint32_t e) {
+        const float rx = float(int32_t(me.x - o.x)) * F.stepf;
+        const float ry = float(int32_t(me.y - o.y)) * F.stepf;
+        const float rz = float(int32_t(me.z - o.z)) * F.stepf;
         const float hbar = 0.5f * (me.h + o.h);
         const float hInv = __builtin_amdgcn_rcpf(hbar);
         const float hInv2 = hInv * hInv;
@@ -1020,17 +1030,18 @@ __device__ __forceinline__ void pairs_mixed_body(size_t t, DevState s, const Par
         const float gk = m4_grad_f(d2 * hInv2);
         const float h5g = (hInv2 * hInv2) * hInv * gk;
         const float gx = rx * h5g, gy = ry * h5g, gz = rz * h5g;
-        const int vxj = int(o.vlat & 1023u), vyj = int((o.vlat >> 10) & 1023u), vzj = int(o.vlat >> 20);
-        const float wx = float(vxj - vxi) * L.Svf + (o.ux - me.ux);  // v_j - v_i
-        const float wy = float(vyj - vyi) * L.Svf + (o.uy - me.uy);
-        const float wz = float(vzj - vzi) * L.Svf + (o.uz - me.uz);
+        const float wx = float(int32_t(o.vx - me.vx)) * F.vstepf;  // v_j - v_i
+        const float wy = float(int32_t(o.vy - me.vy)) * F.vstepf;
+        const float wz = float(int32_t(o.vz - me.vz)) * F.vstepf;
         const float mj = o.m;
         const float mjr = mj * rhoInv;
         const float wg = wx * gx + wy * gy + wz * gz;
         divv.add(mjr * wg);
         const float coef = me.pr + o.pr;
         float fx = -(coef * gx), fy = -(coef * gy), fz = -(coef * gz);
-        if (solid && gi == o.group && gi != NO_GROUP) {
+        bool same = grouped && (__float_as_uint(o.cs) & 15u) == gci;
+        if (same && gci == GROUP_CODE_WIDE) same = group[e] == gi;
+        if (same) {
             gxx.add(mjr * (wx * gx));
             gyy.add(mjr * (wy * gy));
             gzz.add(mjr * (wz * gz));
@@ -1046,7 +1057,7 @@ __device__ __forceinline__ void pairs_mixed_body(size_t t, DevState s, const Par
         const float dvdr = -(wx * rx + wy * ry + wz * rz);
         if (dvdr < 0.0f) {
             const float rhobar = 0.5f * (me.rho + o.rho);
-            const float csbar = 0.5f * (me.cs + o.cs);
+            const float csbar = 0.5f * (csi + __uint_as_float(__float_as_uint(o.cs) & ~15u));
             const float mu = hbar * dvdr * __builtin_amdgcn_rcpf(d2 + 1.e-2f * (hbar * hbar));
             const float av = (-alpha * csbar * mu + beta * (mu * mu)) * __builtin_amdgcn_rcpf(rhobar);
             fx -= av * gx;
@@ -1103,14 +1114,15 @@ __device__ __forceinline__ void pairs_mixed_body(size_t t, DevState s, const Par
 }
 
 template <int LPP, bool FUSED>
-__global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Lattice L,
+__global__ void __launch_bounds__(BLOCK) k_pairs_mixed(DevState s, Params prm, Fixed F,
                                                        const PairRecF* __restrict__ rec,
+                                                       const uint32_t* __restrict__ group,
                                                        const uint32_t* __restrict__ order,
                                                        const uint32_t* __restrict__ list,
                                                        const uint32_t* __restrict__ count, uint32_t kmax, size_t n,
                                                        FuseArgs fa) {
-    pairs_mixed_body<LPP, FUSED>(size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x, s, prm, L, rec,
-                                 order, list, count, kmax, n, fa);
+    pairs_mixed_body<LPP, FUSED>(size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x, s, prm, F, rec,
+                                 group, order, list, count, kmax, n, fa);
 }
 
 // ScalarGradyKippModel::integrate (core/physics/Damage.cpp:128-170) for particle i.
@@ -1768,13 +1780,13 @@ void launch_pairs_fp64(const Launch& l, DevState s, Params prm, const PairRec* r
 #undef OSPH_PAIRS64
 }
 
-void launch_pack_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const uint32_t* order, PairRecF* rec,
+void launch_pack_mixed(const Launch& l, DevState s, Params prm, Fixed fx, const uint32_t* order, PairRecF* rec,
                        double4* pos, float4* posf, Screen sc, uint32_t* group, size_t n) {
     if (!n) return;
-    k_pack_mixed<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, lat, order, rec, pos, posf, sc, group, n);
+    k_pack_mixed<<<grid_for(n), BLOCK, 0, l.stream>>>(s, prm, fx, order, rec, pos, posf, sc, group, n);
 }
 
-void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, const PairRecF* rec,
+void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Fixed fx, const PairRecF* rec, const uint32_t* group,
                         const uint32_t* order, const uint32_t* list, const uint32_t* count, uint32_t kmax, size_t n,
                         const FuseArgs* fuse) {
     if (!n) return;
@@ -1783,7 +1795,7 @@ void launch_pairs_mixed(const Launch& l, DevState s, Params prm, Lattice lat, co
     const size_t lds = size_t(l.tune.pairs_lds);
     const FuseArgs fa = fuse ? *fuse : FuseArgs{};
 #define OSPH_PAIRSMX(P, F) \
-    k_pairs_mixed<P, F><<<grid, b, lds, l.stream>>>(s, prm, lat, rec, order, list, count, kmax, n, fa)
+    k_pairs_mixed<P, F><<<grid, b, lds, l.stream>>>(s, prm, fx, rec, group, order, list, count, kmax, n, fa)
     if (lpp == 4) {
         if (fuse) OSPH_PAIRSMX(4, true); else OSPH_PAIRSMX(4, false);
     } else if (lpp == 2) {
@@ -1954,7 +1966,7 @@ __global__ void __launch_bounds__(BLOCK) kb_pack(const BatchMember* __restrict__
     const size_t k = size_t(xcd_block(bid, nb)) * BLOCK + threadIdx.x;
     if (k >= a.n) return;
     if constexpr (MIXED) {
-        pack_mixed_at(a.s2, a.prm, a.lat, a.order, static_cast<PairRecF*>(a.rec), a.pos, a.posf, a.screen, a.group, k);
+        pack_mixed_at(a.s2, a.prm, a.fx, a.order, static_cast<PairRecF*>(a.rec), a.pos, a.posf, a.screen, a.group, k);
     } else {
         pack_at(a.s2, a.prm, a.order, static_cast<PairRec*>(a.rec), a.pos, a.posf, a.screen, a.group, k);
     }
@@ -1978,8 +1990,8 @@ __global__ void __launch_bounds__(BLOCK) kb_pairs(const BatchMember* __restrict_
     const size_t t = size_t(xcd_block(bid, nb)) * blockDim.x + threadIdx.x;
     const FuseArgs fa{ a.mats, a.dt, a.small + SMALL_CRITERIA, overflow_of(a) };
     if constexpr (MIXED) {
-        pairs_mixed_body<LPP, FUSED>(t, a.s2, a.prm, a.lat, static_cast<const PairRecF*>(a.rec), a.order, a.list,
-                                     a.count, a.kmax, a.n, fa);
+        pairs_mixed_body<LPP, FUSED>(t, a.s2, a.prm, a.fx, static_cast<const PairRecF*>(a.rec), a.group, a.order,
+                                     a.list, a.count, a.kmax, a.n, fa);
     } else {
         pairs_fp64_body<LPP, FUSED>(t, a.s2, a.prm, static_cast<const PairRec*>(a.rec), a.group, a.order, a.list,
                                     a.count, lut, a.kmax, a.n, fa);

@@ -1,5 +1,5 @@
-"""The mixed-precision path (FP32 lattice-relative pair sums accumulated in FP64, FP64 state and
-integration): derivative error against the FP64 path on the same state, exact neighbour counts, and
+"""The mixed-precision path (FP32 pair sums on fixed-point positions and velocities, accumulated in FP64;
+FP64 state and integration): derivative error against the FP64 path on the same state, exact neighbour counts, and
 energy conservation over 100 predictor-corrector steps."""
 import numpy as np
 import pytest
@@ -30,6 +30,40 @@ def test_mixed_derivatives_match_fp64(name, lanes):
     da = a.download(st)
     b = gpu.GpuSolver(st, cfg, precision="mixed")
     b.tune(pairs_lanes=lanes)
+    b.integrate()
+    db = b.download(st)
+    assert np.array_equal(da["ncnt"], db["ncnt"])
+    assert rel_l2(db["dv"][:, :3], da["dv"][:, :3]) < ACCEL_TOL
+    for k in ("divv", "drho", "du", "gradv", "ds"):
+        assert rel_l2(db[k], da[k]) < 1e-4, k
+
+
+@pytest.mark.parametrize("flags", ["codes", "boundary", "wide", "shifted"])
+def test_mixed_groups_and_frames_match_fp64(flags):
+    """SUM_ONLY_UNDAMAGED groups in the mixed records (4-bit codes, and the group array beyond them) and the
+    fixed-point frames far from the origin at a high bulk speed: the mixed derivatives match the FP64
+    path, whose group test compares the flags themselves.  A group compared wrongly adds or drops whole
+    velocity-gradient terms, far beyond the tolerance."""
+    from cases import exercised_impact
+
+    st, cfg = exercised_impact()
+    body = st["flag"].copy()
+    rng = np.random.default_rng(11)
+    if flags == "codes":
+        st["flag"][:] = np.where(body == 0, 3, 9)
+    elif flags == "boundary":  # 13 is the last coded group, 14 the first one compared through the array
+        st["flag"][:] = np.where(body == 0, 13, 14)
+    elif flags == "wide":  # groups beyond the codes next to coded ones, split inside the target
+        st["flag"][:] = np.where(body == 0, np.where(rng.random(st.n) < 0.5, 500, 2), 1 << 20)
+    else:
+        st["r"][:, :3] += np.array([3.0e7, -2.0e7, 1.0e7])
+        st["v"][:, :3] += np.array([2.0e4, -3.0e4, 1.0e4])
+    assert cfg.sum_only_undamaged and cfg.solid_stress
+    print(f"[case] mixed groups/frames {flags} n={st.n}", flush=True)
+    a = gpu.GpuSolver(st, cfg, precision="fp64")
+    a.integrate()
+    da = a.download(st)
+    b = gpu.GpuSolver(st, cfg, precision="mixed")
     b.integrate()
     db = b.download(st)
     assert np.array_equal(da["ncnt"], db["ncnt"])
