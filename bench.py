@@ -45,6 +45,7 @@ MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (2 * 237493 + 97504) * 1024.0,
 # The fused FP64 kernel adds the finalize + corrector + criteria pass (about 60 doubles read and 25
 # written per particle), less the 92 B it no longer writes for that pass to read back.
 PAIR_BYTES_PER_PARTICLE = {"pairs_mixed": 168.0, "pairs_fp64": 232.0,
+                           "pairs_mixed_finalize_correct": 168.0 + 680.0 - 92.0,
                            "pairs_fp64_finalize_correct": 232.0 + 680.0 - 92.0}
 PAIR_BYTES_PER_NEIGHBOR = 4.0
 # VALU wave-instructions per dispatch of the pair kernels at the 1M impact after contact (rocprofv3 --pmc

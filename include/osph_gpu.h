@@ -249,13 +249,13 @@ int osph_adjoint_clear(osph_ctx* ctx);
  * "pairs_lanes" (1, 2 or 4 lanes share one particle's neighbour list in the pair kernels, default 1),
  * "cell_divisor" (1 to 4: neighbour-grid cells of 2 h_max / divisor, default 3), "lists_screen" (1: FP32-screened
  * neighbour test, 0: FP64 only, -1: screened in mixed precision only), "reorder_period" (the resident state
- * is moved into cell order every this many neighbour searches; 0 never), "fuse_finalize" (1: the PC step's
- * finalize, corrector and criteria run in the pair kernel, 0: as a separate pass, -1: fused for FP64 only,
- * the default). */
+ * is moved into cell order every this many neighbour searches; 0 never; default 40), "fuse_finalize" (1: the
+ * PC step's finalize, corrector and criteria run in the pair kernel, the default, 0: as a separate pass,
+ * -1: fused for FP64 only). */
 int osph_tune(osph_ctx* ctx, const char* key, long long value);
 /* Launch-shape autotuner (no reference counterpart): coordinate descent over osph_tune keys.  keys[k] takes
  * the n_values[k] candidates values[sum(n_values[<k]) ...], the first of them the starting value (keys = NULL:
- * pairs_lanes 1/2/4, cell_divisor 3/2/4, lists_block 256/128, pairs_block 256/128, fuse_finalize -1/0/1).
+ * pairs_lanes 1/2/4, cell_divisor 3/2/4, lists_block 256/128, pairs_block 256/128, fuse_finalize 1/0/-1).
  * Each candidate is timed over `steps` osph_step_pc calls with time step dt (after one untimed step) on the
  * resident state, which is restored on the device after every candidate: the tuning leaves the run where it
  * was.  One key at a time, in order, the fastest value is kept.  The chosen values are applied and written

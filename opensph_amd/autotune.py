@@ -23,7 +23,7 @@ DEFAULT_SPACE: List[Tuple[str, Sequence[int]]] = [
     ("cell_divisor", (3, 2, 4)),
     ("lists_block", (256, 128)),
     ("pairs_block", (256, 128)),
-    ("fuse_finalize", (-1, 0, 1)),
+    ("fuse_finalize", (1, 0, -1)),
 ]
 
 

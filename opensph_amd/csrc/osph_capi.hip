@@ -1872,7 +1872,7 @@ int osph_autotune(osph_ctx* c, double dt, int steps, const char* const* keys, co
     static const char* const default_keys[] = { "pairs_lanes", "cell_divisor", "lists_block", "pairs_block",
                                                 "fuse_finalize" };
     static const int default_counts[] = { 3, 3, 2, 2, 3 };
-    static const long long default_values[] = { 1, 2, 4, 3, 2, 4, 256, 128, 256, 128, -1, 0, 1 };
+    static const long long default_values[] = { 1, 2, 4, 3, 2, 4, 256, 128, 256, 128, 1, 0, -1 };
     if (!keys) {
         keys = default_keys;
         n_values = default_counts;

@@ -177,9 +177,11 @@ struct Tuning {
     int lists_lds = 0;
     int cell_divisor = 3;  // cells of 2 h_max(class) / divisor
     int lists_screen = -1; // FP32 screen in the list kernel: 1 on, 0 off, -1 on for mixed precision only
-    int reorder_period = 10;  // move the state into cell order every this many searches (0: never)
-    int fuse_finalize = -1;   // PC step: finalize, corrector and criteria in the pair kernel: 1 always, 0 never,
-                              // -1 for FP64 only (measured faster there, slower in mixed precision)
+    int reorder_period = 40;  // move the state into cell order every this many searches (0: never); at the
+                              // 1M impact 10 / 20 / 40 / 80 measured 1.738 / 1.724 / 1.723 / 1.736 ms per
+                              // FP64 step and 1.243 / 1.223 / 1.216 / 1.222 ms per mixed step (fused)
+    int fuse_finalize = 1;    // PC step: finalize, corrector and criteria in the pair kernel: 1 always, 0 never,
+                              // -1 for FP64 only (the mixed step measured 1.246 ms unfused, 1.234 ms fused)
     int batch_graph = 1;      // batched steps replay their stages after the sort as a captured HIP graph
 };
 
