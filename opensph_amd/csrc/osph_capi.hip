@@ -464,6 +464,18 @@ int ensure_href(osph_ctx* c) {
     return OSPH_OK;
 }
 
+// PredictorCorrector's swap of the highest derivatives into the predictions storage, and the zeroing
+// of the derivatives for the next evaluation (TimeStepping.cpp:331-334), as an exchange of plane
+// pointers: the prediction planes take the derivatives of the last evaluation without a copy, and the
+// derivative planes, whose stale values the next evaluation overwrites, take the place of the old
+// predictions.  Done once before each prediction pass.
+void swap_predictions(osph_ctx* c) {
+    static constexpr int PAIRS[11][2] = { { AX, PAX }, { AY, PAY }, { AZ, PAZ }, { DRHO, PDRHO }, { DU, PDU },
+                                          { DSXX, PDSXX }, { DSYY, PDSYY }, { DSXY, PDSXY }, { DSXZ, PDSXZ },
+                                          { DSYZ, PDSYZ }, { DD, PDD } };
+    for (const auto& p : PAIRS) std::swap(c->s.d[p[0]], c->s.d[p[1]]);
+}
+
 // The pass before the neighbour search (k_advance: predictions for mode 2, the material initialize for
 // modes 1 and 2) with the per-class grid bounds, then the grid, the sort by cell, the cell ranges and
 // the packed neighbour records (ISymmetricFinder::build).  One host synchronisation reads the class
@@ -473,6 +485,7 @@ int search_begin(osph_ctx* c, int mode, double dt) {
     const size_t n = c->n;
     Launch l = L(c);
     if (int e = ensure_href(c)) return e;
+    if (mode == 2) swap_predictions(c);
     timed(c, "init_small", [&] { launch_init_small(l, c->small); });
     const double href = c->href;
     unsigned long long* cacc = c->small + SMALL_CLASSES;
@@ -1389,7 +1402,6 @@ int batch_step(const std::vector<osph_ctx*>& ms, const std::vector<double>& dts,
             c->bounds_part_cap = need;
         }
         BatchMember& b = P->host[k];
-        b.s = c->s;
         b.mats = c->mats_dev;
         b.prm = c->prm;
         b.dt = dts[k];
@@ -1397,6 +1409,11 @@ int batch_step(const std::vector<osph_ctx*>& ms, const std::vector<double>& dts,
         b.part = c->bounds_part;
         b.n = uint32_t(c->n);
         b.small = P->small + SMALL_WORDS * k;
+    }
+    // the prediction planes of every member, once all allocations have succeeded
+    for (size_t k = 0; k < count; ++k) {
+        swap_predictions(ms[k]);
+        P->host[k].s = ms[k]->s;
     }
     const Launch l{ lead->stream, P->tune };
     hipStream_t st = lead->stream;
@@ -1916,6 +1933,9 @@ int osph_autotune(osph_ctx* c, double dt, int steps, const char* const* keys, co
                              hipMemcpyDeviceToDevice, c->stream));
     const int since = c->since_reorder;
     const double href = c->href;
+    // the plane order within the block (the prediction planes trade places with the derivative planes)
+    std::vector<ptrdiff_t> doff(NDPLANES);
+    for (int k = 0; k < NDPLANES; ++k) doff[k] = c->s.d[k] - c->dblock;
     auto restore = [&]() -> int {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipMemcpyAsync(c->dblock, dsave.p, sizeof(double) * NDPLANES * cap, hipMemcpyDeviceToDevice, c->stream));
@@ -1925,7 +1945,7 @@ int osph_autotune(osph_ctx* c, double dt, int steps, const char* const* keys, co
                                  hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->slot_of, usave.p + size_t(NIPLANES + 1) * cap, sizeof(uint32_t) * cap,
                                  hipMemcpyDeviceToDevice, c->stream));
-        for (int k = 0; k < NDPLANES; ++k) c->s.d[k] = c->dblock + size_t(k) * cap;
+        for (int k = 0; k < NDPLANES; ++k) c->s.d[k] = c->dblock + doff[k];
         for (int k = 0; k < NIPLANES; ++k) c->s.u[k] = c->ublock + size_t(k) * cap;
         c->since_reorder = since;
         c->href = href;

@@ -1143,19 +1143,24 @@ __device__ void damage_rate(DevState& s, const osph_material& m, size_t i) {
     const double young_red = dmax((1.0 - D0 * D0 * D0) * m.young_modulus, 1.e-20);
     const double strain = sigMax / young_red;
     const double ratio = strain / s.d[EPSMIN][i];
-    if (ratio <= 1.0) return;
+    if (ratio <= 1.0) {
+        s.d[DD][i] = 0.0;
+        return;
+    }
     s.d[DD][i] = s.d[GROWTH][i] * cbrt(dmin(pow(ratio, s.d[MZERO][i]), double(s.u[NFLAWS][i])));
 }
 
 // EquationHolder::finalize in reverse term order (EquationTerm.h:293-297): AdaptiveSmoothingLength
 // (EquationTerm.cpp:366-387) or ConstSmoothingLength (:427-434), ContinuityEquation (:289-299),
-// SolidStressForce (:177-203), PressureForce (:90-99); then the material finalize (damage).
+// SolidStressForce (:177-203), PressureForce (:90-99); then the material finalize (damage).  Every
+// highest derivative the step integrates is assigned here or in the pair sums (the reference zeroes them
+// before the evaluation and adds the terms' contributions, which is the same value).
 __device__ void finalize_at(DevState& s, const osph_material& m, const Params& prm, size_t i) {
 #pragma clang fp contract(off)
     const double rho = s.d[RHO][i], divv = s.d[DIVV][i];
     const double h = s.d[H][i];
     s.d[VH][i] = (prm.adaptive_h && h > 2.0 * prm.h_lo) ? h / 3.0 * divv : 0.0;
-    s.d[DRHO][i] += -rho * divv;
+    s.d[DRHO][i] = -rho * divv;
     double du = s.d[DU][i];
     if (prm.solid_stress && m.yielding != 0) {
         const double sxx = s.d[SXX][i], syy = s.d[SYY][i], szz = -sxx - syy;
@@ -1166,15 +1171,23 @@ __device__ void finalize_at(DevState& s, const osph_material& m, const Params& p
         du += 1.0 / rho * dd;
         const double tr3 = (gxx * 1.0 + gyy * 1.0 + gzz * 1.0) / 3.0;
         const double twomu = 2.0 * m.shear_modulus;
-        s.d[DSXX][i] += twomu * (gxx - tr3);
-        s.d[DSYY][i] += twomu * (gyy - tr3);
-        s.d[DSXY][i] += twomu * gxy;
-        s.d[DSXZ][i] += twomu * gxz;
-        s.d[DSYZ][i] += twomu * gyz;
+        s.d[DSXX][i] = twomu * (gxx - tr3);
+        s.d[DSYY][i] = twomu * (gyy - tr3);
+        s.d[DSXY][i] = twomu * gxy;
+        s.d[DSXZ][i] = twomu * gxz;
+        s.d[DSYZ][i] = twomu * gyz;
+    } else if (prm.solid_stress) {
+        for (int k = 0; k < 5; ++k) s.d[DSXX + k][i] = 0.0;
     }
     du -= s.d[P][i] / rho * divv;
     s.d[DU][i] = du;
-    if (prm.has_damage && m.damage == 1) damage_rate(s, m, i);
+    if (prm.has_damage) {
+        if (m.damage == 1) {
+            damage_rate(s, m, i);
+        } else {
+            s.d[DD][i] = 0.0;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_finalize(DevState s, const osph_material* __restrict__ mats, Params prm,
@@ -1200,52 +1213,38 @@ __device__ __forceinline__ void clamp_deriv(double& v, double& dv, double lo, do
     if (zero) dv = 0.0;
 }
 
-// PredictorCorrector::makePredictions (core/timestepping/TimeStepping.cpp:286-300), then the swap of
-// the highest derivatives into the predictions storage and Storage::zeroHighestDerivatives (:331-334).
+// PredictorCorrector::makePredictions (core/timestepping/TimeStepping.cpp:286-300).  The swap of the
+// highest derivatives into the predictions storage and Storage::zeroHighestDerivatives (:331-334) are
+// done by the host before this pass, by exchanging the plane pointers of each highest derivative and its
+// prediction (AX <-> PAX, DRHO <-> PDRHO, ...): the derivative of the last evaluation is read here from
+// the prediction planes, and the derivative planes are left for the next evaluation, which assigns every
+// one of them (the pair sums, the finalize).  A derivative zeroed by a clamp is written back to its
+// prediction plane, as the reference clamps the derivative it saves.
 __device__ void predict_at(DevState& s, const osph_material& m, const Params& prm, double dt, size_t i) {
 #pragma clang fp contract(off)
     const double dt2 = 0.5 * (dt * dt);
     for (int k = 0; k < 3; ++k) {
-        const double a = s.d[AX + k][i];
+        const double a = s.d[PAX + k][i];
         const double v = s.d[VX + k][i];
         s.d[X + k][i] += v * dt + a * dt2;
         s.d[VX + k][i] = v + a * dt;
-        s.d[PAX + k][i] = a;
-        s.d[AX + k][i] = 0.0;
     }
     s.d[H][i] += s.d[VH][i] * dt + 0.0 * dt2;
-    {
-        double x = s.d[RHO][i], dx = s.d[DRHO][i];
+    auto first_order = [&](int q, int pq, double lo, double hi) {
+        double x = s.d[q][i];
+        const double dx0 = s.d[pq][i];
+        double dx = dx0;
         x += dx * dt;
-        clamp_deriv(x, dx, m.rho_lo, m.rho_hi);
-        s.d[RHO][i] = x;
-        s.d[PDRHO][i] = dx;
-        s.d[DRHO][i] = 0.0;
-    }
-    {
-        double x = s.d[U][i], dx = s.d[DU][i];
-        x += dx * dt;
-        clamp_deriv(x, dx, m.u_lo, m.u_hi);
-        s.d[U][i] = x;
-        s.d[PDU][i] = dx;
-        s.d[DU][i] = 0.0;
-    }
+        clamp_deriv(x, dx, lo, hi);
+        s.d[q][i] = x;
+        if (dx != dx0) s.d[pq][i] = dx;
+    };
+    first_order(RHO, PDRHO, m.rho_lo, m.rho_hi);
+    first_order(U, PDU, m.u_lo, m.u_hi);
     if (prm.solid_stress) {
-        for (int k = 0; k < 5; ++k) {
-            const double dx = s.d[DSXX + k][i];
-            s.d[SXX + k][i] += dx * dt;
-            s.d[PDSXX + k][i] = dx;
-            s.d[DSXX + k][i] = 0.0;
-        }
+        for (int k = 0; k < 5; ++k) s.d[SXX + k][i] += s.d[PDSXX + k][i] * dt;
     }
-    if (prm.has_damage) {
-        double x = s.d[D][i], dx = s.d[DD][i];
-        x += dx * dt;
-        clamp_deriv(x, dx, m.d_lo, m.d_hi);
-        s.d[D][i] = x;
-        s.d[PDD][i] = dx;
-        s.d[DD][i] = 0.0;
-    }
+    if (prm.has_damage) first_order(D, PDD, m.d_lo, m.d_hi);
 }
 
 __device__ __forceinline__ void atomic_min_pos(unsigned long long* addr, double v) {
