@@ -12,7 +12,10 @@
  *                                              core/sph/solvers/SymmetricSolver.cpp:109-141)
  *   osph_integrate                         <- ISolver::integrate, SymmetricSolver<3>::integrate
  *                                             (core/timestepping/ISolver.h:38,
- *                                              core/sph/solvers/SymmetricSolver.cpp:58-95)
+ *                                              core/sph/solvers/SymmetricSolver.cpp:58-95), and
+ *                                             AsymmetricSolver::integrate / loop, whose per-particle
+ *                                             sums over all neighbours are the kernels' own form
+ *                                             (core/sph/solvers/AsymmetricSolver.cpp:70-96,153-202)
  *   osph_step_pc                           <- ITimeStepping::step of PredictorCorrector
  *                                             (core/timestepping/TimeStepping.cpp:34-75,286-346)
  *   osph_timestep                          <- MultiCriterion::compute

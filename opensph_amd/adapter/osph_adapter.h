@@ -6,6 +6,9 @@
 //                          (core/sph/solvers/SymmetricSolver.cpp:15-196): create() sets up the quantities as
 //                          the CPU solver does; integrate() evaluates the derivatives on the GPU.  It owns the
 //                          device context and the device copy of the state.
+//   GpuAsymmetricSolver    ISolver in place of AsymmetricSolver (core/sph/solvers/AsymmetricSolver.cpp:119-202):
+//                          the device kernels sum each particle's terms over all its neighbours, which is
+//                          AsymmetricSolver::loop; the same device path as GpuSymmetricSolver.
 //   GpuPredictorCorrector  ITimeStepping (core/timestepping/TimeStepping.h:46-87) in place of
 //                          PredictorCorrector (TimeStepping.cpp:272-346): the whole step, the time-step
 //                          criterion included, runs on the device with the state resident between steps.
@@ -81,6 +84,18 @@ public:
 
 private:
     void ensureContext(const Storage& storage);
+};
+
+/// AsymmetricSolver (core/sph/solvers/AsymmetricSolver.cpp:119-202) on the GPU.  AsymmetricSolver::loop
+/// (:153-202) finds all neighbours of particle i, keeps those within the symmetric cutoff 2 hbar, takes the
+/// gradient of the symmetrized kernel and evaluates the derivatives of i alone; the device kernels do
+/// exactly that for every particle (the gather form).  The pair set, the gradient and the terms are those
+/// of SymmetricSolver<3>, and IAsymmetricSolver::create (:98-102) sets up the same quantities as
+/// SymmetricSolver<3>::create, so the solver is GpuSymmetricSolver under the asymmetric solver's name;
+/// the results differ from either CPU solver only in the order of the per-particle sums.
+class GpuAsymmetricSolver : public GpuSymmetricSolver {
+public:
+    using GpuSymmetricSolver::GpuSymmetricSolver;
 };
 
 /// PredictorCorrector with the state resident in HBM: each step is one osph_step_pc (predictions, the

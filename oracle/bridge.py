@@ -203,7 +203,8 @@ class Oracle:
 
 
 class RefRun:
-    """The real OpenSPH SymmetricSolver<3> + PredictorCorrector on a copy of the given state."""
+    """The real OpenSPH SymmetricSolver<3> (or, with asymmetric=True, AsymmetricSolver) + PredictorCorrector
+    on a copy of the given state."""
 
     _lib = None
 
@@ -218,6 +219,9 @@ class RefRun:
             L.ref_create.restype = C.c_void_p
             L.ref_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_size_t,
                                      C.c_double, C.c_int]
+            L.ref_create_solver.restype = C.c_void_p
+            L.ref_create_solver.argtypes = [C.POINTER(Config), C.POINTER(C.c_double), C.POINTER(C.c_uint32),
+                                            C.c_size_t, C.c_double, C.c_int, C.c_int]
             L.ref_destroy.argtypes = [C.c_void_p]
             L.ref_integrate.argtypes = [C.c_void_p]
             L.ref_step.restype = C.c_double
@@ -234,13 +238,14 @@ class RefRun:
             cls._lib = L
         return cls._lib
 
-    def __init__(self, st, cfg, nthreads=0, dt0=None):
+    def __init__(self, st, cfg, nthreads=0, dt0=None, asymmetric=False):
         L = self.lib()
         self.st = st.copy()
         self.P, self.I = to_planes(st)
         self.c = make_config(cfg, st.materials)
         dt = cfg.initial_dt if dt0 is None else dt0
-        self.h = L.ref_create(C.byref(self.c), _ptr(self.P), _ptr(self.I, C.c_uint32), st.n, dt, nthreads)
+        self.h = L.ref_create_solver(C.byref(self.c), _ptr(self.P), _ptr(self.I, C.c_uint32), st.n, dt, nthreads,
+                                     int(asymmetric))
 
     def close(self):
         if self.h:

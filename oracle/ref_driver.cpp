@@ -13,6 +13,7 @@
 #include "sph/Materials.h"
 #include "sph/solvers/GravitySolver.h"
 #include "sph/solvers/StandardSets.h"
+#include "sph/solvers/AsymmetricSolver.h"
 #include "sph/solvers/SymmetricSolver.h"
 #include "system/Factory.h"
 #include "system/Settings.h"
@@ -33,7 +34,7 @@ struct RefRun {
     RunSettings settings;
     SharedPtr<IScheduler> scheduler;
     SharedPtr<Storage> storage;
-    AutoPtr<SymmetricSolver<3>> solver;
+    AutoPtr<ISolver> solver;
     AutoPtr<PredictorCorrector> pc;
     Statistics stats;
     bool solid = false;
@@ -138,6 +139,11 @@ void writeState(RefRun& run, const double* P, const uint32_t* I, size_t n) {
 extern "C" {
 
 void* ref_create(const orc_config* cfg, const double* P, const uint32_t* I, size_t n, double dt0, int nthreads) {
+    return ref_create_solver(cfg, P, I, n, dt0, nthreads, 0);
+}
+
+void* ref_create_solver(const orc_config* cfg, const double* P, const uint32_t* I, size_t n, double dt0,
+                        int nthreads, int asymmetric) {
     RefRun* run = new RefRun();
     RunSettings& s = run->settings;
     Flags<ForceEnum> forces(ForceEnum::PRESSURE);
@@ -182,7 +188,13 @@ void* ref_create(const orc_config* cfg, const double* P, const uint32_t* I, size
         // the self-gravity of the reference's GravitySolver, with the direct sum
         s.set(RunSettingsId::GRAVITY_SOLVER, GravityEnum::BRUTE_FORCE);
         s.set(RunSettingsId::GRAVITY_KERNEL, GravityKernelEnum::SPH_KERNEL);
-        run->solver = makeAuto<GravitySolver<SymmetricSolver<3>>>(*run->scheduler, s, eqs);
+        if (asymmetric) {
+            run->solver = makeAuto<GravitySolver<AsymmetricSolver>>(*run->scheduler, s, eqs);
+        } else {
+            run->solver = makeAuto<GravitySolver<SymmetricSolver<3>>>(*run->scheduler, s, eqs);
+        }
+    } else if (asymmetric) {
+        run->solver = makeAuto<AsymmetricSolver>(*run->scheduler, s, eqs);
     } else {
         run->solver = makeAuto<SymmetricSolver<3>>(*run->scheduler, s, eqs);
     }

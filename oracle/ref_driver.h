@@ -3,7 +3,8 @@
  *
  * extern "C" driver over the real OpenSPH core (compiled from its own sources into
  * oracle/_ref/libosph_ref.so by oracle/Makefile).  It builds a Storage from the oracle's plane layout
- * (osph_oracle.h), runs the reference SymmetricSolver<3> / PredictorCorrector / MultiCriterion on it,
+ * (osph_oracle.h), runs the reference SymmetricSolver<3> (or AsymmetricSolver) / PredictorCorrector /
+ * MultiCriterion on it,
  * and reads the state back.  Used to pin the oracle and as the CPU baseline ("reference" kind).
  *
  * Particles must be grouped by body: each maximal run of equal FLAG values becomes one body with its
@@ -21,6 +22,10 @@ extern "C" {
 /* nthreads == 0: SequentialScheduler (bit-stable, used for parity); > 0: ThreadPool of that size. */
 void* ref_create(const orc_config* cfg, const double* planes, const uint32_t* iplanes, size_t n, double dt0,
                  int nthreads);
+/* The same with the solver chosen: asymmetric == 0 SymmetricSolver<3>, 1 AsymmetricSolver (each particle
+ * sums over all its neighbours, AsymmetricSolver.cpp:153-202), both under GravitySolver with cfg->gravity. */
+void* ref_create_solver(const orc_config* cfg, const double* planes, const uint32_t* iplanes, size_t n, double dt0,
+                        int nthreads, int asymmetric);
 void ref_destroy(void* h);
 
 /* Storage::zeroHighestDerivatives + SymmetricSolver<3>::integrate on the current state. */
