@@ -486,10 +486,9 @@ int search_begin(osph_ctx* c, int mode, double dt) {
     Launch l = L(c);
     if (int e = ensure_href(c)) return e;
     if (mode == 2) swap_predictions(c);
-    timed(c, "init_small", [&] { launch_init_small(l, c->small); });
     const double href = c->href;
     unsigned long long* cacc = c->small + SMALL_CLASSES;
-    unsigned long long* vacc = c->cfg.precision == OSPH_PRECISION_MIXED ? c->small + SMALL_VEL : nullptr;
+    const bool vel = c->cfg.precision == OSPH_PRECISION_MIXED;
     if (bounds_partials(n) > c->bounds_part_cap) {
         (void)hipFree(c->bounds_part);
         c->bounds_part = nullptr;
@@ -498,7 +497,7 @@ int search_begin(osph_ctx* c, int mode, double dt) {
         c->bounds_part_cap = bounds_partials(n);
     }
     timed(c, mode == 2 ? "advance" : (mode == 1 ? "material_init" : "class_bounds"),
-          [&] { launch_advance(l, c->s, c->mats_dev, c->prm, mode, dt, href, cacc, vacc, c->bounds_part, n); });
+          [&] { launch_advance(l, c->s, c->mats_dev, c->prm, mode, dt, href, c->small, vel, c->bounds_part, n); });
     HIPCHK(c, hipMemcpyAsync(c->small_host + SMALL_CLASSES, cacc,
                              (SMALL_VEL + 6 - SMALL_CLASSES) * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                              c->stream));
@@ -654,7 +653,7 @@ int search_finish(osph_ctx* c) {
     if (int e = plan_search(c, c->small_host, &rows)) return e;
     const Grid g = c->grid;
 
-    timed(c, "cell_keys", [&] { launch_cell_keys(l, c->s, g, c->keys, c->iota, n); });
+    timed(c, "cell_keys", [&] { launch_cell_keys(l, c->s, g, c->keys, c->iota, c->row_range, 2 * rows, n); });
     const size_t sort_need = sort_pairs_temp_bytes(n, g.key_bits);
     if (sort_need > c->sort_temp_bytes) {
         HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -135,6 +135,16 @@ __host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) 
     return morton_spread(x) | (morton_spread(y) << 1) | (morton_spread(z) << 2);
 }
 
+// The inverse of spread2: the even bits of v, packed.
+__host__ __device__ inline uint32_t compact2(uint32_t v) {
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0f0f0f0fu;
+    v = (v | (v >> 4)) & 0x00ff00ffu;
+    v = (v | (v >> 8)) & 0x0000ffffu;
+    return v;
+}
+
 __host__ __device__ inline uint32_t spread2(uint32_t v) {
     v &= 0x3ffu;
     v = (v | (v << 8)) & 0x00ff00ffu;
@@ -209,11 +219,11 @@ void launch_init_small(const Launch& l, unsigned long long* small);
 // acc[0..2] = min x, y, z; acc[3..5] = max x, y, z; acc[6] = max h; acc[7] = min h (order-encoded).
 void launch_bounds_reduce(const Launch& l, DevState s, unsigned long long* acc, size_t n);
 // mode 2: predictions + material initialize; 1: material initialize; 0: neither.  Then the per-class
-// grid bounds (class of h relative to href) into acc[CLASS_STRIDE * c + 0..8] and, when vel is not null,
-// the velocity bounds into vel[0..5].
+// grid bounds (class of h relative to href) into small[SMALL_CLASSES + CLASS_STRIDE * c + 0..8] and, with
+// vel, the velocity bounds into small[SMALL_VEL + 0..5]; the step's other accumulators (small[0 ..
+// SMALL_CLASSES - 1]) start at their identities.
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
-                    double href, unsigned long long* acc, unsigned long long* vel, unsigned long long* part,
-                    size_t n);
+                    double href, unsigned long long* small, bool vel, unsigned long long* part, size_t n);
 // Entries of the per-block partial bounds that launch_advance needs for n particles.
 size_t bounds_partials(size_t n);
 // Values of the per-block partial bounds (partials per block).
@@ -242,7 +252,10 @@ void launch_scatter_planes(const Launch& l, const double* src, int nc, double* c
                            size_t n);
 void launch_gather_planes(const Launch& l, double* dst, int nc, double* const* planes, const uint32_t* slot_of,
                           size_t n);
-void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n);
+// Cell keys and the identity permutation; also clears the first `clear` words of `range` (the row
+// ranges that launch_cell_first fills).
+void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, uint32_t* range,
+                      size_t clear, size_t n);
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value);
 // first[cell] = the first sorted position whose cell index is >= cell, for every cell (x, y, z) of a
 // row (y < dim_y, z < dim_z) with x in [0, dim_x]; row_range (2 per row) is scratch.

@@ -256,8 +256,10 @@ __device__ __forceinline__ uint32_t cell_key_of(const DevState& s, const Grid& g
     return g.c[cls].base + row_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]), g.c[cls].bits);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cell_keys(DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n) {
+__global__ void __launch_bounds__(BLOCK) k_cell_keys(DevState s, Grid g, uint32_t* keys, uint32_t* idx,
+                                                     uint32_t* range, size_t clear, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t t = i; t < clear; t += size_t(gridDim.x) * blockDim.x) range[t] = 0u;
     if (i >= n) return;
     keys[i] = cell_key_of(s, g, i);
     idx[i] = uint32_t(i);
@@ -314,10 +316,35 @@ __device__ __forceinline__ void row_range_at(const uint32_t* __restrict__ keys, 
     range[2 * q + 1] = first_at_least(keys, g, row + (1u << G.bits), 0u, n);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_row_range(const uint32_t* __restrict__ keys, Grid g,
-                                                     uint32_t* __restrict__ range, size_t rows, uint32_t n) {
-    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (q < rows) row_range_at(keys, g, range, q, n);
+// Row number (as row_of numbers them) of a cell key: its class, then the (y, z) of its Morton row key.
+__device__ __forceinline__ uint32_t row_index_of(const Grid& g, uint32_t key) {
+    uint32_t off = 0, q = 0;
+    for (int c = 0; c < g.nclass; ++c) {
+        const GridClass& G = g.c[c];
+        if (G.empty) continue;
+        if (key >= G.base) {
+            const uint32_t m = (key - G.base) >> G.bits;
+            q = off + compact2(m >> 1) * uint32_t(G.dim[1]) + compact2(m);
+        }
+        off += uint32_t(G.dim[1]) * uint32_t(G.dim[2]);
+    }
+    return q;
+}
+
+// The rows' ranges of sorted positions in one pass over the sorted keys (rows are contiguous runs): the
+// first position of each row starts it and ends the row before; rows without particles keep the empty
+// range [0, 0) that k_cell_keys left.
+__global__ void __launch_bounds__(BLOCK) k_row_bounds(const uint32_t* __restrict__ keys, Grid g,
+                                                      uint32_t* __restrict__ range, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t q = row_index_of(g, keys[k]);
+    const uint32_t qp = k > 0 ? row_index_of(g, keys[k - 1]) : ~0u;
+    if (q != qp) {
+        range[2 * size_t(q)] = k;
+        if (k > 0) range[2 * size_t(qp) + 1] = k;
+    }
+    if (k + 1 == n) range[2 * size_t(q) + 1] = n;
 }
 
 // One thread per cell x in [0, dim_x] of a row: the binary search runs inside the row's own range.
@@ -1520,10 +1547,13 @@ __device__ __forceinline__ void bounds_final_at(int vi, const unsigned long long
     }
 }
 
+// Block v reduces bound value v; block 0 also starts the step's other accumulators (global bounds,
+// criterion minima, list overflow flag: words 0 .. SMALL_CLASSES - 1, which no earlier pass of the step
+// uses), in place of a launch of its own.
 __global__ void __launch_bounds__(BLOCK) k_bounds_final(const unsigned long long* __restrict__ part, uint32_t nblocks,
-                                                        unsigned long long* __restrict__ acc,
-                                                        unsigned long long* __restrict__ vel) {
-    bounds_final_at(blockIdx.x, part, nblocks, acc, vel);
+                                                        unsigned long long* __restrict__ small) {
+    if (blockIdx.x == 0 && threadIdx.x < SMALL_CLASSES) init_small_at(small, threadIdx.x);
+    bounds_final_at(blockIdx.x, part, nblocks, small + SMALL_CLASSES, small + SMALL_VEL);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_criteria(DevState s, const osph_material* __restrict__ mats, Params prm,
@@ -1694,9 +1724,10 @@ void launch_bounds_reduce(const Launch& l, DevState s, unsigned long long* acc, 
     k_bounds<<<g, BLOCK, 0, l.stream>>>(s, acc, n);
 }
 
-void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, size_t n) {
+void launch_cell_keys(const Launch& l, DevState s, Grid g, uint32_t* keys, uint32_t* idx, uint32_t* range,
+                      size_t clear, size_t n) {
     if (!n) return;
-    k_cell_keys<<<grid_for(n), BLOCK, 0, l.stream>>>(s, g, keys, idx, n);
+    k_cell_keys<<<grid_for(n), BLOCK, 0, l.stream>>>(s, g, keys, idx, range, clear, n);
 }
 
 void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value) {
@@ -1708,12 +1739,13 @@ void launch_clear_u32(const Launch& l, uint32_t* p, size_t count, uint32_t value
 
 void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* row_range, uint32_t* first,
                        size_t rows, size_t n) {
-    if (!rows) return;
+    // without particles nothing reads the tables (and the row ranges were not cleared)
+    if (!rows || !n) return;
     int xslots = 1;
     for (int c = 0; c < g.nclass; ++c) {
         if (!g.c[c].empty) xslots = xslots > g.c[c].dim[0] + 1 ? xslots : g.c[c].dim[0] + 1;
     }
-    k_row_range<<<grid_for(rows), BLOCK, 0, l.stream>>>(keys, g, row_range, rows, uint32_t(n));
+    k_row_bounds<<<grid_for(n), BLOCK, 0, l.stream>>>(keys, g, row_range, uint32_t(n));
     k_cell_first<<<grid_for(rows * size_t(xslots)), BLOCK, 0, l.stream>>>(keys, g, row_range, first, rows, xslots);
 }
 
@@ -1721,11 +1753,10 @@ size_t bounds_partials(size_t n) { return size_t(grid_for(n)) * BOUND_VALUES; }
 size_t bounds_values() { return BOUND_VALUES; }
 
 void launch_advance(const Launch& l, DevState s, const osph_material* mats, Params prm, int mode, double dt,
-                    double href, unsigned long long* acc, unsigned long long* vel, unsigned long long* part,
-                    size_t n) {
+                    double href, unsigned long long* small, bool vel, unsigned long long* part, size_t n) {
     if (!n) return;
     const unsigned g = grid_for(n);
-    const bool v = vel != nullptr;
+    const bool v = vel;
     if (mode == 2) {
         if (v) k_advance<2, true><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
         else k_advance<2, false><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
@@ -1736,7 +1767,7 @@ void launch_advance(const Launch& l, DevState s, const osph_material* mats, Para
         if (v) k_advance<0, true><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
         else k_advance<0, false><<<g, BLOCK, 0, l.stream>>>(s, mats, prm, dt, href, part, n);
     }
-    k_bounds_final<<<v ? BOUND_VALUES : CLASS_VALUES * MAX_CLASSES, BLOCK, 0, l.stream>>>(part, g, acc, vel);
+    k_bounds_final<<<v ? BOUND_VALUES : CLASS_VALUES * MAX_CLASSES, BLOCK, 0, l.stream>>>(part, g, small);
 }
 
 void launch_pack(const Launch& l, DevState s, Params prm, const uint32_t* order, PairRec* rec, double4* pos,
