@@ -316,8 +316,9 @@ __device__ __forceinline__ void row_range_at(const uint32_t* __restrict__ keys, 
     range[2 * q + 1] = first_at_least(keys, g, row + (1u << G.bits), 0u, n);
 }
 
-// Row number (as row_of numbers them) of a cell key: its class, then the (y, z) of its Morton row key.
-__device__ __forceinline__ uint32_t row_index_of(const Grid& g, uint32_t key) {
+// Row number (as row_of numbers them) of a cell key, from its class and the (y, z) of its Morton row key;
+// *cls = the class.
+__device__ __forceinline__ uint32_t row_index_of(const Grid& g, uint32_t key, int* cls) {
     uint32_t off = 0, q = 0;
     for (int c = 0; c < g.nclass; ++c) {
         const GridClass& G = g.c[c];
@@ -325,26 +326,61 @@ __device__ __forceinline__ uint32_t row_index_of(const Grid& g, uint32_t key) {
         if (key >= G.base) {
             const uint32_t m = (key - G.base) >> G.bits;
             q = off + compact2(m >> 1) * uint32_t(G.dim[1]) + compact2(m);
+            *cls = c;
         }
         off += uint32_t(G.dim[1]) * uint32_t(G.dim[2]);
     }
     return q;
 }
 
-// The rows' ranges of sorted positions in one pass over the sorted keys (rows are contiguous runs): the
-// first position of each row starts it and ends the row before; rows without particles keep the empty
-// range [0, 0) that k_cell_keys left.
-__global__ void __launch_bounds__(BLOCK) k_row_bounds(const uint32_t* __restrict__ keys, Grid g,
-                                                      uint32_t* __restrict__ range, uint32_t n) {
+// The cell table of the rows that hold particles, in one pass over the sorted keys (rows are contiguous
+// runs of them, cells contiguous runs inside a row).  The thread of sorted position k writes
+// first[cell] = k for the cells after the previous particle's cell (from x = 0 when k starts its row)
+// up to its own, and, when k ends its row, first[cell] = k + 1 for the cells after its own up to the
+// row's spare slot x = dim_x; it also records the row's range of sorted positions.  Rows without
+// particles keep the empty range [0, 0) that k_cell_keys left, and k_empty_rows fills their cells.
+__global__ void __launch_bounds__(BLOCK) k_row_cells(const uint32_t* __restrict__ keys, Grid g,
+                                                     uint32_t* __restrict__ range, uint32_t* __restrict__ first,
+                                                     uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t q = row_index_of(g, keys[k]);
-    const uint32_t qp = k > 0 ? row_index_of(g, keys[k - 1]) : ~0u;
-    if (q != qp) {
-        range[2 * size_t(q)] = k;
-        if (k > 0) range[2 * size_t(qp) + 1] = k;
+    int cls = 0, cp = 0, cn = 0;
+    const uint32_t key = keys[k];
+    const uint32_t q = row_index_of(g, key, &cls);
+    const GridClass& G = g.c[cls];
+    // the cell's x in its row (the class's keys start at G.base, which need not be a multiple of 2^bits)
+    const uint32_t xmask = (1u << G.bits) - 1u;
+    const uint32_t x = (key - G.base) & xmask;
+    const uint32_t rowbase = key - x;
+    uint32_t xlo = 0;
+    if (k > 0) {
+        const uint32_t kp = keys[k - 1];
+        if (row_index_of(g, kp, &cp) == q) xlo = ((kp - G.base) & xmask) + 1u;
     }
-    if (k + 1 == n) range[2 * size_t(q) + 1] = n;
+    if (xlo == 0) range[2 * size_t(q)] = k;
+    for (uint32_t c = xlo; c <= x; ++c) first[rowbase + c] = k;
+    const bool last = k + 1 == n || row_index_of(g, keys[k + 1], &cn) != q;
+    if (last) {
+        range[2 * size_t(q) + 1] = k + 1;
+        for (uint32_t c = x + 1; c <= uint32_t(G.dim[0]); ++c) first[rowbase + c] = k + 1;
+    }
+}
+
+// The cells of the rows without particles (empty range): equal entries, so that every x-run of them
+// reads as an empty range of sorted positions.
+__global__ void __launch_bounds__(BLOCK) k_empty_rows(Grid g, const uint32_t* __restrict__ range,
+                                                      uint32_t* __restrict__ first, size_t rows, int xslots) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t q = t / size_t(xslots);
+    const uint32_t x = uint32_t(t % size_t(xslots));
+    if (q >= rows) return;
+    if (range[2 * q] != range[2 * q + 1]) return;
+    int cls;
+    uint32_t cy, cz;
+    if (!row_of(g, q, cls, cy, cz)) return;
+    const GridClass& G = g.c[cls];
+    if (x > uint32_t(G.dim[0])) return;
+    first[G.base + row_key(x, cy, cz, G.bits)] = 0u;
 }
 
 // One thread per cell x in [0, dim_x] of a row: the binary search runs inside the row's own range.
@@ -361,12 +397,6 @@ __device__ __forceinline__ void cell_first_at(const uint32_t* __restrict__ keys,
     if (x > uint32_t(G.dim[0])) return;
     const uint32_t cell = G.base + row_key(x, cy, cz, G.bits);
     first[cell] = first_at_least(keys, g, cell, range[2 * q], range[2 * q + 1]);
-}
-
-__global__ void __launch_bounds__(BLOCK) k_cell_first(const uint32_t* __restrict__ keys, Grid g,
-                                                      const uint32_t* __restrict__ range, uint32_t* __restrict__ first,
-                                                      size_t rows, int xslots) {
-    cell_first_at(keys, g, range, first, size_t(blockIdx.x) * blockDim.x + threadIdx.x, rows, xslots);
 }
 
 // Pack the neighbour-side data of particle order[k] into rec[k].
@@ -1745,8 +1775,8 @@ void launch_cell_first(const Launch& l, const uint32_t* keys, Grid g, uint32_t* 
     for (int c = 0; c < g.nclass; ++c) {
         if (!g.c[c].empty) xslots = xslots > g.c[c].dim[0] + 1 ? xslots : g.c[c].dim[0] + 1;
     }
-    k_row_bounds<<<grid_for(n), BLOCK, 0, l.stream>>>(keys, g, row_range, uint32_t(n));
-    k_cell_first<<<grid_for(rows * size_t(xslots)), BLOCK, 0, l.stream>>>(keys, g, row_range, first, rows, xslots);
+    k_row_cells<<<grid_for(n), BLOCK, 0, l.stream>>>(keys, g, row_range, first, uint32_t(n));
+    k_empty_rows<<<grid_for(rows * size_t(xslots)), BLOCK, 0, l.stream>>>(g, row_range, first, rows, xslots);
 }
 
 size_t bounds_partials(size_t n) { return size_t(grid_for(n)) * BOUND_VALUES; }
