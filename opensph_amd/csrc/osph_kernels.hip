@@ -674,9 +674,25 @@ struct PairLane {
     }
 };
 
-__device__ void finalize_at(DevState& s, const osph_material& m, const Params& prm, size_t i);
-__device__ void correct_at(DevState& s, const osph_material& m, const Params& prm, double dt, size_t i);
-__device__ void criteria_values(const DevState& s, const osph_material& m, const Params& prm, size_t i, double out[4]);
+// The pair sums of particle i that the finalize, the corrector and the criteria use: acceleration, du,
+// velocity divergence and gradient (xx, yy, zz, xy, xz, yz).  The fused pair kernels hand them over in
+// registers; the separate passes read them from the state (sums_of).
+struct PairSums {
+    double a[3], du, divv, g[6];
+};
+__device__ __forceinline__ PairSums sums_of(const DevState& s, bool solid, size_t i) {
+    PairSums ps;
+    for (int k = 0; k < 3; ++k) ps.a[k] = s.d[AX + k][i];
+    ps.du = s.d[DU][i];
+    ps.divv = s.d[DIVV][i];
+    for (int k = 0; k < 6; ++k) ps.g[k] = solid ? s.d[GXX + k][i] : 0.0;
+    return ps;
+}
+__device__ void finalize_at(DevState& s, const osph_material& m, const Params& prm, size_t i, const PairSums& ps);
+__device__ void correct_at(DevState& s, const osph_material& m, const Params& prm, double dt, size_t i,
+                           const double a[3]);
+__device__ void criteria_values(const DevState& s, const osph_material& m, const Params& prm, size_t i,
+                                const double a[3], double divv, double out[4]);
 __device__ void reduce_criteria(double v[4], unsigned long long* mins);
 
 
@@ -945,9 +961,10 @@ __device__ __forceinline__ void pairs_fp64_body(size_t t, DevState s, const Para
         s.u[NCNT][i] = cnt;
         if (FUSED && !*fa.overflow) {
             const osph_material m = fa.mats[s.u[MAT][i]];
-            finalize_at(s, m, prm, i);
-            correct_at(s, m, prm, fa.dt, i);
-            criteria_values(s, m, prm, i, cv);
+            const PairSums ps{ { ax, ay, az }, du, divv, { gxx, gyy, gzz, gxy, gxz, gyz } };
+            finalize_at(s, m, prm, i, ps);
+            correct_at(s, m, prm, fa.dt, i, ps.a);
+            criteria_values(s, m, prm, i, ps.a, ps.divv, cv);
         }
     }
     if (FUSED && !*fa.overflow) reduce_criteria(cv, fa.mins);
@@ -1140,15 +1157,12 @@ __device__ __forceinline__ void pairs_mixed_body(size_t t, DevState s, const Par
     double cv[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
     if (ln.valid && ln.sub == 0) {
         const uint32_t i = order[k];
-        if (prm.add_accel) {
-            s.d[AX][i] = sax + s.d[AX][i];
-            s.d[AY][i] = say + s.d[AY][i];
-            s.d[AZ][i] = saz + s.d[AZ][i];
-        } else {
-            s.d[AX][i] = sax;
-            s.d[AY][i] = say;
-            s.d[AZ][i] = saz;
-        }
+        const double fax = prm.add_accel ? sax + s.d[AX][i] : sax;
+        const double fay = prm.add_accel ? say + s.d[AY][i] : say;
+        const double faz = prm.add_accel ? saz + s.d[AZ][i] : saz;
+        s.d[AX][i] = fax;
+        s.d[AY][i] = fay;
+        s.d[AZ][i] = faz;
         s.d[DU][i] = sdu;
         s.d[DIVV][i] = sdivv;
         if (solid) {
@@ -1162,9 +1176,10 @@ __device__ __forceinline__ void pairs_mixed_body(size_t t, DevState s, const Par
         s.u[NCNT][i] = cnt;
         if (FUSED && !*fa.overflow) {
             const osph_material m = fa.mats[s.u[MAT][i]];
-            finalize_at(s, m, prm, i);
-            correct_at(s, m, prm, fa.dt, i);
-            criteria_values(s, m, prm, i, cv);
+            const PairSums ps{ { fax, fay, faz }, sdu, sdivv, { g[0], g[1], g[2], g[3], g[4], g[5] } };
+            finalize_at(s, m, prm, i, ps);
+            correct_at(s, m, prm, fa.dt, i, ps.a);
+            criteria_values(s, m, prm, i, ps.a, ps.divv, cv);
         }
     }
     if (FUSED && !*fa.overflow) reduce_criteria(cv, fa.mins);
@@ -1212,17 +1227,17 @@ __device__ void damage_rate(DevState& s, const osph_material& m, size_t i) {
 // SolidStressForce (:177-203), PressureForce (:90-99); then the material finalize (damage).  Every
 // highest derivative the step integrates is assigned here or in the pair sums (the reference zeroes them
 // before the evaluation and adds the terms' contributions, which is the same value).
-__device__ void finalize_at(DevState& s, const osph_material& m, const Params& prm, size_t i) {
+__device__ void finalize_at(DevState& s, const osph_material& m, const Params& prm, size_t i, const PairSums& ps) {
 #pragma clang fp contract(off)
-    const double rho = s.d[RHO][i], divv = s.d[DIVV][i];
+    const double rho = s.d[RHO][i], divv = ps.divv;
     const double h = s.d[H][i];
     s.d[VH][i] = (prm.adaptive_h && h > 2.0 * prm.h_lo) ? h / 3.0 * divv : 0.0;
     s.d[DRHO][i] = -rho * divv;
-    double du = s.d[DU][i];
+    double du = ps.du;
     if (prm.solid_stress && m.yielding != 0) {
         const double sxx = s.d[SXX][i], syy = s.d[SYY][i], szz = -sxx - syy;
-        const double gxx = s.d[GXX][i], gyy = s.d[GYY][i], gzz = s.d[GZZ][i];
-        const double gxy = s.d[GXY][i], gxz = s.d[GXZ][i], gyz = s.d[GYZ][i];
+        const double gxx = ps.g[0], gyy = ps.g[1], gzz = ps.g[2];
+        const double gxy = ps.g[3], gxz = ps.g[4], gyz = ps.g[5];
         const double dd = (sxx * gxx + syy * gyy + szz * gzz) +
                           2.0 * (s.d[SXY][i] * gxy + s.d[SXZ][i] * gxz + s.d[SYZ][i] * gyz);
         du += 1.0 / rho * dd;
@@ -1251,7 +1266,7 @@ __global__ void __launch_bounds__(BLOCK) k_finalize(DevState s, const osph_mater
                                                     size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    finalize_at(s, mats[s.u[MAT][i]], prm, i);
+    finalize_at(s, mats[s.u[MAT][i]], prm, i, sums_of(s, prm.solid_stress != 0, i));
 }
 
 // Storage::zeroHighestDerivatives (core/quantities/Storage.cpp:594-602)
@@ -1312,7 +1327,8 @@ __device__ __forceinline__ void atomic_min_pos(unsigned long long* addr, double 
 // Per-particle values of the four criteria of MultiCriterion (TimeStepCriterion.cpp:315-356 Courant,
 // :123-211 DerivativeCriterion with the minimal-step mean, :221-257 Acceleration, :267-305 Divergence),
 // reduced to the minimum of each criterion in mins[0..3].
-__device__ void criteria_values(const DevState& s, const osph_material& m, const Params& prm, size_t i, double out[4]) {
+__device__ void criteria_values(const DevState& s, const osph_material& m, const Params& prm, size_t i,
+                                const double a[3], double divv, double out[4]) {
 #pragma clang fp contract(off)
     out[0] = out[1] = out[2] = out[3] = INFINITY;
     const double h = s.d[H][i];
@@ -1343,12 +1359,12 @@ __device__ void criteria_values(const DevState& s, const osph_material& m, const
         out[1] = mn;
     }
     if (prm.criteria & 4) {
-        const double ax = s.d[AX][i], ay = s.d[AY][i], az = s.d[AZ][i];
+        const double ax = a[0], ay = a[1], az = a[2];
         const double dvNorm = ax * ax + ay * ay + az * az;
         if (dvNorm > EPS_D) out[2] = prm.deriv_factor * sqrt(sqrt((h * h) / dvNorm));
     }
     if (prm.criteria & 8) {
-        const double dv = fabs(s.d[DIVV][i]);
+        const double dv = fabs(divv);
         if (dv > EPS_D) out[3] = prm.div_factor / dv;
     }
 }
@@ -1378,13 +1394,14 @@ __device__ void reduce_criteria(double v[4], unsigned long long* mins) {
 
 // PredictorCorrector::makeCorrections (TimeStepping.cpp:302-322): pdv is the new derivative, cdv the
 // saved prediction; the clamp zeroes the new derivative at the bounds.
-__device__ void correct_at(DevState& s, const osph_material& m, const Params& prm, double dt, size_t i) {
+__device__ void correct_at(DevState& s, const osph_material& m, const Params& prm, double dt, size_t i,
+                           const double a[3]) {
 #pragma clang fp contract(off)
     {
         const double dt2 = 0.5 * (dt * dt);
         const double ca = 1.0 / 3.0, cb = 0.5;
         for (int k = 0; k < 3; ++k) {
-            const double pdv = s.d[AX + k][i], cdv = s.d[PAX + k][i];
+            const double pdv = a[k], cdv = s.d[PAX + k][i];
             s.d[X + k][i] -= ca * (cdv - pdv) * dt2;
             s.d[VX + k][i] -= cb * (cdv - pdv) * dt;
         }
@@ -1428,9 +1445,10 @@ __device__ __forceinline__ void finalize_correct_body(size_t i, DevState s, cons
     double v[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
     if (i < n) {
         const osph_material m = mats[s.u[MAT][i]];
-        finalize_at(s, m, prm, i);
-        correct_at(s, m, prm, dt, i);
-        criteria_values(s, m, prm, i, v);
+        const PairSums ps = sums_of(s, prm.solid_stress != 0, i);
+        finalize_at(s, m, prm, i, ps);
+        correct_at(s, m, prm, dt, i, ps.a);
+        criteria_values(s, m, prm, i, ps.a, ps.divv, v);
     }
     reduce_criteria(v, mins);
 }
@@ -1592,7 +1610,8 @@ __global__ void __launch_bounds__(BLOCK) k_criteria(DevState s, const osph_mater
     double v[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
     if (i < n) {
         const osph_material m = mats[s.u[MAT][i]];
-        criteria_values(s, m, prm, i, v);
+        const double a[3] = { s.d[AX][i], s.d[AY][i], s.d[AZ][i] };
+        criteria_values(s, m, prm, i, a, s.d[DIVV][i], v);
     }
     reduce_criteria(v, mins);
 }
