@@ -37,24 +37,26 @@ FLOP_PER_QUADRUPOLE = 61.0
 # HBM bytes per launch of the pair kernel at the default workload and tuning, after contact, from two
 # rocprofv3 --pmc passes over this bench (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000: the factors the
 # 1 GiB device copy of --calibrate reads).  None where not measured.
-MEASURED_TRAFFIC_BYTES = {"pairs_mixed": (2 * 237493 + 97504) * 1024.0,
-                          "pairs_fp64_finalize_correct": (2 * 631505 + 307617) * 1024.0}
+MEASURED_TRAFFIC_BYTES = {"pairs_mixed_finalize_correct": (2 * 420926 + 312715) * 1024.0,
+                          "pairs_fp64_finalize_correct": (2 * 557645 + 315804) * 1024.0}
 # Algorithmic HBM bytes of one pair-kernel launch per particle: each particle's own neighbour record
 # (64 B mixed, 128 B FP64), its order, count and group entries and the 92 B it writes (acceleration,
 # du, divergence, velocity gradient: 11 doubles, and the neighbour count); plus 4 B per listed neighbour.
-# The fused FP64 kernel adds the finalize + corrector + criteria pass (about 60 doubles read and 25
-# written per particle), less the 92 B it no longer writes for that pass to read back.
+# The fused kernels (the PC step's default) add the finalize + corrector + criteria pass, which takes the
+# pair sums in registers: 32 doubles read (state, predictions, damage and EoS inputs), 23 written, and
+# the material id, 444 B.
 PAIR_BYTES_PER_PARTICLE = {"pairs_mixed": 168.0, "pairs_fp64": 232.0,
-                           "pairs_mixed_finalize_correct": 168.0 + 680.0 - 92.0,
-                           "pairs_fp64_finalize_correct": 232.0 + 680.0 - 92.0}
+                           "pairs_mixed_finalize_correct": 168.0 + 444.0,
+                           "pairs_fp64_finalize_correct": 232.0 + 444.0}
 PAIR_BYTES_PER_NEIGHBOR = 4.0
-# VALU wave-instructions per dispatch of the pair kernels at the 1M impact after contact (rocprofv3 --pmc
-# SQ_INSTS_VALU, averaged over 40 dispatches), per 64 pair evaluations (the gather form evaluates each
-# unordered pair from both sides): the issue-rate roof counts instructions, not flops.  A CU issues one
+# VALU wave-instructions per dispatch of the (fused) pair kernels at the 1M impact after contact
+# (rocprofv3 --pmc SQ_INSTS_VALU, averaged over 40 dispatches; the finalize + corrector + criteria pass
+# included), per 64 pair evaluations (the gather form evaluates each unordered pair from both sides):
+# the issue-rate roof counts instructions, not flops.  A CU issues one
 # wave64 VALU instruction per cycle (four SIMDs, four cycles each), so the chip's roof is
 # 256 CUs x 2.4 GHz = 6.14e11 wave-instructions/s (x 64 lanes x 2 = the 78.6 TFLOP/s FMA peak).
-VALU_INSTR_PER_64_PAIR_EVALS = {"pairs_mixed": 1.57e8 / (2 * 35218390 / 64),
-                                "pairs_fp64_finalize_correct": 1.96e8 / (2 * 35218390 / 64)}
+VALU_INSTR_PER_64_PAIR_EVALS = {"pairs_mixed_finalize_correct": 1.833e8 / (2 * 35218390 / 64),
+                                "pairs_fp64_finalize_correct": 1.970e8 / (2 * 35218390 / 64)}
 VALU_ISSUE_PEAK = 256 * 2.4e9
 # The same for the Barnes-Hut evaluation of the 8,000,000-particle N-body sphere (FETCH_SIZE x 2 +
 # WRITE_SIZE, KiB per dispatch, averaged over 5 dispatches).
