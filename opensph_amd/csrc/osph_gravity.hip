@@ -1023,11 +1023,10 @@ __device__ __forceinline__ void add_multipole_f(float M, const float* q2, const 
     a[2] += w * (M * u2 + w * hz);
 }
 
-// add_pair<false> in FP32 on scaled coordinates (r = source - target, h and m scaled).  The LUT index is
-// clamped: an FP32 q^2 may round up to 4 where the FP64 test admitted it.
-__device__ __forceinline__ void add_pair_f(const double* __restrict__ grads, float rx, float ry, float rz, float h,
-                                           float m, float a[3]) {
-    const float d2 = rx * rx + ry * ry + rz * rz;
+// add_pair<false> in FP32 on scaled coordinates (r = source - target with d2 = |r|^2, h and m scaled).
+// The LUT index is clamped: an FP32 q^2 may round up to 4 where the FP64 test admitted it.
+__device__ __forceinline__ void add_pair_f(const double* __restrict__ grads, float rx, float ry, float rz, float d2,
+                                           float h, float m, float a[3]) {
     if (grads && d2 < 4.f * (h * h)) {
         const float hInv = __builtin_amdgcn_rcpf(h);
         const float qSqr = d2 * (hInv * hInv);
@@ -1182,18 +1181,23 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mixed(Tree t, Planes s, const ui
             }
             const float close = fmaxf(wave_max_f(o2), me2) * (1.f / 64.f);  // (O / 8)^2
             float b[3] = { 0.f, 0.f, 0.f };
+            // the target itself lies within the close radius (or at zero separation when the radius is zero),
+            // so its entry is recognised there, where the source index is read anyway
             for (uint32_t j = j0; j < cnt; j += S) {
-                const uint32_t src = S_.sidx[j];
-                if (src == k) continue;
                 const float4 o = S_.src[j];
                 float rx = o.x - mx, ry = o.y - my, rz = o.z - mz;
-                if (rx * rx + ry * ry + rz * rz < close) {
+                float d2 = rx * rx + ry * ry + rz * rz;
+                bool self = false;
+                if (d2 < close || d2 == 0.f) {
+                    const uint32_t src = S_.sidx[j];
+                    self = src == k;
                     const double4 od = t.pos[src], me = t.pos[k];
                     rx = float((od.x - me.x) * sc.invL);
                     ry = float((od.y - me.y) * sc.invL);
                     rz = float((od.z - me.z) * sc.invL);
+                    d2 = rx * rx + ry * ry + rz * rz;
                 }
-                add_pair_f(grads, rx, ry, rz, 0.5f * (o.w + mh), S_.sm[j], b);
+                if (!self) add_pair_f(grads, rx, ry, rz, d2, 0.5f * (o.w + mh), S_.sm[j], b);
             }
             for (int c = 0; c < 3; ++c) a[c] += double(b[c]);
             wave_sync();
