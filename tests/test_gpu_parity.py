@@ -290,6 +290,59 @@ def test_full_size_impact_pc_steps_match_oracle(oracle):
     g.close()
 
 
+@pytest.mark.parametrize("fuse,gravity", [(1, 0), (0, 0), (1, 1)])
+def test_pc_steps_through_list_overflow_match_oracle(oracle, fuse, gravity):
+    """PC steps whose neighbour lists overflow inside the step (more neighbours than the lists hold): the
+    step grows the lists and repeats its second half from the predicted state, with the self-gravity
+    copied in again when it is attached.  Overflows on the first step (capacity 96, ~150 neighbours) and
+    again after a denser state is uploaded into the same context; every step matches the oracle."""
+    import dataclasses
+
+    st, cfg = initial.fluid_sphere(n_requested=3000, radius=1000.0, sigma_v=10.0)
+    st["r"][:, 3] *= 1.7
+    cfg = dataclasses.replace(cfg, gravity=gravity)
+    _print_case(f"list overflow in the step fuse={fuse} gravity={gravity}", st)
+    g = gpu.GpuSolver(st, cfg)
+    g.tune(fuse_finalize=fuse)
+    for k, s in enumerate((st, None)):
+        if s is None:
+            s = st.copy()
+            s["r"][:, 3] *= 1.4  # ~2.7x the neighbours: the grown lists overflow again
+            g.upload(s)
+            g.dt = cfg.initial_dt
+        glog = [g.step() for _ in range(3)]
+        so, olog = oracle.run_pc(s, cfg, 3)
+        for (dt, nxt, crit), (odt, onxt, ocrit) in zip(glog, olog):
+            assert nxt == pytest.approx(onxt, rel=TOL), k
+            assert crit == ocrit, k
+        out = g.download(s)
+        _assert_close(out, so, STATE)
+        assert np.array_equal(out["ncnt"], so["ncnt"])
+        assert out["ncnt"].max() > (96 if k == 0 else 200)
+    g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_list_overflow_inside_the_step_is_bit_exact(precision):
+    """A PC step whose lists overflow and are grown inside the step gives the same bits as the same step
+    on a context whose lists were grown beforehand (by a neighbour search)."""
+    st, cfg = initial.fluid_sphere(n_requested=3000, radius=1000.0, sigma_v=10.0)
+    st["r"][:, 3] *= 1.7
+    _print_case(f"list overflow bit-exact {precision}", st)
+    a = gpu.GpuSolver(st, cfg, precision=precision)
+    b = gpu.GpuSolver(st, cfg, precision=precision)
+    b.pairs()  # grows b's lists before its first step
+    la = [a.step() for _ in range(3)]
+    lb = [b.step() for _ in range(3)]
+    assert la == lb
+    da, db = a.download(st), b.download(st)
+    assert da["ncnt"].max() > 96
+    for k in STATE + ("ncnt", "p", "cs"):
+        assert np.array_equal(da[k], db[k]), k
+    a.close()
+    b.close()
+
+
 @pytest.mark.slow
 def test_full_size_impact_steps():
     """The unperturbed 1M impact: neighbour statistics of the reference probe (29 / 66.3 / 68), first
