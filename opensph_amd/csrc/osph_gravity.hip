@@ -327,15 +327,82 @@ __device__ void combine(const Tree& t, uint32_t node) {
     t.mom[node] = out;
 }
 
-// Depth of every internal node (the number of its ancestors) and the node count per depth.  The
-// moments are then built one depth per launch, deepest first: the children of a node are complete
-// when the launch of its depth starts, with no fences between work-groups.
+// Moments of a leaf (an internal node of at most leaf_size particles whose parent is larger, or the
+// root when it is that small) from its particles, as BarnesHut::buildLeaf computes them
+// (BarnesHut.cpp:401-437; computeMultipole, Moments.h:167-180): mass, box, centre of mass, r_open, and
+// the traceless moments about the centre summed particle by particle.  The centre is accumulated relative to the
+// first particle, so that a leaf of coincident particles has its centre on them exactly (and r_open 0).
+// The internal nodes inside a leaf are never read by the walks or the evaluations and get no moments.
+// One thread per entry of the leaf list (k_mark_leaves, compacted in Morton order); single-particle
+// leaves need nothing.
+__global__ void __launch_bounds__(BLOCK) k_leaf_moments(Tree t, const uint32_t* __restrict__ leaves,
+                                                        const uint32_t* __restrict__ nleaves) {
+    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= *nleaves) return;
+    const uint32_t i = leaves[q];
+    if (is_particle(t, i)) return;
+    const uint32_t cnt = t.count[i];
+    const uint32_t f = t.first[i];
+    const double4 p0 = t.pos[f];
+    double lo[3] = { p0.x, p0.y, p0.z }, hi[3] = { p0.x, p0.y, p0.z };
+    double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    for (uint32_t j = f; j < f + cnt; ++j) {
+        const double4 p = t.pos[j];
+        const double m = t.gm[j];
+        lo[0] = dmin(lo[0], p.x);
+        lo[1] = dmin(lo[1], p.y);
+        lo[2] = dmin(lo[2], p.z);
+        hi[0] = dmax(hi[0], p.x);
+        hi[1] = dmax(hi[1], p.y);
+        hi[2] = dmax(hi[2], p.z);
+        M += m;
+        sx += m * (p.x - p0.x);
+        sy += m * (p.y - p0.y);
+        sz += m * (p.z - p0.z);
+    }
+    NodeMom out{};
+    double c[3] = { 0.0, 0.0, 0.0 }, r_open = 0.0;
+    if (M != 0.0) {
+        c[0] = p0.x + sx / M;
+        c[1] = p0.y + sy / M;
+        c[2] = p0.z + sz / M;
+        double rm[3];
+        for (int q = 0; q < 3; ++q) rm[q] = dmax(c[q] - lo[q], hi[q] - c[q]);
+        r_open = 2.0 / sqrt(3.0) * t.thetaInv * sqrt(rm[0] * rm[0] + rm[1] * rm[1] + rm[2] * rm[2]);
+        out.M = M;
+        Moments pm{};
+        for (uint32_t j = f; j < f + cnt; ++j) {
+            const double4 p = t.pos[j];
+            pm.M = t.gm[j];
+            const double d[3] = { p.x - c[0], p.y - c[1], p.z - c[2] };
+            add_shifted(pm, d, out.q2, out.q3);  // a point mass: only the shift terms
+        }
+    }
+    t.geo[i] = make_double4(c[0], c[1], c[2], r_open);
+    t.wn[i].geo = make_double4(c[0], c[1], c[2], r_open);
+    t.lo[i] = make_double4(lo[0], lo[1], lo[2], 0.0);
+    t.hi[i] = make_double4(hi[0], hi[1], hi[2], 0.0);
+    t.mom[i] = out;
+}
+
+// Depth of every internal node above the leaves (more than leaf_size particles; the number of its
+// ancestors, all of them such nodes too) and the node count per depth; the other nodes get NO_DEPTH.
+// Their moments are then built one depth per launch, deepest first, from the leaves' moments: the
+// children of a node are complete when the launch of its depth starts, with no fences between
+// work-groups.  A bounded grid strides over the nodes, so that few blocks add their per-depth counts
+// to the ~100 global counters.
+constexpr uint32_t NO_DEPTH = 0xffffffffu;
+constexpr unsigned LEVEL_BLOCKS = 1024;
 __global__ void __launch_bounds__(BLOCK) k_depth(Tree t, uint32_t* __restrict__ depth, uint32_t* __restrict__ per_level) {
     __shared__ uint32_t hist[MAX_DEPTH + 1];
     for (int d = threadIdx.x; d <= MAX_DEPTH; d += blockDim.x) hist[d] = 0u;
     __syncthreads();
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i + 1 < t.n) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i + 1 < t.n; i += stride) {
+        if (t.count[i] <= t.leaf) {
+            depth[i] = NO_DEPTH;
+            continue;
+        }
         uint32_t c = uint32_t(i), d = 0;
         while (c != 0u && d < uint32_t(MAX_DEPTH)) {
             c = t.parent[c];
@@ -354,24 +421,32 @@ __global__ void __launch_bounds__(BLOCK) k_depth(Tree t, uint32_t* __restrict__ 
     }
 }
 
-// Scatter of the internal nodes into per-depth buckets: positions within the block from LDS counters,
-// one global reservation per depth and block.
+// Scatter of the internal nodes above the leaves into per-depth buckets.  Each block of a bounded grid
+// counts its strided share of the nodes per depth in LDS, reserves one range per depth, then places
+// its nodes from LDS cursors (the order inside a bucket is free: each node's moments depend only on
+// its children).
 __global__ void __launch_bounds__(BLOCK) k_bucket(uint32_t n_internal, const uint32_t* __restrict__ depth,
                                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ nodes) {
     __shared__ uint32_t cnt[MAX_DEPTH + 1];
     __shared__ uint32_t base[MAX_DEPTH + 1];
     for (int d = threadIdx.x; d <= MAX_DEPTH; d += blockDim.x) cnt[d] = 0u;
     __syncthreads();
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool active = i < n_internal;
-    const uint32_t d = active ? depth[i] : 0u;
-    const uint32_t local = active ? atomicAdd(&cnt[d], 1u) : 0u;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    const size_t i0 = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t i = i0; i < n_internal; i += stride) {
+        const uint32_t d = depth[i];
+        if (d != NO_DEPTH) atomicAdd(&cnt[d], 1u);
+    }
     __syncthreads();
     for (int k = threadIdx.x; k <= MAX_DEPTH; k += blockDim.x) {
         base[k] = cnt[k] ? atomicAdd(&cursor[k], cnt[k]) : 0u;
+        cnt[k] = 0u;
     }
     __syncthreads();
-    if (active) nodes[base[d] + local] = uint32_t(i);
+    for (size_t i = i0; i < n_internal; i += stride) {
+        const uint32_t d = depth[i];
+        if (d != NO_DEPTH) nodes[base[d] + atomicAdd(&cnt[d], 1u)] = uint32_t(i);
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_moments_level(Tree t, const uint32_t* __restrict__ nodes, uint32_t count) {
@@ -1729,8 +1804,10 @@ Tree tree_of(osph_gravity* g) {
 
 double decode(unsigned long long e) { return decode_ord_host(e); }
 
-// Node moments, deepest level first (k_depth, k_bucket, k_moments_level).  The flag array holds the
-// depths and leaf_at the nodes bucketed by depth; both are free until the leaves are marked.
+// Node moments: the leaves' from their particles (k_leaf_moments over the leaf list, which must be
+// selected into g->targets first), then the nodes above them, deepest level first (k_depth, k_bucket,
+// k_moments_level).  The flag array holds the depths and leaf_at the nodes bucketed by depth; leaf_at
+// is free once the leaf list is selected.
 int build_moments(osph_gravity* g, const Tree& t) {
     const size_t n = g->n;
     hipStream_t st = g->stream;
@@ -1738,7 +1815,7 @@ int build_moments(osph_gravity* g, const Tree& t) {
     uint32_t* cursor = g->levels + MAX_DEPTH + 1;
     timed(g, "depth", [&] {
         k_fill_u32g<<<1, BLOCK, 0, st>>>(per_level, MAX_DEPTH + 1, 0u);
-        k_depth<<<grid_for(n - 1), BLOCK, 0, st>>>(t, g->flag, per_level);
+        k_depth<<<std::min<unsigned>(grid_for(n - 1), LEVEL_BLOCKS), BLOCK, 0, st>>>(t, g->flag, per_level);
     });
     std::vector<uint32_t> counts(MAX_DEPTH + 1);
     GCHK(g, hipMemcpyAsync(counts.data(), per_level, counts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -1751,7 +1828,9 @@ int build_moments(osph_gravity* g, const Tree& t) {
     }
     GCHK(g, hipMemcpyAsync(cursor, off.data(), (MAX_DEPTH + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     timed(g, "moments", [&] {
-        k_bucket<<<grid_for(n - 1), BLOCK, 0, st>>>(uint32_t(n - 1), g->flag, cursor, g->leaf_at);
+        k_leaf_moments<<<grid_for(n), BLOCK, 0, st>>>(t, g->targets, g->ntargets_dev);
+        k_bucket<<<std::min<unsigned>(grid_for(n - 1), LEVEL_BLOCKS), BLOCK, 0, st>>>(uint32_t(n - 1), g->flag, cursor,
+                                                                                   g->leaf_at);
         for (int d = deepest; d >= 0; --d) {
             if (counts[d]) {
                 k_moments_level<<<grid_for(counts[d]), BLOCK, 0, st>>>(t, g->leaf_at + off[d], counts[d]);
@@ -1802,7 +1881,6 @@ int build_tree(osph_gravity* g) {
     if (n > 1) {
         timed(g, "radix_tree", [&] { k_radix_tree<<<grid_for(n - 1), BLOCK, 0, st>>>(t, g->keys_sorted); });
         timed(g, "skip", [&] { k_skip<<<grid_for(2 * n - 1), BLOCK, 0, st>>>(t); });
-        if (int e = build_moments(g, t)) return e;
     } else {
         const uint32_t endv = END;
         GCHK(g, hipMemcpyAsync(g->parent, &endv, sizeof(uint32_t), hipMemcpyHostToDevice, st));
@@ -1814,6 +1892,9 @@ int build_tree(osph_gravity* g) {
         (void)hipcub::DeviceSelect::If(g->select_temp, g->select_temp_bytes, g->leaf_at, g->targets, g->ntargets_dev,
                                        int(n), NotEnd(), st);
     });
+    if (n > 1) {
+        if (int e = build_moments(g, t)) return e;
+    }
     uint32_t nt = 0;
     GCHK(g, hipMemcpyAsync(g->small_host + 7, g->ntargets_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (int e = check_traversal(g)) return e;
