@@ -210,11 +210,13 @@ __global__ void __launch_bounds__(BLOCK) k_radix_tree(Tree t, const uint64_t* __
 }
 
 // Depth-first successor of each node: the right sibling of the nearest ancestor-or-self that is a left
-// child, or END.
+// child, or END.  Only for the nodes a walk can visit: the root and the children of nodes of more than
+// leaf_size particles (the nodes inside a leaf are never visited).
 __global__ void __launch_bounds__(BLOCK) k_skip(Tree t) {
     const size_t id = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t total = 2 * size_t(t.n) - 1;
     if (id >= total) return;
+    if (id != 0 && t.count[t.parent[id]] <= t.leaf) return;
     uint32_t c = uint32_t(id), next = END;
     for (int level = 0; c != 0u && level < MAX_DEPTH; ++level) {
         const uint32_t p = t.parent[c];
