@@ -78,7 +78,11 @@ int osph_gravity_set_precision(osph_gravity* g, int precision);
 /* Launch tuning (no reference counterpart).  Key "walk_screen": 1 takes the Barnes-Hut walk's opening
  * tests in FP32 on 32-byte node records, with the FP64 test wherever the FP32 margin is within its error
  * bound, so the interaction lists are those of the FP64 walk, entry for entry; 0 walks in FP64 only; -1
- * (the default) screens in mixed precision only. */
+ * (the default) screens in mixed precision only.  Key "reorder_period": the LeapFrog steps
+ * (osph_gravity_step_leapfrog) move the resident state into the tree's Morton order on their first
+ * tree build after osph_gravity_set_particles and then every that many builds (default 10; 0 never).
+ * Every call still takes and returns particles in the caller's order; sums over all particles
+ * (energy, integrals, the direct sum's sources) then run in the resident order. */
 int osph_gravity_tune(osph_gravity* g, const char* key, long long value);
 
 /* Particles in the reference's element layouts: r and v are Vectors (x, y, z, h; stride bytes, 0 = 32),

@@ -155,6 +155,34 @@ __global__ void __launch_bounds__(BLOCK) k_morton(Planes s, double lo0, double l
     idx[i] = uint32_t(i);
 }
 
+// The state moved into the slots of the Morton order: slot k of dst takes every plane of slot order[k]
+// of src, and the particle's index in the caller's order (perm_in of that slot, or the slot itself while
+// the state is in the caller's order) goes with it, into perm_out and its inverse slot_of.
+__global__ void __launch_bounds__(BLOCK) k_permute_planes(Planes src, Planes dst, const uint32_t* __restrict__ order,
+                                                          const uint32_t* __restrict__ perm_in,
+                                                          uint32_t* __restrict__ perm_out,
+                                                          uint32_t* __restrict__ slot_of, size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = order[k];
+    for (int p = 0; p < NGPLANES; ++p) dst.d[p][k] = src.d[p][i];
+    const uint32_t u = perm_in ? perm_in[i] : i;
+    perm_out[k] = u;
+    slot_of[u] = uint32_t(k);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sequence(uint32_t* __restrict__ p, size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = uint32_t(k);
+}
+
+// Particle indices in the caller's order to the slots that hold them.
+__global__ void __launch_bounds__(BLOCK) k_to_slots(uint32_t* __restrict__ idx, const uint32_t* __restrict__ slot_of,
+                                                    size_t n) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) idx[k] = slot_of[idx[k]];
+}
+
 __global__ void __launch_bounds__(BLOCK) k_gather(Planes s, const uint32_t* __restrict__ order, double G,
                                                   double4* __restrict__ pos, double* __restrict__ gm, size_t n) {
     const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1511,19 +1539,21 @@ __global__ void k_init_bounds(unsigned long long* a) {
 }
 
 // Strided element layouts <-> planes.
+// Host rows (the caller's order) into the planes and back; perm (slot -> caller's index) is null while
+// the state is in the caller's order.
 __global__ void __launch_bounds__(BLOCK) k_unstage(const unsigned char* __restrict__ st, size_t stride, int ncomp,
-                                                   Planes s, int plane0, size_t n) {
+                                                   Planes s, int plane0, const uint32_t* __restrict__ perm, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double* e = reinterpret_cast<const double*>(st + i * stride);
+    const double* e = reinterpret_cast<const double*>(st + size_t(perm ? perm[i] : i) * stride);
     for (int c = 0; c < ncomp; ++c) s.d[plane0 + c][i] = e[c];
 }
 
 __global__ void __launch_bounds__(BLOCK) k_stage(unsigned char* __restrict__ st, size_t stride, int ncomp, Planes s,
-                                                 int plane0, size_t n) {
+                                                 int plane0, const uint32_t* __restrict__ perm, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double* e = reinterpret_cast<double*>(st + i * stride);
+    double* e = reinterpret_cast<double*>(st + size_t(perm ? perm[i] : i) * stride);
     for (int c = 0; c < ncomp; ++c) e[c] = s.d[plane0 + c][i];
 }
 
@@ -1563,6 +1593,14 @@ struct osph_gravity {
     size_t n = 0, cap = 0;
     double* dblock = nullptr;
     Planes s{};
+    // LeapFrog steps move the state into the Morton order of the tree every reorder_period builds, so
+    // that the tree's gather and the accelerations' writes run over consecutive slots.  perm maps a
+    // slot to the particle's index in the caller's order and slot_of back; identity: no move since the
+    // particles were set.
+    double* dblock2 = nullptr;
+    uint32_t *perm = nullptr, *perm2 = nullptr, *slot_of = nullptr;
+    bool identity = true;
+    int reorder_period = 10, since_reorder = 0;
     // tree
     uint64_t *keys = nullptr, *keys_sorted = nullptr;
     uint32_t *iota = nullptr, *order = nullptr;
@@ -1677,11 +1715,13 @@ int check_traversal(osph_gravity* g) {
 }
 
 void free_particles(osph_gravity* g) {
-    void* ptrs[] = { g->dblock, g->keys, g->keys_sorted, g->iota, g->order, g->sort_temp, g->pos, g->gm, g->left,
+    void* ptrs[] = { g->dblock, g->dblock2, g->perm, g->perm2, g->slot_of, g->keys, g->keys_sorted, g->iota, g->order, g->sort_temp, g->pos, g->gm, g->left,
                      g->right, g->parent, g->skip, g->first, g->count, g->flag, g->geo, g->lo, g->hi, g->mom, g->wn,
                      g->wnf, g->leaf_at, g->targets, g->ntargets_dev, g->select_temp };
     for (void* p : ptrs) (void)hipFree(p);
-    g->dblock = nullptr;
+    g->dblock = g->dblock2 = nullptr;
+    g->perm = g->perm2 = g->slot_of = nullptr;
+    g->identity = true;
     g->keys = g->keys_sorted = nullptr;
     g->iota = g->order = nullptr;
     g->sort_temp = g->select_temp = nullptr;
@@ -1703,6 +1743,10 @@ int ensure_particles(osph_gravity* g, size_t n) {
     free_particles(g);
     const size_t c = n, nodes = c > 1 ? c - 1 : 1, ids = 2 * c;
     GCHK(g, hipMalloc(&g->dblock, sizeof(double) * NGPLANES * c));
+    GCHK(g, hipMalloc(&g->dblock2, sizeof(double) * NGPLANES * c));
+    GCHK(g, hipMalloc(&g->perm, sizeof(uint32_t) * c));
+    GCHK(g, hipMalloc(&g->perm2, sizeof(uint32_t) * c));
+    GCHK(g, hipMalloc(&g->slot_of, sizeof(uint32_t) * c));
     GCHK(g, hipMalloc(&g->keys, sizeof(uint64_t) * c));
     GCHK(g, hipMalloc(&g->keys_sorted, sizeof(uint64_t) * c));
     GCHK(g, hipMalloc(&g->iota, sizeof(uint32_t) * c));
@@ -1842,7 +1886,8 @@ int build_moments(osph_gravity* g, const Tree& t) {
     return OK;
 }
 
-int build_tree(osph_gravity* g) {
+// With reorder (LeapFrog steps only), the state is moved into the Morton order when due.
+int build_tree(osph_gravity* g, bool reorder = false) {
     const size_t n = g->n;
     g->built = false;
     if (n == 0) {
@@ -1879,6 +1924,20 @@ int build_tree(osph_gravity* g) {
         (void)hipcub::DeviceRadixSort::SortPairs(g->sort_temp, g->sort_temp_bytes, g->keys, g->keys_sorted, g->iota,
                                                  g->order, int(n), 0, 63, st);
     });
+    if (reorder && g->reorder_period > 0 && ++g->since_reorder >= g->reorder_period) {
+        g->since_reorder = 0;
+        Planes dst{};
+        for (int k = 0; k < NGPLANES; ++k) dst.d[k] = g->dblock2 + size_t(k) * g->cap;
+        timed(g, "reorder", [&] {
+            k_permute_planes<<<grid_for(n), BLOCK, 0, st>>>(g->s, dst, g->order, g->identity ? nullptr : g->perm,
+                                                            g->perm2, g->slot_of, n);
+            k_sequence<<<grid_for(n), BLOCK, 0, st>>>(g->order, n);
+        });
+        std::swap(g->dblock, g->dblock2);
+        std::swap(g->perm, g->perm2);
+        g->s = dst;
+        g->identity = false;
+    }
     timed(g, "gather", [&] { k_gather<<<grid_for(n), BLOCK, 0, st>>>(g->s, g->order, g->cfg.G, g->pos, g->gm, n); });
     if (n > 1) {
         timed(g, "radix_tree", [&] { k_radix_tree<<<grid_for(n - 1), BLOCK, 0, st>>>(t, g->keys_sorted); });
@@ -2044,7 +2103,8 @@ int upload_vectors(osph_gravity* g, const double* host, size_t stride, int plane
     if (!n) return OK;
     if (int e = ensure_staging(g, n * stride)) return e;
     GCHK(g, hipMemcpyAsync(g->staging, host, n * stride, hipMemcpyHostToDevice, g->stream));
-    k_unstage<<<grid_for(n), BLOCK, 0, g->stream>>>(g->staging, stride, ncomp, g->s, plane0, n);
+    k_unstage<<<grid_for(n), BLOCK, 0, g->stream>>>(g->staging, stride, ncomp, g->s, plane0,
+                                                    g->identity ? nullptr : g->perm, n);
     return OK;
 }
 
@@ -2055,7 +2115,8 @@ int download_vectors(osph_gravity* g, double* host, size_t stride, int plane0, i
         // keep the caller's other lanes: start from their current values
         GCHK(g, hipMemcpyAsync(g->staging, host, n * stride, hipMemcpyHostToDevice, g->stream));
     }
-    k_stage<<<grid_for(n), BLOCK, 0, g->stream>>>(g->staging, stride, ncomp, g->s, plane0, n);
+    k_stage<<<grid_for(n), BLOCK, 0, g->stream>>>(g->staging, stride, ncomp, g->s, plane0,
+                                                  g->identity ? nullptr : g->perm, n);
     GCHK(g, hipMemcpyAsync(host, g->staging, n * stride, hipMemcpyDeviceToHost, g->stream));
     return sync(g);
 }
@@ -2097,6 +2158,7 @@ int gravity_self_from_device(osph_gravity* g, size_t n, const double* const src[
     if (int e = ensure_particles(g, n > 0 ? n : 1)) return e;
     g->n = n;
     g->built = false;
+    g->identity = true;  // the planes are loaded in the source order and stay in it
     for (int k = 0; k < 3; ++k) accel[k] = g->s.d[GAX + k];
     GCHK(g, hipStreamWaitEvent(g->stream, ready, 0));
     if (n) {
@@ -2180,6 +2242,10 @@ int osph_gravity_tune(osph_gravity* g, const char* key, long long value) {
         g->walk_screen = int(value);
         return OK;
     }
+    if (k == "reorder_period" && value >= 0 && value <= 1000000) {
+        g->reorder_period = int(value);
+        return OK;
+    }
     return g->fail(ERR_ARG, "unknown tuning key or value out of range: " + k);
 }
 
@@ -2192,6 +2258,8 @@ int osph_gravity_set_particles(osph_gravity* g, size_t n, const double* r, const
     if (int e = ensure_particles(g, n > 0 ? n : 1)) return e;
     g->n = n;
     g->built = false;
+    g->identity = true;
+    g->since_reorder = g->reorder_period;  // the first LeapFrog build moves the state into Morton order
     if (!n) return sync(g);
     if (int e = upload_vectors(g, r, stride, GX, 4, n)) return e;
     if (v) {
@@ -2291,6 +2359,7 @@ int osph_gravity_direct(osph_gravity* g, const uint32_t* t, size_t nt, double* a
     double4* out = reinterpret_cast<double4*>(g->scratch);
     uint32_t* tg = reinterpret_cast<uint32_t*>(g->scratch + 4 * nt);
     GCHK(g, hipMemcpyAsync(tg, t, nt * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+    if (!g->identity) k_to_slots<<<grid_for(nt), BLOCK, 0, g->stream>>>(tg, g->slot_of, nt);
     if (int e = eval_direct(g, tg, nt, out)) return e;
     std::vector<double4> ha(nt);
     GCHK(g, hipMemcpyAsync(ha.data(), out, nt * sizeof(double4), hipMemcpyDeviceToHost, g->stream));
@@ -2364,7 +2433,7 @@ int osph_gravity_step_leapfrog(osph_gravity* g, double dt, int steps) {
         timed(g, "drift", [&] { k_drift<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, dt, n); });
         k_zero_acc<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, n);
         if (g->cfg.solver == OSPH_GRAVITY_BARNES_HUT) {
-            if (int e = build_tree(g)) return e;
+            if (int e = build_tree(g, true)) return e;
         }
         if (int e = eval_self(g, nullptr)) return e;
         timed(g, "kick_drift", [&] { k_kick_drift<<<grid_for(n), BLOCK, 0, g->stream>>>(g->s, dt, n); });

@@ -111,7 +111,7 @@ class GpuGravity:
             pass
 
     def tune(self, **kw):
-        """Launch tuning (see osph_gravity_tune): walk_screen = -1, 0 or 1."""
+        """Launch tuning (see osph_gravity_tune): walk_screen = -1, 0 or 1; reorder_period >= 0."""
         for k, v in kw.items():
             self._check(self.L.osph_gravity_tune(self.ctx, k.encode(), int(v)), f"osph_gravity_tune({k})")
 
