@@ -59,8 +59,8 @@ VALU_INSTR_PER_64_PAIR_EVALS = {"pairs_mixed_finalize_correct": 1.833e8 / (2 * 3
                                 "pairs_fp64_finalize_correct": 1.970e8 / (2 * 35218390 / 64)}
 VALU_ISSUE_PEAK = 256 * 2.4e9
 # The same for the Barnes-Hut evaluation of the 8,000,000-particle N-body sphere (FETCH_SIZE x 2 +
-# WRITE_SIZE, KiB per dispatch, averaged over 5 dispatches).
-NBODY_TRAFFIC_BYTES = {"eval_mixed": (2 * 3746826 + 751902) * 1024.0, "eval": (2 * 3631237 + 751959) * 1024.0}
+# WRITE_SIZE, KiB per dispatch, averaged over the 2 dispatches with the state in Morton order).
+NBODY_TRAFFIC_BYTES = {"eval_mixed": (2 * 2045976 + 245187) * 1024.0, "eval": (2 * 1957327 + 248606) * 1024.0}
 
 
 def log(*a):
