@@ -363,11 +363,28 @@ __device__ void combine(const Tree& t, uint32_t node) {
 // the traceless moments about the centre summed particle by particle.  The centre is accumulated relative to the
 // first particle, so that a leaf of coincident particles has its centre on them exactly (and r_open 0).
 // The internal nodes inside a leaf are never read by the walks or the evaluations and get no moments.
-// One thread per entry of the leaf list (k_mark_leaves, compacted in Morton order); single-particle
-// leaves need nothing.
+// LEAF_LANES lanes per entry of the leaf list (k_mark_leaves, compacted in Morton order), each summing
+// every LEAF_LANES-th particle of the leaf, the partial sums then added across the group in a fixed
+// order; single-particle leaves need nothing.
+constexpr uint32_t LEAF_LANES = 8;
+__device__ __forceinline__ double leaf_sum(double v) {
+    for (int o = LEAF_LANES / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, LEAF_LANES);
+    return v;
+}
+__device__ __forceinline__ double leaf_min(double v) {
+    for (int o = LEAF_LANES / 2; o >= 1; o >>= 1) v = dmin(v, __shfl_xor(v, o, LEAF_LANES));
+    return v;
+}
+__device__ __forceinline__ double leaf_max(double v) {
+    for (int o = LEAF_LANES / 2; o >= 1; o >>= 1) v = dmax(v, __shfl_xor(v, o, LEAF_LANES));
+    return v;
+}
 __global__ void __launch_bounds__(BLOCK) k_leaf_moments(Tree t, const uint32_t* __restrict__ leaves,
                                                         const uint32_t* __restrict__ nleaves) {
-    const size_t q = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t q = gid / LEAF_LANES;
+    const uint32_t sub = uint32_t(gid % LEAF_LANES);
+    // uniform over the group of LEAF_LANES lanes, so every shuffle below reads lanes of the same leaf
     if (q >= *nleaves) return;
     const uint32_t i = leaves[q];
     if (is_particle(t, i)) return;
@@ -376,7 +393,7 @@ __global__ void __launch_bounds__(BLOCK) k_leaf_moments(Tree t, const uint32_t* 
     const double4 p0 = t.pos[f];
     double lo[3] = { p0.x, p0.y, p0.z }, hi[3] = { p0.x, p0.y, p0.z };
     double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-    for (uint32_t j = f; j < f + cnt; ++j) {
+    for (uint32_t j = f + sub; j < f + cnt; j += LEAF_LANES) {
         const double4 p = t.pos[j];
         const double m = t.gm[j];
         lo[0] = dmin(lo[0], p.x);
@@ -390,6 +407,14 @@ __global__ void __launch_bounds__(BLOCK) k_leaf_moments(Tree t, const uint32_t* 
         sy += m * (p.y - p0.y);
         sz += m * (p.z - p0.z);
     }
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = leaf_min(lo[k]);
+        hi[k] = leaf_max(hi[k]);
+    }
+    M = leaf_sum(M);
+    sx = leaf_sum(sx);
+    sy = leaf_sum(sy);
+    sz = leaf_sum(sz);
     NodeMom out{};
     double c[3] = { 0.0, 0.0, 0.0 }, r_open = 0.0;
     if (M != 0.0) {
@@ -397,17 +422,20 @@ __global__ void __launch_bounds__(BLOCK) k_leaf_moments(Tree t, const uint32_t* 
         c[1] = p0.y + sy / M;
         c[2] = p0.z + sz / M;
         double rm[3];
-        for (int q = 0; q < 3; ++q) rm[q] = dmax(c[q] - lo[q], hi[q] - c[q]);
+        for (int k = 0; k < 3; ++k) rm[k] = dmax(c[k] - lo[k], hi[k] - c[k]);
         r_open = 2.0 / sqrt(3.0) * t.thetaInv * sqrt(rm[0] * rm[0] + rm[1] * rm[1] + rm[2] * rm[2]);
         out.M = M;
         Moments pm{};
-        for (uint32_t j = f; j < f + cnt; ++j) {
+        for (uint32_t j = f + sub; j < f + cnt; j += LEAF_LANES) {
             const double4 p = t.pos[j];
             pm.M = t.gm[j];
             const double d[3] = { p.x - c[0], p.y - c[1], p.z - c[2] };
             add_shifted(pm, d, out.q2, out.q3);  // a point mass: only the shift terms
         }
+        for (int k = 0; k < 5; ++k) out.q2[k] = leaf_sum(out.q2[k]);
+        for (int k = 0; k < 7; ++k) out.q3[k] = leaf_sum(out.q3[k]);
     }
+    if (sub != 0) return;
     t.geo[i] = make_double4(c[0], c[1], c[2], r_open);
     t.wn[i].geo = make_double4(c[0], c[1], c[2], r_open);
     t.lo[i] = make_double4(lo[0], lo[1], lo[2], 0.0);
@@ -1874,7 +1902,7 @@ int build_moments(osph_gravity* g, const Tree& t) {
     }
     GCHK(g, hipMemcpyAsync(cursor, off.data(), (MAX_DEPTH + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     timed(g, "moments", [&] {
-        k_leaf_moments<<<grid_for(n), BLOCK, 0, st>>>(t, g->targets, g->ntargets_dev);
+        k_leaf_moments<<<grid_for(n * size_t(LEAF_LANES)), BLOCK, 0, st>>>(t, g->targets, g->ntargets_dev);
         k_bucket<<<std::min<unsigned>(grid_for(n - 1), LEVEL_BLOCKS), BLOCK, 0, st>>>(uint32_t(n - 1), g->flag, cursor,
                                                                                    g->leaf_at);
         for (int d = deepest; d >= 0; --d) {
