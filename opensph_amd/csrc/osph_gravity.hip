@@ -360,8 +360,9 @@ __device__ void combine(const Tree& t, uint32_t node) {
 // Moments of a leaf (an internal node of at most leaf_size particles whose parent is larger, or the
 // root when it is that small) from its particles, as BarnesHut::buildLeaf computes them
 // (BarnesHut.cpp:401-437; computeMultipole, Moments.h:167-180): mass, box, centre of mass, r_open, and
-// the traceless moments about the centre summed particle by particle.  The centre is accumulated relative to the
-// first particle, so that a leaf of coincident particles has its centre on them exactly (and r_open 0).
+// the traceless moments about the centre summed particle by particle.  The centre is accumulated
+// relative to the first particle, so that a leaf of coincident particles has its centre on them exactly
+// (and r_open 0).
 // The internal nodes inside a leaf are never read by the walks or the evaluations and get no moments.
 // LEAF_LANES lanes per entry of the leaf list (k_mark_leaves, compacted in Morton order), each summing
 // every LEAF_LANES-th particle of the leaf, the partial sums then added across the group in a fixed
