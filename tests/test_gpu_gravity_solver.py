@@ -146,3 +146,30 @@ def test_gravity_context_serves_one_solver():
         s.gravity = None
         s.close()
     g.close()
+
+
+def test_gravity_context_moved_by_leapfrog_then_attached():
+    """A gravity context whose state LeapFrog steps moved into Morton order, then attached to an SPH solver:
+    it loads the solver's particles in the solver's order, and the SPH + Barnes-Hut steps give the same
+    bits as with a fresh gravity context."""
+    from opensph_amd.gravity import GpuGravity
+
+    st, cfg = _cases()["impact_exercised"]()
+    cfg = dataclasses.replace(cfg, gravity=2, gravity_G=1.0)
+    print(f"[case] gravity moved then attached n={st.n}", flush=True)
+    fresh = gpu.GpuSolver(st, cfg)
+    plain = gpu.GpuSolver(st, dataclasses.replace(cfg, gravity=0))
+    rng = np.random.default_rng(5)
+    n = st.n + 2000  # more than the solver's particles: its planes are reused, not reallocated
+    r = np.c_[rng.uniform(-1e4, 1e4, (n, 3)), np.full(n, 50.0)]
+    g = GpuGravity(r, np.full(n, 1e10), theta=cfg.gravity_theta, order=cfg.gravity_order,
+                   leaf_size=cfg.gravity_leaf, G=cfg.gravity_G, precision="fp64")
+    g.leapfrog(1.0, 3)  # the first build moves its state into Morton order
+    plain.attach_gravity(g)
+    for _ in range(3):
+        assert plain.step() == fresh.step()
+    a, b = plain.download(st), fresh.download(st)
+    for k in STATE:
+        assert np.array_equal(a[k], b[k]), k
+    plain.close()
+    fresh.close()
